@@ -1,0 +1,292 @@
+"""Context, run configuration and results over the C ABI (include/mcint.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _abi as A
+
+
+class MIError(RuntimeError):
+    """A library call returned a negative status; ``status`` holds it."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def physics_default(which: int = 1, **overrides) -> A.MiPhysics:
+    """Workload defaults: which=0 partonic gg->gg at sHat=(100 GeV)^2, |cos|<0.9, alphaS=0.118;
+    which=1 the 5-process library at sqrt(S)=13.6 TeV, pT>20 GeV, toy PDFs, 1-loop alphaS."""
+    p = A.MiPhysics()
+    rc = A.load().mi_physics_default(which, C.byref(p))
+    if rc:
+        raise MIError(rc, f"mi_physics_default({which})")
+    for k, v in overrides.items():
+        if k in ("pdf_g", "pdf_val", "pdf_sea"):
+            getattr(p, k)[0], getattr(p, k)[1] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def make_run(phys: A.MiPhysics, n_points: int, *, precision: int = A.MI_PREC_FP32,
+             sampler: int = A.MI_SAMPLER_FLAT, rng: int = A.MI_RNG_PHILOX, key: int = 42,
+             counter0: int = 0) -> A.MiRun:
+    r = A.MiRun()
+    r.phys = phys
+    r.precision = precision
+    r.sampler = sampler
+    r.rng = rng
+    r.key = key
+    r.counter0 = counter0
+    r.n_points = n_points
+    return r
+
+
+@dataclass
+class SigmaEstimate:
+    """Pythia's sigma-estimate view of one call (Info::sigmaGen / sigmaErr; mb)."""
+    raw: A.MiSigmaEst
+
+    def _idx(self, code: int) -> int:
+        return 0 if code == 0 else code - 110
+
+    def sigmaGen(self, code: int = 0) -> float:
+        return self.raw.sigmaGen[self._idx(code)]
+
+    def sigmaErr(self, code: int = 0) -> float:
+        return self.raw.sigmaErr[self._idx(code)]
+
+    def sumW(self, code: int = 0) -> float:
+        return self.raw.sumW[self._idx(code)]
+
+    def sumW2(self, code: int = 0) -> float:
+        return self.raw.sumW2[self._idx(code)]
+
+    def wMax(self, code: int = 0) -> float:
+        return self.raw.wMax[self._idx(code)]
+
+    def nTried(self) -> int:
+        return self.raw.nTry
+
+    def nSelected(self) -> int:
+        return self.raw.nSel
+
+    def nAccepted(self) -> int:
+        return self.raw.nAcc
+
+    @property
+    def kernel_ms(self) -> float:
+        return self.raw.kernel_ms
+
+
+@dataclass
+class Events:
+    """Compacted unweighted events (device tensors, length n)."""
+    sHat: torch.Tensor
+    tHat: torch.Tensor
+    x1: torch.Tensor
+    x2: torch.Tensor
+    weight: torch.Tensor
+    pid: torch.Tensor
+    index: torch.Tensor
+    n: int
+    wmax: float
+    estimate: Optional[SigmaEstimate] = None
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device tensors required (bulk arrays cross the ABI as device pointers)")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensors required")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One mi_ctx on a HIP device; launches on torch's current stream of that device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = A.load()
+        self.device = device
+        h = C.c_void_p()
+        rc = self.lib.mi_ctx_create(device, C.byref(h))
+        if rc:
+            raise MIError(rc, f"mi_ctx_create({device})")
+        self.h = h
+        self.sync_stream()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -------------------------------------------------------------- plumbing
+    def sync_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        self.lib.mi_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream))
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            msg = self.lib.mi_last_error(self.h)
+            raise MIError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @property
+    def dev(self):
+        return torch.device("cuda", self.device)
+
+    # -------------------------------------------------------------- RNG
+    def philox_fill(self, key: int, counter0: int, n: int, stream: int = 0) -> torch.Tensor:
+        out = torch.empty(4 * n, dtype=torch.int32, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_rng_philox_fill(self.h, key, counter0, stream, n, _ptr(out)),
+                    "mi_rng_philox_fill")
+        return out
+
+    # -------------------------------------------------------------- sigmaHat
+    def sigma_hat(self, code: int, sH, tH, alpS, uH=None, id1=None, id2=None, nQuarkNew: int = 3,
+                  kfactor: float = 1.0) -> torch.Tensor:
+        n = sH.numel()
+        out = torch.empty(n, dtype=torch.float64, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_sigma_hat(self.h, code, n, _ptr(sH), _ptr(tH), _ptr(uH), _ptr(alpS),
+                                          _ptr(id1), _ptr(id2), nQuarkNew, kfactor, _ptr(out)),
+                    "mi_sigma_hat")
+        return out
+
+    # -------------------------------------------------------------- integration
+    def points(self, run: A.MiRun, with_kin: bool = True):
+        n = int(run.n_points)
+        w = torch.empty((n, A.MI_MAX_PROC + 1), dtype=torch.float64, device=self.dev)
+        kin = torch.empty((n, 4), dtype=torch.float64, device=self.dev) if with_kin else None
+        self.sync_stream()
+        self._check(self.lib.mi_points(self.h, C.byref(run), _ptr(w), _ptr(kin)), "mi_points")
+        return w, kin
+
+    def integrate(self, run: A.MiRun) -> SigmaEstimate:
+        est = A.MiSigmaEst()
+        self.sync_stream()
+        self._check(self.lib.mi_integrate(self.h, C.byref(run), C.byref(est)), "mi_integrate")
+        return SigmaEstimate(est)
+
+    # -------------------------------------------------------------- VEGAS
+    def vegas_init(self, ndim: int, nbins: int):
+        self._check(self.lib.mi_vegas_init(self.h, ndim, nbins), "mi_vegas_init")
+
+    def vegas_adapt(self, run: A.MiRun, n_iter: int, alpha: float = 1.5) -> A.MiVegasInfo:
+        info = A.MiVegasInfo()
+        self.sync_stream()
+        self._check(self.lib.mi_vegas_adapt(self.h, C.byref(run), n_iter, alpha, C.byref(info)),
+                    "mi_vegas_adapt")
+        return info
+
+    def vegas_get_grid(self):
+        import numpy as np
+        nd, nb = C.c_int(), C.c_int()
+        self._check(self.lib.mi_vegas_get_grid(self.h, None, C.byref(nd), C.byref(nb)), "mi_vegas_get_grid")
+        e = np.zeros(nd.value * (nb.value + 1), dtype=np.float64)
+        self._check(self.lib.mi_vegas_get_grid(self.h, e.ctypes.data_as(C.POINTER(C.c_double)), None, None),
+                    "mi_vegas_get_grid")
+        return e.reshape(nd.value, nb.value + 1)
+
+    def vegas_set_grid(self, edges):
+        import numpy as np
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        nd, nb1 = e.shape
+        self._check(self.lib.mi_vegas_set_grid(self.h, e.ctypes.data_as(C.POINTER(C.c_double)), nd, nb1 - 1),
+                    "mi_vegas_set_grid")
+
+    # -------------------------------------------------------------- unweighting
+    def unweight_compact(self, run: A.MiRun, wmax: float = 0.0, capacity: Optional[int] = None) -> Events:
+        n = int(run.n_points)
+        cap = n if capacity is None else int(capacity)
+        d = self.dev
+        f64 = dict(dtype=torch.float64, device=d)
+        ev = Events(torch.empty(cap, **f64), torch.empty(cap, **f64), torch.empty(cap, **f64),
+                    torch.empty(cap, **f64), torch.empty(cap, **f64),
+                    torch.empty(cap, dtype=torch.int32, device=d), torch.empty(cap, dtype=torch.int64, device=d),
+                    0, 0.0)
+        raw = A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
+                         _ptr(ev.index), cap)
+        nacc, wused = C.c_uint64(), C.c_double()
+        est = A.MiSigmaEst()
+        self.sync_stream()
+        rc = self.lib.mi_unweight_compact(self.h, C.byref(run), wmax, C.byref(raw), C.byref(nacc),
+                                          C.byref(wused), C.byref(est))
+        self._check(rc, "mi_unweight_compact")
+        k = nacc.value
+        ev = Events(ev.sHat[:k], ev.tHat[:k], ev.x1[:k], ev.x2[:k], ev.weight[:k], ev.pid[:k], ev.index[:k], k,
+                    wused.value, SigmaEstimate(est))
+        return ev
+
+    # -------------------------------------------------------------- adjoint
+    def adjoint(self, run: A.MiRun) -> A.MiSigmaGrad:
+        g = A.MiSigmaGrad()
+        self.sync_stream()
+        self._check(self.lib.mi_adjoint(self.h, C.byref(run), C.byref(g)), "mi_adjoint")
+        return g
+
+    # -------------------------------------------------------------- flow
+    def flow_init(self, seed: int = 1):
+        self._check(self.lib.mi_flow_init(self.h, seed), "mi_flow_init")
+
+    def flow_train(self, run: A.MiRun, steps: int, batch: int, lr: float = 1e-3) -> A.MiFlowInfo:
+        info = A.MiFlowInfo()
+        self.sync_stream()
+        self._check(self.lib.mi_flow_train(self.h, C.byref(run), steps, batch, lr, C.byref(info)), "mi_flow_train")
+        return info
+
+    def flow_forward(self, y: torch.Tensor):
+        n = y.shape[0]
+        z = torch.empty_like(y)
+        ld = torch.empty(n, dtype=torch.float32, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_flow_forward(self.h, n, _ptr(y), _ptr(z), _ptr(ld)), "mi_flow_forward")
+        return z, ld
+
+    def flow_inverse(self, z: torch.Tensor):
+        n = z.shape[0]
+        y = torch.empty_like(z)
+        ld = torch.empty(n, dtype=torch.float32, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_flow_inverse(self.h, n, _ptr(z), _ptr(y), _ptr(ld)), "mi_flow_inverse")
+        return y, ld
+
+    def flow_backward(self, y: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+        n = y.shape[0]
+        grad = torch.empty(A.MI_FLOW_NPARAMS, dtype=torch.float32, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_flow_backward(self.h, n, _ptr(y), _ptr(g), _ptr(grad)), "mi_flow_backward")
+        return grad
+
+    def flow_get_params(self):
+        import numpy as np
+        p = np.zeros(A.MI_FLOW_NPARAMS, dtype=np.float32)
+        self._check(self.lib.mi_flow_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
+                    "mi_flow_get_params")
+        return p
+
+    def flow_set_params(self, params):
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == A.MI_FLOW_NPARAMS
+        self._check(self.lib.mi_flow_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
+                    "mi_flow_set_params")

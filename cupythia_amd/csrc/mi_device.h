@@ -1,0 +1,341 @@
+// mi_device.h -- device-side building blocks of the MI355X matrix-element integration path:
+// Philox4x32-10 in registers, uniforms, this project's FP32 exp/log, the partonic gg->gg
+// weight (FP64) and the hadronic 3-D integrand of the QCD 2->2 library in FP32 and FP64.
+//
+// Physics (paths relative to the reference repository):
+//   sigmaHat brackets      pythia8317/src/SigmaQCD.cc:115-378 (Sigma2gg2gg ... Sigma2qqbar2qqbarNew)
+//   CONVERT2MB             pythia8317/src/SigmaProcess.cc:24
+//   parton flux + PDFs     pythia8317/src/SigmaProcess.cc:124-193 (initFlux), :423-480 (sigmaPDF)
+//   (tau, y, z) map        pythia8317/src/PhaseSpace.cc:1256-1337 (selectTau, iTau = 0),
+//                          :1373-1444 (y range), :1453-1568 (selectZ, stable tHat/uHat)
+//   scales, alphaS         pythia8317/src/SigmaProcess.cc:876-922, pythia8317/src/StandardModel.cc:188-215
+//   fused flat gg->gg      cuPythia/03_fused_resident_me/fused_gg2gg.cu:40-51
+//
+// The hadronic weight is written in terms of a = -t/s = (1 - z)/2 and b = -u/s = (1 + z)/2, both
+// formed without cancellation from the uniforms, so FP32 holds ~1e-6 relative accuracy up to
+// the pT cut.  With a, b, and L_x = sum over the flux list of x1 f(x1) x2 f(x2):
+//   w_p = CONV pi LT / (4 pTmin^2) * alphaS^2 * (2 yMax) * zMax * (tauMin/tau) * L_p * B_p(a, b)
+// which equals Pythia's sigmaPDF * wtTau * wtY * wtZ for tau = tauMin (1/tauMin)^u0,
+// y = yMax (2 u1 - 1), z = zMax (2 u2 - 1).
+//
+// FP32 arithmetic here is bit-reproducible on the host: only correctly rounded IEEE operations,
+// explicit fmaf, no contraction (the library is compiled with -ffp-contract=off), and the own
+// exp/log below (Cody-Waite reduction + Taylor polynomial; atanh series).  That is what makes
+// the unweighting decisions, and hence the compacted event indices, bit-exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mi {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kConvert2mb = 0.389380;   // Pythia's GeV^-2 -> mb
+constexpr int kMaxProc = 6;
+
+// ------------------------------------------------------------------ Philox4x32-10
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// Event e of stream s under a 64-bit key: counter (e_lo, e_hi, s, 0), key (key_lo, key_hi).
+__device__ __forceinline__ U4 philox_event(uint64_t key, uint64_t e, uint32_t stream)
+{
+    return philox4x32_10((uint32_t)e, (uint32_t)(e >> 32), stream, 0u, (uint32_t)key, (uint32_t)(key >> 32));
+}
+
+__device__ __forceinline__ double u53(uint32_t w0, uint32_t w1)
+{
+    const uint64_t x = ((uint64_t)w1 << 32) | (uint64_t)w0;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// 24-bit uniform as its integer k (u = k * 2^-24): exact in FP32 and FP64, and 1 - u is exact.
+__device__ __forceinline__ uint32_t k24(uint32_t w) { return w >> 8; }
+
+// SplitMix64 + u01 (reference-compatibility stream, cuPythia/common/rng.cuh:47-57).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
+
+// ------------------------------------------------------------------ own FP32 exp / log
+__device__ __forceinline__ float own_expf(float x)
+{
+    if (x < -86.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    const float n = rintf(x * 1.4426950408889634f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    r = __builtin_fmaf(n, -1.4286068203094172e-6f, r);
+    float p = 1.9841269841269841e-4f;
+    p = __builtin_fmaf(p, r, 1.3888888888888889e-3f);
+    p = __builtin_fmaf(p, r, 8.3333333333333333e-3f);
+    p = __builtin_fmaf(p, r, 4.1666666666666667e-2f);
+    p = __builtin_fmaf(p, r, 1.6666666666666667e-1f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return p * __int_as_float(((int)n + 127) << 23);
+}
+
+// 1 - e^x (x <= 0) without cancellation near 0; ex = e^x already computed.
+__device__ __forceinline__ float one_minus_expf(float x, float ex)
+{
+    if (x > -0.35f) {
+        float p = __builtin_fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
+        p = __builtin_fmaf(x, p, 1.3888888888888889e-3f);
+        p = __builtin_fmaf(x, p, 8.3333333333333333e-3f);
+        p = __builtin_fmaf(x, p, 4.1666666666666667e-2f);
+        p = __builtin_fmaf(x, p, 1.6666666666666667e-1f);
+        p = __builtin_fmaf(x, p, 0.5f);
+        p = __builtin_fmaf(x, p, 1.0f);
+        return -(x * p);
+    }
+    return 1.0f - ex;
+}
+
+__device__ __forceinline__ float own_logf(float x)
+{
+    if (!(x > 0.0f)) return -1.0e30f;
+    const int ix = __float_as_int(x);
+    int e = ((ix >> 23) & 0xff) - 127;
+    float m = __int_as_float((ix & 0x007fffff) | 0x3f800000);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    const float f = m - 1.0f;
+    const float s = f / (m + 1.0f);
+    const float s2 = s * s;
+    float p = __builtin_fmaf(s2, 0.11111111111111111f, 0.14285714285714286f);
+    p = __builtin_fmaf(s2, p, 0.2f);
+    p = __builtin_fmaf(s2, p, 0.33333333333333333f);
+    const float two_s = s + s;
+    const float lm = __builtin_fmaf(two_s * s2, p, two_s);
+    const float fe = (float)e;
+    return __builtin_fmaf(fe, 0.693145751953125f, __builtin_fmaf(fe, 1.4286068203094172e-6f, lm));
+}
+
+// ------------------------------------------------------------------ constants of the integrand
+// Layout identical to the host-side fill (mi_host.cpp: fill_had_consts).
+template <typename T>
+struct HadConst {
+    T LT;          // ln(1 / tauMin)
+    T fourPT2;     // 4 pTmin^2 = tauMin S
+    T pref;        // CONVERT2MB pi LT K / (4 pTmin^2)
+    T xiR2;        // renorm_fac^2
+    T as_order;    // 0 / 1
+    T as_fixed;
+    T scale2Min, mt2, mb2, mc2, nf6;
+    T lnL6s, lnL5s, lnL4s, lnL3s;
+    T c6, c5, c4, c3;  // 12 pi / b0
+    T lnAg, ag, bg, lnAs, as, bs, lnAuv, av, bv, rdu;
+    T nSea;        // 2 nQuarkIn
+    T nQuarkNew;
+    T spare;
+};
+static_assert(sizeof(HadConst<float>) == 32 * sizeof(float), "HadConst layout");
+
+// ------------------------------------------------------------------ FP32 hadronic point
+// Inputs: (u, 1-u) for each dimension (exact pairs); writes ws[i] = weight of code 111+i
+// (0 if masked out), returns the total.  Operation-for-operation identical to the host
+// restatement used to check compaction indices.
+__device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
+                                               float u0, float om0, float u1, float om1,
+                                               float u2, float om2, float* ws, float* sHo, float* tHo,
+                                               float* x1o, float* x2o)
+{
+    const float LT = k.LT;
+    const float targ = -(u0 * LT);
+    const float r = own_expf(targ);                        // tauMin / tau
+    const float yMax = 0.5f * (LT * om0);
+    const float twoYMax = yMax + yMax;
+    const float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
+    const float x1 = own_expf(lx1), x2 = own_expf(lx2);
+    const float zMax = sqrtf(one_minus_expf(targ, r));
+    const float omz = r / (1.0f + zMax);
+    const float twoz = zMax + zMax;
+    const float A = __builtin_fmaf(twoz, om2, omz);        // 1 - z
+    const float B = __builtin_fmaf(twoz, u2, omz);         // 1 + z
+    const float a = 0.5f * A, b = 0.5f * B;
+    const float ab = a * b;
+    const float inv_ab = 1.0f / ab;
+    const float ia = b * inv_ab, ib = a * inv_ab;
+    const float sH = k.fourPT2 / r;
+    const float pT2 = sH * ab;
+
+    float alpS;
+    if (k.as_order == 0.0f) {
+        alpS = k.as_fixed;
+    } else {
+        float Q2 = k.xiR2 * pT2;
+        if (Q2 < k.scale2Min) Q2 = k.scale2Min;
+        const float lq = own_logf(Q2);
+        float lam, cf;
+        if (Q2 > k.mt2 && k.nf6 != 0.0f) { lam = k.lnL6s; cf = k.c6; }
+        else if (Q2 > k.mb2)             { lam = k.lnL5s; cf = k.c5; }
+        else if (Q2 > k.mc2)             { lam = k.lnL4s; cf = k.c4; }
+        else                             { lam = k.lnL3s; cf = k.c3; }
+        alpS = cf / (lq - lam);
+    }
+
+    const float l1a = own_logf(one_minus_expf(lx1, x1)), l1b = own_logf(one_minus_expf(lx2, x2));
+    const float g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
+    const float g2 = own_expf(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
+    const float S1 = own_expf(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
+    const float S2 = own_expf(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
+    const float V1 = own_expf(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
+    const float V2 = own_expf(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
+    const float Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
+    const float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+    const float Q1 = __builtin_fmaf(k.nSea, S1, V1 + Vd1), Q2s = __builtin_fmaf(k.nSea, S2, V2 + Vd2);
+    const float SS = S1 * S2;
+    const float Lgg = g1 * g2;
+    const float Lqg = __builtin_fmaf(Q1, g2, g1 * Q2s);
+    const float LQQ = Q1 * Q2s;
+    const float Lsame = __builtin_fmaf(U1, U2, __builtin_fmaf(D1, D2, (k.nSea - 2.0f) * SS));
+    const float Lopp = __builtin_fmaf(U1 + D1, S2, __builtin_fmaf(S1, U2 + D2, (k.nSea - 4.0f) * SS));
+
+    const float q2 = __builtin_fmaf(a, a, b * b);
+    const float common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;
+    float w[6];
+    w[0] = Lgg * (2.25f * __builtin_fmaf(b * ia, ia, __builtin_fmaf(a * ib, ib, 3.0f - ab)));
+    w[1] = Lgg * (k.nQuarkNew * q2 * __builtin_fmaf(inv_ab, 0.16666666666666667f, -0.375f));
+    w[2] = Lqg * (__builtin_fmaf(b, b, 1.0f) * __builtin_fmaf(ia, ia, 0.44444444444444444f * ib));
+    {
+        const float sigT = 0.44444444444444444f * __builtin_fmaf(b, b, 1.0f) * (ia * ia);
+        const float sigU = 0.44444444444444444f * __builtin_fmaf(a, a, 1.0f) * (ib * ib);
+        const float sigTU = -0.29629629629629630f * inv_ab;
+        const float sigST = 0.29629629629629630f * (b * b) * ia;
+        const float csame = 0.5f * (sigU + sigTU - sigT);
+        w[3] = __builtin_fmaf(sigT, LQQ, __builtin_fmaf(csame, Lsame, sigST * Lopp));
+    }
+    w[4] = Lopp * (q2 * __builtin_fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
+    w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
+    float tot = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        float wi = (mask & (1 << i)) ? common * w[i] : 0.0f;
+        if (!(wi > 0.0f)) wi = 0.0f;
+        ws[i] = wi;
+        tot += wi;
+    }
+    if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    return tot;
+}
+
+// ------------------------------------------------------------------ FP64 hadronic point
+// Same map and algebra in FP64 with the device libm (exp, log, expm1, sqrt); the per-point
+// parity target against the host Pythia-form oracle is 1e-12 relative.
+__device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int mask,
+                                                double u0, double om0, double u1, double om1,
+                                                double u2, double om2, double* ws, double* sHo,
+                                                double* tHo, double* x1o, double* x2o)
+{
+    const double LT = k.LT;
+    const double targ = -(u0 * LT);
+    const double r = exp(targ);
+    const double yMax = 0.5 * (LT * om0);
+    const double twoYMax = yMax + yMax;
+    const double lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
+    const double x1 = exp(lx1), x2 = exp(lx2);
+    const double zMax = sqrt(-expm1(targ));
+    const double omz = r / (1.0 + zMax);
+    const double twoz = zMax + zMax;
+    const double A = fma(twoz, om2, omz);
+    const double B = fma(twoz, u2, omz);
+    const double a = 0.5 * A, b = 0.5 * B;
+    const double ab = a * b;
+    const double inv_ab = 1.0 / ab;
+    const double ia = b * inv_ab, ib = a * inv_ab;
+    const double sH = k.fourPT2 / r;
+    const double pT2 = sH * ab;
+
+    double alpS;
+    if (k.as_order == 0.0) {
+        alpS = k.as_fixed;
+    } else {
+        double Q2 = k.xiR2 * pT2;
+        if (Q2 < k.scale2Min) Q2 = k.scale2Min;
+        const double lq = log(Q2);
+        double lam, cf;
+        if (Q2 > k.mt2 && k.nf6 != 0.0) { lam = k.lnL6s; cf = k.c6; }
+        else if (Q2 > k.mb2)            { lam = k.lnL5s; cf = k.c5; }
+        else if (Q2 > k.mc2)            { lam = k.lnL4s; cf = k.c4; }
+        else                            { lam = k.lnL3s; cf = k.c3; }
+        alpS = cf / (lq - lam);
+    }
+
+    const double l1a = log(-expm1(lx1)), l1b = log(-expm1(lx2));
+    const double g1 = exp(fma(k.bg, l1a, fma(k.ag, lx1, k.lnAg)));
+    const double g2 = exp(fma(k.bg, l1b, fma(k.ag, lx2, k.lnAg)));
+    const double S1 = exp(fma(k.bs, l1a, fma(k.as, lx1, k.lnAs)));
+    const double S2 = exp(fma(k.bs, l1b, fma(k.as, lx2, k.lnAs)));
+    const double V1 = exp(fma(k.bv, l1a, fma(k.av, lx1, k.lnAuv)));
+    const double V2 = exp(fma(k.bv, l1b, fma(k.av, lx2, k.lnAuv)));
+    const double Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
+    const double U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+    const double Q1 = fma(k.nSea, S1, V1 + Vd1), Q2s = fma(k.nSea, S2, V2 + Vd2);
+    const double SS = S1 * S2;
+    const double Lgg = g1 * g2;
+    const double Lqg = fma(Q1, g2, g1 * Q2s);
+    const double LQQ = Q1 * Q2s;
+    const double Lsame = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
+    const double Lopp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+
+    const double q2 = fma(a, a, b * b);
+    const double common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;
+    double w[6];
+    w[0] = Lgg * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
+    w[1] = Lgg * (k.nQuarkNew * q2 * fma(inv_ab, 1.0 / 6.0, -0.375));
+    w[2] = Lqg * (fma(b, b, 1.0) * fma(ia, ia, (4.0 / 9.0) * ib));
+    {
+        const double sigT = (4.0 / 9.0) * fma(b, b, 1.0) * (ia * ia);
+        const double sigU = (4.0 / 9.0) * fma(a, a, 1.0) * (ib * ib);
+        const double sigTU = -(8.0 / 27.0) * inv_ab;
+        const double sigST = (8.0 / 27.0) * (b * b) * ia;
+        const double csame = 0.5 * (sigU + sigTU - sigT);
+        w[3] = fma(sigT, LQQ, fma(csame, Lsame, sigST * Lopp));
+    }
+    w[4] = Lopp * (q2 * fma(inv_ab, 16.0 / 27.0, -4.0 / 3.0));
+    w[5] = Lopp * (k.nQuarkNew * (4.0 / 9.0) * q2);
+    double tot = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double wi = (mask & (1 << i)) ? common * w[i] : 0.0;
+        if (!(wi > 0.0)) wi = 0.0;
+        ws[i] = wi;
+        tot += wi;
+    }
+    if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    return tot;
+}
+
+// ------------------------------------------------------------------ partonic gg -> gg, FP64
+// w(c) = CONV * pi alphaS^2 / s^2 * B_gg(a, b) * (2 cMax)(s / 2), a = (1 - c)/2, b = (1 + c)/2.
+// pref = CONV pi alphaS^2 cMax / s.
+__device__ __forceinline__ double c1_weight(double pref, double cMax, double u)
+{
+    const double c = cMax * (2.0 * u - 1.0);
+    const double a = 0.5 * (1.0 - c), b = 0.5 * (1.0 + c);
+    const double ab = a * b;
+    const double inv_ab = 1.0 / ab;
+    const double ia = b * inv_ab, ib = a * inv_ab;
+    return pref * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
+}
+
+}  // namespace mi

@@ -1,0 +1,134 @@
+// mi_internal.h -- host-side context and kernel launchers shared by the library's translation
+// units.  Not part of the C ABI (include/mcint.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/mcint.h"
+#include "mi_device.h"
+
+namespace mi {
+
+// Block partial record of the integration kernels (FP64):
+//   [0..6]   sum w       (0 = total, 1 + i = code 111 + i)
+//   [7..13]  sum w^2
+//   [14..20] max w
+//   [21]     points with total weight > 0
+//   [22..23] spare (adjoint / unweighting use them)
+constexpr int kRec = 24;
+constexpr int kBlock = 256;
+constexpr int kMaxBins = 256;
+
+struct Grid {
+    int ndim = 0, nbins = 0;
+    double *edges = nullptr;   // ndim x (nbins + 1), FP64 (device)
+    float *tab32 = nullptr;    // ndim x nbins x {lo, width, 1 - lo} (device)
+    double *tab64 = nullptr;   // same in FP64 (device)
+};
+
+struct FlowState;              // mi_flow.hip
+
+}  // namespace mi
+
+struct mi_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // scratch, grown on demand (never shrunk)
+    double *partials = nullptr;  size_t partials_cap = 0;   // doubles
+    double *hist = nullptr;      size_t hist_cap = 0;       // doubles
+    double *red = nullptr;       size_t red_cap = 0;        // doubles (reduced records)
+    uint64_t *counts = nullptr;  size_t counts_cap = 0;     // per-block counts / offsets
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    mi::Grid grid;
+    mi::FlowState *flow = nullptr;
+};
+
+namespace mi {
+
+// ------------------------------------------------------------------ constants
+void fill_had_consts_f32(const mi_physics &p, HadConst<float> &k);
+void fill_had_consts_f64(const mi_physics &p, HadConst<double> &k);
+
+// ------------------------------------------------------------------ launch geometry
+struct Geom { uint32_t blocks; uint32_t ppt; };
+// Points per thread >= min_ppt, blocks <= max_blocks.
+Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks);
+// Resident blocks per CU of the integration kernels (hipOccupancyMaxActiveBlocksPerMultiprocessor).
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins);
+int occ_integrate_c1(bool vegas, bool hist, int nbins);
+
+// ------------------------------------------------------------------ launchers (mi_kernels.hip)
+hipError_t launch_philox_fill(hipStream_t s, uint64_t key, uint64_t ctr0, uint32_t stream_id,
+                              uint64_t n, uint32_t *out);
+hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *sH, const double *tH,
+                            const double *uH, const double *alpS, const int32_t *id1,
+                            const int32_t *id2, int nQuarkNew, double kfactor, double *out);
+
+// Integration of the hadronic integrand: writes blocks x kRec partials; if hist != nullptr
+// also blocks x (ndim * nbins) per-bin sums of w^2 (VEGAS).
+struct HadLaunch {
+    int prec;            // MI_PREC_FP32 / FP64
+    bool vegas;          // map through grid
+    int mask;
+    uint64_t key, ctr0, n;
+    Geom g;
+    HadConst<float> kf;
+    HadConst<double> kd;
+    const Grid *grid;
+    double *partials;
+    double *hist;
+};
+hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L);
+
+struct C1Launch {
+    bool vegas;
+    uint64_t key, ctr0, n;
+    Geom g;
+    double pref, cMax;
+    const Grid *grid;
+    double *partials;
+    double *hist;
+};
+hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L);
+
+// Per-point dump (parity hook).
+hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, double *kin);
+hipError_t launch_points_c1(hipStream_t s, const C1Launch &L, double sHat, double *w, double *kin);
+
+// Deterministic reductions.
+hipError_t launch_reduce_records(hipStream_t s, const double *partials, uint32_t nblk, double *out);
+hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nblk, uint32_t ncol,
+                                 double *out);
+
+// VEGAS refinement on the device (one block).
+hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, double alpha);
+hipError_t launch_vegas_tables(hipStream_t s, Grid &g);
+
+// Unweighting (mi_unweight.hip).
+struct UnwLaunch {
+    HadLaunch h;          // hadronic candidates (h.partials receives the estimate partials)
+    bool partonic;        // partonic gg->gg (C1 integrand) instead
+    C1Launch c;
+    double sHat;          // partonic
+    bool compat;          // SplitMix64 reference-compat stream (partonic only)
+    double wmax;
+};
+hipError_t launch_unweight_count(hipStream_t s, const UnwLaunch &L, uint64_t *counts);
+hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk);
+hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64_t *offsets,
+                                 const mi_events &out);
+
+// Adjoint (mi_adjoint.hip): partials of kRec doubles per block, see that file for the layout.
+struct AdjLaunch {
+    HadLaunch h;
+    double dlnA_dasMZ[4];   // d ln(Lambda_nf^2) / d alphaS(MZ) for nf = 6, 5, 4, 3
+    double lnxiR2;
+};
+hipError_t launch_adjoint(hipStream_t s, const AdjLaunch &L);
+
+}  // namespace mi

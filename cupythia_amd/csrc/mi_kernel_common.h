@@ -1,0 +1,148 @@
+// mi_kernel_common.h -- device helpers shared by the kernel translation units: wave / block
+// reductions, the VEGAS grid map and the per-candidate integrand evaluation.
+#pragma once
+
+#include "mi_internal.h"
+
+namespace mi {
+
+// ------------------------------------------------------------------------------------ utils
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// Combine kRec per-thread values into one block record (sum except [14..20] = max).
+// If cnt is given, it receives slot 22 (the unweighting accept count) as an integer.
+__device__ __forceinline__ void block_record(const double *v, double *dst, uint64_t *cnt = nullptr)
+{
+    __shared__ double red[kBlock / 64][kRec];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kRec; ++i) {
+        const bool isMax = (i >= 14 && i <= 20);
+        const double r = isMax ? wave_max(v[i]) : wave_sum(v[i]);
+        if (lane == 0) red[wave][i] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRec) {
+        const int i = threadIdx.x;
+        const bool isMax = (i >= 14 && i <= 20);
+        double r = red[0][i];
+        for (int w = 1; w < kBlock / 64; ++w) r = isMax ? fmax(r, red[w][i]) : r + red[w][i];
+        dst[i] = r;
+        if (cnt && i == 22) *cnt = (uint64_t)r;
+    }
+}
+
+// ------------------------------------------------------------------------------------ map
+// 24-bit uniform k -> (x, 1 - x, bin, jacobian factor) through the grid table of one dim.
+template <typename T>
+struct MapOut { T x, omx, jac; int bin; };
+
+template <typename T>
+__device__ __forceinline__ MapOut<T> map_flat(uint32_t k)
+{
+    MapOut<T> m;
+    m.x = (T)k * (T)(1.0 / 16777216.0);
+    m.omx = (T)(16777216u - k) * (T)(1.0 / 16777216.0);
+    m.jac = (T)1;
+    m.bin = 0;
+    return m;
+}
+
+template <typename T>
+__device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbins, T nbf)
+{
+    const uint32_t kb = k * (uint32_t)nbins;              // < 2^32 for nbins <= 256
+    const int b = (int)(kb >> 24);
+    const T frac = (T)(kb & 0xFFFFFFu) * (T)(1.0 / 16777216.0);
+    const T lo = tab[3 * b + 0], wd = tab[3 * b + 1], omlo = tab[3 * b + 2];
+    MapOut<T> m;
+    m.x = fma(frac, wd, lo);
+    // FP32: 1 - x from the complement table keeps ~1e-7 relative accuracy next to x = 1.
+    // FP64: 1 - x of the mapped point itself (exact for x >= 1/2), so the point is exactly x.
+    m.omx = (sizeof(T) == 4) ? fma(-frac, wd, omlo) : (T)1 - m.x;
+    m.jac = nbf * wd;
+    m.bin = b;
+    return m;
+}
+
+template <typename T> struct HadPoint;
+template <> struct HadPoint<float> {
+    static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
+                                                 float u1, float o1, float u2, float o2, float *ws,
+                                                 float *sH, float *tH, float *x1, float *x2)
+    { return had_point_f32(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2); }
+};
+template <> struct HadPoint<double> {
+    static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
+                                                  double u1, double o1, double u2, double o2, double *ws,
+                                                  double *sH, double *tH, double *x1, double *x2)
+    { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2); }
+};
+
+// One candidate of the hadronic integrand: Philox stream 0 of event e -> three 24-bit
+// uniforms -> (optional VEGAS map) -> weights (mapped by the grid Jacobian).
+template <typename T, bool VEGAS>
+__device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint64_t key, uint64_t e,
+                                           const T *stab, int nbins, T nbf, T *ws, int *bins,
+                                           T *sH, T *tH, T *x1, T *x2)
+{
+    const U4 r = philox_event(key, e, 0u);
+    MapOut<T> m0, m1, m2;
+    if (VEGAS) {
+        m0 = map_grid<T>(k24(r.x), stab, nbins, nbf);
+        m1 = map_grid<T>(k24(r.y), stab + 3 * nbins, nbins, nbf);
+        m2 = map_grid<T>(k24(r.z), stab + 6 * nbins, nbins, nbf);
+    } else {
+        m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
+    }
+    T tot = HadPoint<T>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2);
+    if (VEGAS) {
+        T jac = (T)1;
+        jac = jac * m0.jac;
+        jac = jac * m1.jac;
+        jac = jac * m2.jac;
+#pragma unroll
+        for (int i = 0; i < kMaxProc; ++i) ws[i] = ws[i] * jac;
+        tot = tot * jac;
+        bins[0] = m0.bin; bins[1] = m1.bin; bins[2] = m2.bin;
+    }
+    return tot;
+}
+
+// ------------------------------------------------------------------------------------ partonic candidate
+// Event e: Philox stream 0, u = 53-bit uniform from words (0, 1); optional 1-D grid
+// (FP64 edges, bin from the top bits of the 53-bit integer).
+__device__ __forceinline__ double c1_candidate(double pref, double cMax, uint64_t key, uint64_t e, bool vegas,
+                                               const double *tab, int nbins, int *bin)
+{
+    const U4 r = philox_event(key, e, 0u);
+    const uint64_t x53 = ((((uint64_t)r.y) << 32) | r.x) >> 11;
+    double u, jac = 1.0;
+    if (vegas) {
+        // bin = floor(u * nbins) exactly: the top bits of x53 * nbins (fits in 61 bits)
+        const uint64_t kb = x53 * (uint64_t)nbins;
+        const int b = (int)(kb >> 53);
+        const double frac = (double)(kb & ((1ULL << 53) - 1)) * (1.0 / 9007199254740992.0);
+        const double lo = tab[3 * b], wd = tab[3 * b + 1];
+        u = fma(frac, wd, lo);
+        jac = (double)nbins * wd;
+        *bin = b;
+    } else {
+        u = (double)x53 * (1.0 / 9007199254740992.0);
+        *bin = 0;
+    }
+    return c1_weight(pref, cMax, u) * jac;
+}
+
+}  // namespace mi

@@ -1,0 +1,499 @@
+// mi_kernels.hip -- gfx950 kernels of the integration path: Philox fill, batched sigmaHat,
+// the fused hadronic / partonic integration kernels (RNG in registers -> map -> PDFs x
+// sigmaHat -> FP64 sum w, sum w^2, max), per-point dumps, deterministic reductions and the
+// on-device VEGAS refinement.
+//
+// Reduction scheme (deterministic, no floating-point atomics except the per-wave LDS VEGAS
+// histogram): each thread accumulates its points sequentially in FP64; the 64 lanes of a wave
+// combine by an xor butterfly of __shfl_xor; the 4 waves of a block combine in wave order
+// through LDS; one record per block goes to global memory; a single-block kernel folds the
+// block records in a fixed order.  The result is bit-identical from run to run.
+#include "mi_kernel_common.h"
+
+namespace mi {
+
+// ------------------------------------------------------------------------------------ Philox fill
+__global__ __launch_bounds__(kBlock) void k_philox_fill(uint64_t key, uint64_t ctr0, uint32_t sid,
+                                                        uint64_t n, uint32_t *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const U4 r = philox_event(key, ctr0 + i, sid);
+    uint4 v; v.x = r.x; v.y = r.y; v.z = r.z; v.w = r.w;
+    reinterpret_cast<uint4 *>(out)[i] = v;
+}
+
+hipError_t launch_philox_fill(hipStream_t s, uint64_t key, uint64_t ctr0, uint32_t sid, uint64_t n,
+                              uint32_t *out)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_philox_fill, dim3((uint32_t)blocks), dim3(kBlock), 0, s, key, ctr0, sid, n, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ sigmaHat
+// Kfactor * sigmaHatWrap for one code, FP64.  Brackets in terms of the ratios t/s, u/s with
+// three reciprocals; algebraically identical to SigmaQCD.cc:115-378.
+__global__ __launch_bounds__(kBlock) void k_sigma_hat(int code, uint64_t n, const double *__restrict__ sH,
+                                                      const double *__restrict__ tH, const double *__restrict__ uH,
+                                                      const double *__restrict__ alpS,
+                                                      const int32_t *__restrict__ id1,
+                                                      const int32_t *__restrict__ id2, double nQN,
+                                                      double kfac, double *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = sH[i], t = tH[i];
+    const double u = uH ? uH[i] : -(s + t);
+    const double as = alpS[i];
+    const double is = 1.0 / s, it = 1.0 / t, iu = 1.0 / u;
+    const double ts = t * is, us = u * is, st = s * it, su = s * iu, tu = t * iu, ut = u * it;
+    double br;
+    switch (code) {
+    case MI_GG2GG:
+        br = 2.25 * (3.0 - ts * us - us * st * st - ts * su * su);
+        break;
+    case MI_GG2QQBAR:
+        br = nQN * ((1.0 / 6.0) * (ut + tu) - 0.375 * (ts * ts + us * us));
+        break;
+    case MI_QG2QG:
+        br = (1.0 + us * us) * (st * st - (4.0 / 9.0) * su);
+        break;
+    case MI_QQ2QQ: {
+        const int a = id1 ? id1[i] : 1, b = id2 ? id2[i] : 2;
+        const double sigT = (4.0 / 9.0) * (1.0 + us * us) * (st * st);
+        if (b == a) {
+            const double sigU = (4.0 / 9.0) * (1.0 + ts * ts) * (su * su);
+            const double sigTU = -(8.0 / 27.0) * (st * su);
+            br = 0.5 * (sigT + sigU + sigTU);
+        } else if (b == -a) {
+            br = sigT - (8.0 / 27.0) * (us * us) * st;
+        } else {
+            br = sigT;
+        }
+        break;
+    }
+    case MI_QQBAR2GG:
+        br = (16.0 / 27.0) * (ut + tu) - (4.0 / 3.0) * (ts * ts + us * us);
+        break;
+    case MI_QQBAR2QQBARNEW:
+        br = nQN * (4.0 / 9.0) * (ts * ts + us * us);
+        break;
+    default:
+        br = 0.0;
+    }
+    out[i] = kfac * kConvert2mb * (kPi * is * is) * (as * as) * br;
+}
+
+hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *sH, const double *tH,
+                            const double *uH, const double *alpS, const int32_t *id1, const int32_t *id2,
+                            int nQuarkNew, double kfactor, double *out)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sigma_hat, dim3((uint32_t)blocks), dim3(kBlock), 0, s, code, n, sH, tH, uH, alpS,
+                       id1, id2, (double)nQuarkNew, kfactor, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ hadronic integration
+// Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
+template <typename T, bool VEGAS, bool HIST>
+__global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
+                                                          uint64_t ctr0, uint64_t n, uint32_t ppt,
+                                                          const T *__restrict__ gtab, int nbins,
+                                                          double *__restrict__ partials,
+                                                          double *__restrict__ hist)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *stab = reinterpret_cast<T *>(smem);                                   // 3 dims x nbins x {lo, wd, 1-lo}
+    const size_t tabBytes = (sizeof(T) * 9 * (VEGAS ? nbins : 0) + 7) & ~(size_t)7;
+    double *shist = reinterpret_cast<double *>(smem + tabBytes);             // 4 waves x 3 dims x nbins
+    const int wave = threadIdx.x >> 6;
+    if (VEGAS) {
+        for (int i = threadIdx.x; i < 9 * nbins; i += blockDim.x) stab[i] = gtab[i];
+        if (HIST)
+            for (int i = threadIdx.x; i < 4 * 3 * nbins; i += blockDim.x) shist[i] = 0.0;
+        __syncthreads();
+    }
+    const T nbf = (T)nbins;
+    double v[kRec];
+#pragma unroll
+    for (int i = 0; i < kRec; ++i) v[i] = 0.0;
+
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
+    for (uint32_t j = 0; j < ppt; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i >= n) break;
+        T ws[kMaxProc];
+        int bins[3];
+        const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
+                                              (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr);
+        const double dt = (double)tot;
+        v[0] += dt;
+        v[7] = fma(dt, dt, v[7]);
+        v[14] = fmax(v[14], dt);
+        v[21] += (tot > (T)0) ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < kMaxProc; ++p) {
+            const double d = (double)ws[p];
+            v[1 + p] += d;
+            v[8 + p] = fma(d, d, v[8 + p]);
+            v[15 + p] = fmax(v[15 + p], d);
+        }
+        if (HIST) {
+            const double w2 = dt * dt;
+            double *h = shist + wave * 3 * nbins;
+            atomicAdd(&h[bins[0]], w2);
+            atomicAdd(&h[nbins + bins[1]], w2);
+            atomicAdd(&h[2 * nbins + bins[2]], w2);
+        }
+    }
+    block_record(v, partials + (size_t)blockIdx.x * kRec);
+    if (HIST) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < 3 * nbins; c += blockDim.x) {
+            double s = shist[c];
+            for (int w = 1; w < kBlock / 64; ++w) s += shist[w * 3 * nbins + c];
+            hist[(size_t)blockIdx.x * 3 * nbins + c] = s;
+        }
+    }
+}
+
+static size_t had_smem(const HadLaunch &L)
+{
+    if (!L.vegas) return 0;
+    const size_t tb = ((L.prec == MI_PREC_FP32 ? sizeof(float) : sizeof(double)) * 9 * L.grid->nbins + 7) & ~(size_t)7;
+    return tb + (L.hist ? sizeof(double) * 4 * 3 * L.grid->nbins : 0);
+}
+
+template <typename T>
+static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab)
+{
+    const size_t sm = had_smem(L);
+    const int nb = L.vegas ? L.grid->nbins : 0;
+    dim3 grid(L.g.blocks), blk(kBlock);
+    if (!L.vegas)
+        hipLaunchKernelGGL((k_integrate_had<T, false, false>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    else if (!L.hist)
+        hipLaunchKernelGGL((k_integrate_had<T, true, false>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    else
+        hipLaunchKernelGGL((k_integrate_had<T, true, true>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L)
+{
+    if (L.prec == MI_PREC_FP32)
+        return launch_had_t<float>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr);
+    return launch_had_t<double>(s, L, L.kd, L.vegas ? L.grid->tab64 : nullptr);
+}
+
+// ------------------------------------------------------------------------------------ partonic integration
+template <bool VEGAS, bool HIST>
+__global__ __launch_bounds__(kBlock) void k_integrate_c1(double pref, double cMax, uint64_t key, uint64_t ctr0,
+                                                         uint64_t n, uint32_t ppt, const double *__restrict__ gtab,
+                                                         int nbins, double *__restrict__ partials,
+                                                         double *__restrict__ hist)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *stab = reinterpret_cast<double *>(smem);
+    double *shist = stab + 3 * (VEGAS ? nbins : 0);
+    const int wave = threadIdx.x >> 6;
+    if (VEGAS) {
+        for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) stab[i] = gtab[i];
+        if (HIST)
+            for (int i = threadIdx.x; i < 4 * nbins; i += blockDim.x) shist[i] = 0.0;
+        __syncthreads();
+    }
+    double v[kRec];
+#pragma unroll
+    for (int i = 0; i < kRec; ++i) v[i] = 0.0;
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
+    for (uint32_t j = 0; j < ppt; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i >= n) break;
+        int bin;
+        const double w = c1_candidate(pref, cMax, key, ctr0 + i, VEGAS, stab, nbins, &bin);
+        v[0] += w;
+        v[7] = fma(w, w, v[7]);
+        v[14] = fmax(v[14], w);
+        v[21] += (w > 0.0) ? 1.0 : 0.0;
+        if (HIST) atomicAdd(&shist[wave * nbins + bin], w * w);
+    }
+    // gg -> gg is the only process: copy total into slot of code 111
+    v[1] = v[0]; v[8] = v[7]; v[15] = v[14];
+    block_record(v, partials + (size_t)blockIdx.x * kRec);
+    if (HIST) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < nbins; c += blockDim.x) {
+            double s = shist[c];
+            for (int w = 1; w < kBlock / 64; ++w) s += shist[w * nbins + c];
+            hist[(size_t)blockIdx.x * nbins + c] = s;
+        }
+    }
+}
+
+hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L)
+{
+    const int nb = L.vegas ? L.grid->nbins : 0;
+    const size_t sm = L.vegas ? sizeof(double) * (3 * nb + (L.hist ? 4 * nb : 0)) : 0;
+    const double *tab = L.vegas ? L.grid->tab64 : nullptr;
+    dim3 grid(L.g.blocks), blk(kBlock);
+    if (!L.vegas)
+        hipLaunchKernelGGL((k_integrate_c1<false, false>), grid, blk, sm, s, L.pref, L.cMax, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    else if (!L.hist)
+        hipLaunchKernelGGL((k_integrate_c1<true, false>), grid, blk, sm, s, L.pref, L.cMax, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    else
+        hipLaunchKernelGGL((k_integrate_c1<true, true>), grid, blk, sm, s, L.pref, L.cMax, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ per-point dumps
+template <typename T, bool VEGAS>
+__global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, uint64_t key, uint64_t ctr0,
+                                                       uint64_t n, const T *__restrict__ gtab, int nbins,
+                                                       double *__restrict__ w, double *__restrict__ kin)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T ws[kMaxProc];
+    int bins[3];
+    T sH, tH, x1, x2;
+    const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, &sH, &tH,
+                                          &x1, &x2);
+    w[i * (kMaxProc + 1)] = (double)tot;
+#pragma unroll
+    for (int p = 0; p < kMaxProc; ++p) w[i * (kMaxProc + 1) + 1 + p] = (double)ws[p];
+    if (kin) {
+        kin[4 * i + 0] = (double)sH; kin[4 * i + 1] = (double)tH;
+        kin[4 * i + 2] = (double)x1; kin[4 * i + 3] = (double)x2;
+    }
+}
+
+hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, double *kin)
+{
+    if (L.n == 0) return hipSuccess;
+    dim3 grid((uint32_t)((L.n + kBlock - 1) / kBlock)), blk(kBlock);
+    const int nb = L.vegas ? L.grid->nbins : 0;
+    if (L.prec == MI_PREC_FP32) {
+        const float *tab = L.vegas ? L.grid->tab32 : nullptr;
+        if (L.vegas) hipLaunchKernelGGL((k_points_had<float, true>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+        else         hipLaunchKernelGGL((k_points_had<float, false>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+    } else {
+        const double *tab = L.vegas ? L.grid->tab64 : nullptr;
+        if (L.vegas) hipLaunchKernelGGL((k_points_had<double, true>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+        else         hipLaunchKernelGGL((k_points_had<double, false>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+    }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_points_c1(double pref, double cMax, double sHat, uint64_t key,
+                                                      uint64_t ctr0, uint64_t n, int vegas,
+                                                      const double *__restrict__ gtab, int nbins,
+                                                      double *__restrict__ w, double *__restrict__ kin)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int bin;
+    const double wt = c1_candidate(pref, cMax, key, ctr0 + i, vegas != 0, gtab, nbins, &bin);
+    w[i * (kMaxProc + 1)] = wt;
+    w[i * (kMaxProc + 1) + 1] = wt;
+    for (int p = 1; p < kMaxProc; ++p) w[i * (kMaxProc + 1) + 1 + p] = 0.0;
+    if (kin) {
+        // recompute c for the kinematics record
+        const U4 r = philox_event(key, ctr0 + i, 0u);
+        const uint64_t x53 = ((((uint64_t)r.y) << 32) | r.x) >> 11;
+        double u = (double)x53 * (1.0 / 9007199254740992.0);
+        if (vegas) {
+            const uint64_t kb = x53 * (uint64_t)nbins;
+            const int b = (int)(kb >> 53);
+            const double frac = (double)(kb & ((1ULL << 53) - 1)) * (1.0 / 9007199254740992.0);
+            u = fma(frac, gtab[3 * b + 1], gtab[3 * b]);
+        }
+        const double c = cMax * (2.0 * u - 1.0);
+        kin[4 * i + 0] = sHat; kin[4 * i + 1] = -0.5 * sHat * (1.0 - c);
+        kin[4 * i + 2] = 1.0; kin[4 * i + 3] = 1.0;
+    }
+}
+
+hipError_t launch_points_c1(hipStream_t s, const C1Launch &L, double sHat, double *w, double *kin)
+{
+    if (L.n == 0) return hipSuccess;
+    dim3 grid((uint32_t)((L.n + kBlock - 1) / kBlock)), blk(kBlock);
+    hipLaunchKernelGGL(k_points_c1, grid, blk, 0, s, L.pref, L.cMax, sHat, L.key, L.ctr0, L.n, L.vegas ? 1 : 0,
+                       L.vegas ? L.grid->tab64 : nullptr, L.vegas ? L.grid->nbins : 0, w, kin);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ reductions
+// Fold nblk block records into one, in a fixed order: thread t takes records t, t+256, ...
+// sequentially, then a fixed LDS tree.
+__global__ __launch_bounds__(kBlock) void k_reduce_records(const double *__restrict__ parts, uint32_t nblk,
+                                                           double *__restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    for (int i = 0; i < kRec; ++i) {
+        const bool isMax = (i >= 14 && i <= 20);
+        double acc = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblk; b += kBlock) {
+            const double x = parts[(size_t)b * kRec + i];
+            acc = isMax ? fmax(acc, x) : acc + x;
+        }
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = kBlock / 2; off >= 1; off >>= 1) {
+            if ((int)threadIdx.x < off)
+                sh[threadIdx.x] = isMax ? fmax(sh[threadIdx.x], sh[threadIdx.x + off]) : sh[threadIdx.x] + sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[i] = sh[0];
+        __syncthreads();
+    }
+}
+
+hipError_t launch_reduce_records(hipStream_t s, const double *partials, uint32_t nblk, double *out)
+{
+    hipLaunchKernelGGL(k_reduce_records, dim3(1), dim3(kBlock), 0, s, partials, nblk, out);
+    return hipGetLastError();
+}
+
+// Column sums of an nblk x ncol matrix, each column summed sequentially over blocks.
+__global__ __launch_bounds__(kBlock) void k_reduce_columns(const double *__restrict__ parts, uint32_t nblk,
+                                                           uint32_t ncol, double *__restrict__ out)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) s += parts[(size_t)b * ncol + c];
+    out[c] = s;
+}
+
+hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nblk, uint32_t ncol, double *out)
+{
+    hipLaunchKernelGGL(k_reduce_columns, dim3((ncol + kBlock - 1) / kBlock), dim3(kBlock), 0, s, parts, nblk, ncol,
+                       out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ VEGAS refinement
+// Lepage, J. Comput. Phys. 27 (1978) 192: per dimension, d_i = sum of w^2 in bin i, smoothed
+// over neighbours (ends averaged with one neighbour), normalised, damped
+// m_i = ((1 - d_i) / -ln d_i)^alpha, then the edges are moved so every new bin holds an equal
+// share of sum m_i, interpolating linearly inside the old bins.  One thread per dimension.
+__global__ void k_vegas_refine(double *__restrict__ edges, const double *__restrict__ hist, int ndim, int nbins,
+                               double alpha)
+{
+    __shared__ double sm[3][kMaxBins];
+    __shared__ double sne[3][kMaxBins + 1];
+    const int d = threadIdx.x;
+    if (d >= ndim) return;
+    double *m = sm[d], *ne = sne[d];
+    const double *h = hist + (size_t)d * nbins;
+    double *x = edges + (size_t)d * (nbins + 1);
+    double tot = 0.0;
+    for (int i = 0; i < nbins; ++i) {
+        double s;
+        if (nbins == 1) s = h[0];
+        else if (i == 0) s = 0.5 * (h[0] + h[1]);
+        else if (i == nbins - 1) s = 0.5 * (h[nbins - 2] + h[nbins - 1]);
+        else s = (h[i - 1] + h[i] + h[i + 1]) / 3.0;
+        m[i] = s;
+        tot += s;
+    }
+    if (!(tot > 0.0)) return;                                  // nothing to adapt to
+    double msum = 0.0;
+    for (int i = 0; i < nbins; ++i) {
+        const double r = m[i] / tot;
+        double mi = 0.0;
+        if (r > 0.0) mi = (r < 1.0) ? pow((1.0 - r) / -log(r), alpha) : 1.0;
+        m[i] = mi;
+        msum += mi;
+    }
+    if (!(msum > 0.0)) return;
+    const double step = msum / nbins;
+    ne[0] = x[0];
+    ne[nbins] = x[nbins];
+    int old = 0;
+    double acc = 0.0;                                          // sum of m over bins < old
+    for (int j = 1; j < nbins; ++j) {
+        const double target = step * j;
+        while (old < nbins - 1 && acc + m[old] < target) { acc += m[old]; ++old; }
+        double frac = (m[old] > 0.0) ? (target - acc) / m[old] : 0.0;
+        frac = fmin(fmax(frac, 0.0), 1.0);
+        ne[j] = x[old] + frac * (x[old + 1] - x[old]);
+    }
+    for (int j = 0; j <= nbins; ++j) x[j] = ne[j];
+}
+
+// Per-bin tables {lo, width, 1 - lo} in FP64 and FP32 from the FP64 edges.
+__global__ void k_vegas_tables(const double *__restrict__ edges, int ndim, int nbins, double *__restrict__ t64,
+                               float *__restrict__ t32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndim * nbins) return;
+    const int d = i / nbins, b = i % nbins;
+    const double lo = edges[d * (nbins + 1) + b], hi = edges[d * (nbins + 1) + b + 1];
+    t64[3 * i + 0] = lo; t64[3 * i + 1] = hi - lo; t64[3 * i + 2] = 1.0 - lo;
+    t32[3 * i + 0] = (float)lo; t32[3 * i + 1] = (float)(hi - lo); t32[3 * i + 2] = (float)(1.0 - lo);
+}
+
+hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, double alpha)
+{
+    hipLaunchKernelGGL(k_vegas_refine, dim3(1), dim3(64), 0, s, g.edges, hist_sum, g.ndim, g.nbins, alpha);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_vegas_tables(s, g);
+}
+
+hipError_t launch_vegas_tables(hipStream_t s, Grid &g)
+{
+    const int n = g.ndim * g.nbins;
+    hipLaunchKernelGGL(k_vegas_tables, dim3((n + 255) / 256), dim3(256), 0, s, g.edges, g.ndim, g.nbins, g.tab64,
+                       g.tab32);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------ occupancy
+// Resident blocks per CU of each integration kernel instance (for one-wave, equal-work grids).
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins)
+{
+    HadLaunch L{};
+    L.prec = prec; L.vegas = vegas;
+    Grid g; g.nbins = nbins; g.ndim = 3;
+    L.grid = &g; L.hist = hist ? (double *)1 : nullptr;
+    const size_t sm = had_smem(L);
+    int nb = 0;
+    hipError_t e;
+    if (prec == MI_PREC_FP32) {
+        if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, false, false>, kBlock, sm);
+        else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, true, false>, kBlock, sm);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, true, true>, kBlock, sm);
+    } else {
+        if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, false, false>, kBlock, sm);
+        else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, true, false>, kBlock, sm);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, true, true>, kBlock, sm);
+    }
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int occ_integrate_c1(bool vegas, bool hist, int nbins)
+{
+    const int nb = vegas ? nbins : 0;
+    const size_t sm = vegas ? sizeof(double) * (3 * nb + (hist ? 4 * nb : 0)) : 0;
+    int b = 0;
+    hipError_t e;
+    if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_integrate_c1<false, false>, kBlock, sm);
+    else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_integrate_c1<true, false>, kBlock, sm);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_integrate_c1<true, true>, kBlock, sm);
+    return (e == hipSuccess && b > 0) ? b : 1;
+}
+
+}  // namespace mi

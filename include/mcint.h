@@ -1,0 +1,252 @@
+/*
+ * mcint.h -- C ABI of the MI355X (gfx950) Monte-Carlo matrix-element integration library
+ * (cupythia_amd/_lib/libmcint.so).
+ *
+ * Plain C: no C++ types, no exceptions, no torch types cross this boundary.  Every entry point
+ * returns an int status (MI_OK = 0, negative = error); mi_last_error(ctx) returns the message
+ * of the last failure on that context.  Bulk arrays are DEVICE pointers (memory resident in
+ * HBM); configuration and result structs are host memory owned by the caller.  One mi_ctx per
+ * host thread; contexts are independent.  Results are deterministic for a given
+ * (key, counter range, configuration).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   process identity + per-point d(sigmaHat)/d(tHat)   pythia8317/include/Pythia8/SigmaProcess.h:88-402,
+ *       (Sigma2Process::set2Kin / sigmaKin / sigmaHatWrap) SigmaProcess.h:487-546, :516-520;
+ *       codes 111..116                                   pythia8317/include/Pythia8/SigmaQCD.h:188-392
+ *   sigma estimate (sigmaGen / sigmaErr / nTry / nAcc)  pythia8317/src/ProcessContainer.cc:1416-1472,
+ *                                                        pythia8317/include/Pythia8/Info.h:316-330,
+ *                                                        pythia8317/src/ProcessLevel.cc:428-472
+ *   counter RNG                                          cuPythia/common/rng.cuh:47-57 (SplitMix64);
+ *                                                        Pythia hook RndmEngine::flat, Basics.h:360-371
+ *   fused flat MC of gg->gg                              cuPythia/03_fused_resident_me/fused_gg2gg.cu:40-51
+ *   VEGAS sample + rebin                                 cuPythia/11_vegas_importance/vegas.cu:40-85
+ *   unweighting + stable compaction                      cuPythia/10_cub_compaction/cub_compaction.cu:36-80,
+ *                                                        cuPythia/pipeline/generate.cu:56-62,98-103
+ *   same-sample scale variations (adjoint precursor)    cuPythia/pipeline/reweight.cu:36-51
+ */
+#ifndef MCINT_H
+#define MCINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_ABI_VERSION 1
+
+typedef struct mi_ctx mi_ctx;   /* opaque */
+
+/* status codes */
+#define MI_OK            0
+#define MI_ERR_ARG      -1   /* invalid argument / configuration */
+#define MI_ERR_HIP      -2   /* HIP runtime failure */
+#define MI_ERR_STATE    -3   /* call out of order (e.g. no VEGAS grid) */
+#define MI_ERR_RANGE    -4   /* size beyond a documented maximum */
+#define MI_ERR_NUMERIC  -5   /* non-finite result */
+#define MI_ERR_CAPACITY -6   /* output buffer too small (count still reported) */
+
+/* Pythia process codes (SigmaQCD.h): index i of per-process arrays <-> code 111 + i */
+#define MI_GG2GG          111
+#define MI_GG2QQBAR       112
+#define MI_QG2QG          113
+#define MI_QQ2QQ          114
+#define MI_QQBAR2GG       115
+#define MI_QQBAR2QQBARNEW 116
+#define MI_MAX_PROC       6
+
+enum mi_mode      { MI_MODE_PARTONIC_1D = 0, MI_MODE_HADRONIC_3D = 1 };
+enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1 };
+enum mi_sampler   { MI_SAMPLER_FLAT = 0, MI_SAMPLER_VEGAS = 1, MI_SAMPLER_FLOW = 2 };
+enum mi_rng       { MI_RNG_PHILOX = 0, MI_RNG_SPLITMIX_COMPAT = 1 };
+
+/* Physics configuration.  Defaults (mi_physics_default) mirror Pythia's settings where the
+ * workload does not fix them: SigmaProcess:alphaSorder = 1, StandardModel:alphaSnfmax = 6,
+ * SigmaProcess:renormScale2 = 2 (mu_R^2 = pT^2 for massless 2->2), HardQCD:nQuarkNew = 3,
+ * PDFinProcess:nQuarkIn = 5, SigmaProcess:Kfactor = 1. */
+typedef struct mi_physics {
+    int    mode;            /* enum mi_mode */
+    int    proc_mask;       /* bit i <-> code 111 + i (hadronic); partonic mode is gg->gg only */
+    /* partonic 1-D: fixed sHat, fixed alphaS, cos(theta) flat in [-cos_max, cos_max] */
+    double sHat;            /* GeV^2 */
+    double cos_max;
+    /* hadronic 3-D: (tau, y, z) with the pT cut */
+    double eCM;             /* sqrt(S), GeV */
+    double pTHatMin;        /* GeV */
+    double alphaS_value;    /* alphaS(MZ) for order 1; the fixed value for order 0 */
+    int    alphaS_order;    /* 0 fixed, 1 one-loop with flavour thresholds */
+    int    alphaS_nfmax;    /* 5 or 6 */
+    double renorm_fac;      /* mu_R = renorm_fac * pT */
+    int    nQuarkIn;        /* incoming quark flavours (2..5) */
+    int    nQuarkNew;       /* outgoing flavours for 112 / 116 */
+    double pdf_g[2];        /* toy xf = A x^a (1-x)^b exponents (a, b): gluon */
+    double pdf_val[2];      /* valence (u_v, d_v) */
+    double pdf_sea[2];      /* sea (every quark and antiquark) */
+    double kfactor;
+} mi_physics;
+
+typedef struct mi_run {
+    mi_physics phys;
+    int      precision;     /* enum mi_precision (FP32: ME in FP32, sums in FP64) */
+    int      sampler;       /* enum mi_sampler */
+    int      rng;           /* enum mi_rng */
+    uint64_t key;           /* Philox key (64 bit: key_lo, key_hi) or SplitMix seed */
+    uint64_t counter0;      /* first event index */
+    uint64_t n_points;      /* events (points) in this call */
+} mi_run;
+
+/* Pythia's sigma-estimate semantics.  Index 0 = sum over processes (Info::sigmaGen(0)),
+ * index 1 + i = code 111 + i.  For weighted integration nSel = nAcc = nTry; sigmaErr follows
+ * ProcessContainer::sigmaDelta.  The sum's error uses the per-point total weight (the
+ * processes share points), not the quadrature sum of independent containers. */
+typedef struct mi_sigma_est {
+    int      code[MI_MAX_PROC + 1];
+    double   sigmaGen[MI_MAX_PROC + 1];   /* mb */
+    double   sigmaErr[MI_MAX_PROC + 1];   /* mb */
+    double   sumW[MI_MAX_PROC + 1];
+    double   sumW2[MI_MAX_PROC + 1];
+    double   wMax[MI_MAX_PROC + 1];
+    int64_t  nTry, nSel, nAcc;
+    int64_t  nNonZero;                    /* points with total weight > 0 */
+    double   kernel_ms;                   /* device time of the integration kernel (hipEvent pair on the stream) */
+} mi_sigma_est;
+
+/* VEGAS adaptation summary (cumulative inverse-variance combination over the iterations). */
+typedef struct mi_vegas_info {
+    int      n_iter;
+    double   sigma, sigma_err, chi2_dof;  /* combined over the iterations */
+    double   last_sigma, last_err;        /* last iteration alone */
+    double   last_wmax, last_eta;         /* last iteration's max weight and <w>/wMax */
+    double   kernel_ms;
+} mi_vegas_info;
+
+/* Compacted unweighted events, structure of arrays, DEVICE pointers of length `capacity`.
+ * Accepted events keep the candidate order (stable filter). */
+typedef struct mi_events {
+    double   *sHat, *tHat, *x1, *x2, *weight;
+    int32_t  *pid;          /* Pythia process code */
+    uint64_t *index;        /* candidate event index (counter) */
+    uint64_t capacity;
+} mi_events;
+
+/* Gradients of sigma (sum over the selected processes), same points (common random numbers). */
+typedef struct mi_sigma_grad {
+    double sigma, sigma_err;
+    double d_alphaS_MZ, d_alphaS_MZ_err;
+    double d_renorm_fac, d_renorm_fac_err;
+    double d_pdf_g_a, d_pdf_g_a_err;
+    double d_pdf_g_b, d_pdf_g_b_err;
+    double kernel_ms;
+} mi_sigma_grad;
+
+/* Flow training / evaluation summary. */
+typedef struct mi_flow_info {
+    int      steps;
+    double   loss_first, loss_last;      /* variance loss <w^2>/<w>^2 - 1 on the training batches */
+    double   cv_vegas, cv_flow;          /* coefficient of variation of w before / after */
+    double   kernel_ms;
+} mi_flow_info;
+
+/* Flow architecture (fixed): 3-D, 6 rational-quadratic spline couplings with 16 bins on
+ * [0,1], each transforming one coordinate conditioned on the other two through an MLP
+ * 2 -> 64 -> 64 -> 49 (16 widths, 16 heights, 17 knot derivatives), ReLU, BF16 weights for
+ * the 64-wide layers with FP32 accumulation.  The flow maps the unit cube onto the VEGAS
+ * grid's unit cube: x_phys = VEGAS(flow(z)), z ~ U[0,1)^3. */
+#define MI_FLOW_LAYERS   6
+#define MI_FLOW_BINS     16
+#define MI_FLOW_HIDDEN   64
+#define MI_FLOW_NOUT     49
+#define MI_FLOW_NPARAMS  45222           /* 6 x (2*64+64 + 64*64+64 + 64*49+49) */
+
+/* ------------------------------------------------------------------ context ---- */
+int         mi_version(void);
+/* Create a context on HIP device `device`; owns its scratch buffers, VEGAS grid and flow. */
+int         mi_ctx_create(int device, mi_ctx **out);
+int         mi_ctx_destroy(mi_ctx *ctx);
+/* Message of the last failed call on ctx (never NULL). */
+const char *mi_last_error(const mi_ctx *ctx);
+/* Launch on this HIP stream (hipStream_t passed as void*; NULL = the null stream). */
+int         mi_ctx_set_stream(mi_ctx *ctx, void *stream);
+/* Fill *p with the workload defaults: which = 0 partonic gg->gg at sHat = (100 GeV)^2,
+ * |cos| < 0.9, alphaS = 0.118; which = 1 the 5-process library at sqrt(S) = 13.6 TeV,
+ * pT > 20 GeV, toy PDFs, 1-loop alphaS(MZ) = 0.118. */
+int         mi_physics_default(int which, mi_physics *p);
+
+/* --------------------------------------------------------------------- RNG ---- */
+/* Philox4x32-10 blocks for events counter0 .. counter0+n-1: ctr = (e_lo, e_hi, stream, 0),
+ * key = (key_lo, key_hi).  out_dev receives 4*n uint32 words.  (Test hook; the integration
+ * kernels generate the same words in registers.) */
+int mi_rng_philox_fill(mi_ctx *ctx, uint64_t key, uint64_t counter0, uint32_t stream,
+                       uint64_t n, uint32_t *out_dev);
+
+/* ---------------------------------------------------------------- sigmaHat ---- */
+/* Batched Kfactor * sigmaHatWrap(id1, id2) (mb/GeV^2) for code 111..116, FP64, massless.
+ * uH_dev may be NULL (then uH = -(sH + tH) as in Sigma2Process::store2Kin); id1/id2 may be
+ * NULL (only 114 depends on them; NULL means q q' with different flavours). */
+int mi_sigma_hat(mi_ctx *ctx, int code, uint64_t n, const double *sH_dev, const double *tH_dev,
+                 const double *uH_dev, const double *alpS_dev, const int32_t *id1_dev,
+                 const int32_t *id2_dev, int nQuarkNew, double kfactor, double *out_dev);
+
+/* ----------------------------------------------------------- integration ---- */
+/* Per-point weights (mb) of events counter0 .. counter0+n_points-1: w_dev is
+ * n x (MI_MAX_PROC + 1) doubles (column 0 = total, 1 + i = code 111 + i); kin_dev (may be
+ * NULL) is n x 4 doubles (sHat, tHat, x1, x2).  Uses run->sampler's map (flat or the
+ * context's VEGAS grid).  Parity hook for the integration kernels. */
+int mi_points(mi_ctx *ctx, const mi_run *run, double *w_dev, double *kin_dev);
+/* Sigma estimate over run->n_points events; fused RNG -> map -> PDFs x sigmaHat -> sum w,
+ * sum w^2, reduced deterministically on the device. */
+int mi_integrate(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out);
+
+/* ----------------------------------------------------------------- VEGAS ---- */
+/* Uniform grid with nbins (<= 256) bins per dimension (1 for partonic, 3 for hadronic). */
+int mi_vegas_init(mi_ctx *ctx, int ndim, int nbins);
+/* n_iter iterations of n_points each (counters counter0 + it * n_points ...): sample,
+ * accumulate per-bin sum w^2, refine on the device with damping alpha (Lepage 1978).
+ * The grid is used by every later call with sampler = MI_SAMPLER_VEGAS. */
+int mi_vegas_adapt(mi_ctx *ctx, const mi_run *run, int n_iter, double alpha, mi_vegas_info *out);
+/* Host copies of the grid edges: ndim x (nbins + 1) doubles. */
+int mi_vegas_get_grid(mi_ctx *ctx, double *edges_host, int *ndim, int *nbins);
+int mi_vegas_set_grid(mi_ctx *ctx, const double *edges_host, int ndim, int nbins);
+
+/* ------------------------------------------------------- unweighting ---- */
+/* Hit-or-miss unweighting of run->n_points candidates: accept iff u * wMax < w with u from
+ * the disjoint stream 1 of the same counter; accepted events carry weight max(w, wMax) and a
+ * process id drawn with probability w_p / w.  If wmax <= 0 the exact maximum over the
+ * candidates is used (one extra pass).  Writes the accepted events in candidate order;
+ * *n_acc is the accepted count even when it exceeds out->capacity (then MI_ERR_CAPACITY).
+ * est (may be NULL) receives the weighted sigma estimate of the same candidates with
+ * nAcc = accepted count. */
+int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_events *out,
+                        uint64_t *n_acc, double *wmax_used, mi_sigma_est *est);
+
+/* ---------------------------------------------------------------- adjoint ---- */
+/* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
+ * same points, FP64 accumulation.  Hadronic mode with alphaS_order 1 (or 0 for the fixed-
+ * coupling alphaS^2 check). */
+int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out);
+
+/* ------------------------------------------------------------------- flow ---- */
+/* Identity initialisation (last layer of every conditioner zero) with hidden layers drawn
+ * from Philox(seed); composes with the context's VEGAS grid (uniform grid if none). */
+int mi_flow_init(mi_ctx *ctx, uint64_t seed);
+/* Train with the variance loss on batches of `batch` self-generated points (counters from
+ * run->counter0), Adam with learning rate lr, `steps` steps, entirely on the device. */
+int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, double lr,
+                  mi_flow_info *out);
+/* Normalizing direction y -> z (y, z in [0,1)^3, n x 3 floats) with log|det dz/dy|. */
+int mi_flow_forward(mi_ctx *ctx, uint64_t n, const float *y_dev, float *z_dev, float *logdet_dev);
+/* Sampling direction z -> y with log|det dy/dz|. */
+int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, float *logdet_dev);
+/* grad_dev (MI_FLOW_NPARAMS floats, overwritten) = d/dtheta sum_i g_i * logdet_fwd(y_i). */
+int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g_dev,
+                     float *grad_dev);
+/* FP32 master copy of the parameters, layer-major: per layer W1[64][2], b1[64], W2[64][64],
+ * b2[64], W3[49][64], b3[49]. */
+int mi_flow_get_params(mi_ctx *ctx, float *params_host);
+int mi_flow_set_params(mi_ctx *ctx, const float *params_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

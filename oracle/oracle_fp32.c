@@ -1,0 +1,210 @@
+/*
+ * oracle_fp32.c -- bit-for-bit CPU restatement of the FP32 device arithmetic of the
+ * hadronic integrand.  TEST INFRASTRUCTURE ONLY (see mcint_oracle.h).
+ *
+ * Purpose: the unweighting decision u * wMax < w must come out identically on the CPU and
+ * on the GPU for the compacted event indices to be bit-exact, so the FP32 weight has to be
+ * reproducible bit for bit.  Both sides therefore use only correctly rounded IEEE
+ * operations (+ - * / sqrt fma rint and exact bit manipulation), the same operation order,
+ * no contraction (this file is compiled with -ffp-contract=off), and this project's own
+ * exp/log:
+ *   exp: Cody-Waite reduction x = n ln2 + r (ln2 split hi/lo), Taylor polynomial of e^r to
+ *        degree 7 on |r| <= ln2/2 (truncation 5e-9 relative), scaling by 2^n via the
+ *        exponent field (W. J. Cody, W. Waite, "Software Manual for the Elementary
+ *        Functions", 1980).
+ *   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd
+ *        series to s^9 (|s| <= 0.1716, truncation 2e-9 relative).
+ * The accuracy of the whole FP32 weight is pinned against the FP64 Pythia-form oracle
+ * (orc_had_point) in tests/test_oracle_physics.py.
+ */
+#include "mcint_oracle.h"
+#include <math.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static float i2f(int32_t i) { float f; memcpy(&f, &i, 4); return f; }
+static int32_t f2i(float f) { int32_t i; memcpy(&i, &f, 4); return i; }
+
+float orc_expf(float x)
+{
+    if (x < -86.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    float n = rintf(x * 1.4426950408889634f);
+    float r = fmaf(n, -0.693145751953125f, x);
+    r = fmaf(n, -1.4286068203094172e-6f, r);
+    float p = 1.9841269841269841e-4f;
+    p = fmaf(p, r, 1.3888888888888889e-3f);
+    p = fmaf(p, r, 8.3333333333333333e-3f);
+    p = fmaf(p, r, 4.1666666666666667e-2f);
+    p = fmaf(p, r, 1.6666666666666667e-1f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return p * i2f(((int32_t)n + 127) << 23);
+}
+
+/* 1 - e^x for x <= 0: Taylor series of -(e^x - 1) to x^8 when |x| <= 0.35 (no cancellation
+ * near x = 0), plain 1 - exp(x) otherwise (then 1 - e^x >= 0.295 and the subtraction is benign). */
+static float one_minus_exp(float x, float ex)
+{
+    if (x > -0.35f) {
+        float p = fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
+        p = fmaf(x, p, 1.3888888888888889e-3f);
+        p = fmaf(x, p, 8.3333333333333333e-3f);
+        p = fmaf(x, p, 4.1666666666666667e-2f);
+        p = fmaf(x, p, 1.6666666666666667e-1f);
+        p = fmaf(x, p, 0.5f);
+        p = fmaf(x, p, 1.0f);
+        return -(x * p);
+    }
+    return 1.0f - ex;
+}
+
+float orc_logf(float x)
+{
+    if (!(x > 0.0f)) return -1.0e30f;           /* only reached for 1 - x == 0 (x rounds to 1) */
+    int32_t ix = f2i(x);
+    int32_t e = ((ix >> 23) & 0xff) - 127;
+    float m = i2f((ix & 0x007fffff) | 0x3f800000);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    float f = m - 1.0f;
+    float s = f / (m + 1.0f);
+    float s2 = s * s;
+    float p = fmaf(s2, 0.11111111111111111f, 0.14285714285714286f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 0.33333333333333333f);
+    float two_s = s + s;
+    float lm = fmaf(two_s * s2, p, two_s);
+    float fe = (float)e;
+    return fmaf(fe, 0.693145751953125f, fmaf(fe, 1.4286068203094172e-6f, lm));
+}
+
+/* Constant block (all FP32):
+ *  0 LT = ln(1/tauMin)       1 fourPT2min = 4 pTmin^2     2 pref = CONV pi LT K / (4 pTmin^2)
+ *  3 xiR2 = renormFac^2      4 as_order (0/1)             5 as_fixed (order 0 value)
+ *  6 scale2Min               7 mt2  8 mb2  9 mc2          10 nf6 (1 if nfmax >= 6)
+ * 11 lnL6s 12 lnL5s 13 lnL4s 14 lnL3s                     15 c6 16 c5 17 c4 18 c3 (12 pi / b0)
+ * 19 lnAg 20 ag 21 bg        22 lnAs 23 as 24 bs          25 lnAuv 26 av 27 bv 28 rdu = Adv/Auv
+ * 29 nSea = 2 nQuarkIn       30 nQuarkNew                 31 unused
+ */
+void orc32_consts_fill(const orc_hadcfg *c, float k[ORC32_NCONST])
+{
+    double s = c->eCM * c->eCM;
+    double fourPT2 = 4.0 * c->pTHatMin * c->pTHatMin;
+    double LT = log(s / fourPT2);
+    for (int i = 0; i < ORC32_NCONST; ++i) k[i] = 0.0f;
+    k[0] = (float)LT;
+    k[1] = (float)fourPT2;
+    k[2] = (float)(ORC_CONVERT2MB * M_PI * LT * c->kfactor / fourPT2);
+    k[3] = (float)(c->renormFac * c->renormFac);
+    k[4] = (float)c->as.order;
+    k[5] = (float)c->as.valueRef;
+    k[6] = (float)c->as.scale2Min;
+    k[7] = (float)c->as.mt2; k[8] = (float)c->as.mb2; k[9] = (float)c->as.mc2;
+    k[10] = (c->as.nfmax >= 6) ? 1.0f : 0.0f;
+    if (c->as.order == 1) {
+        k[11] = (float)log(c->as.L6s); k[12] = (float)log(c->as.L5s);
+        k[13] = (float)log(c->as.L4s); k[14] = (float)log(c->as.L3s);
+    }
+    k[15] = (float)(12.0 * M_PI / 21.0); k[16] = (float)(12.0 * M_PI / 23.0);
+    k[17] = (float)(12.0 * M_PI / 25.0); k[18] = (float)(12.0 * M_PI / 27.0);
+    k[19] = (float)log(c->pdf.Ag);  k[20] = (float)c->pdf.ag; k[21] = (float)c->pdf.bg;
+    k[22] = (float)log(c->pdf.As);  k[23] = (float)c->pdf.as; k[24] = (float)c->pdf.bs;
+    k[25] = (float)log(c->pdf.Auv); k[26] = (float)c->pdf.av; k[27] = (float)c->pdf.bv;
+    k[28] = (float)(c->pdf.Adv / c->pdf.Auv);
+    k[29] = (float)(2 * c->nQuarkIn);
+    k[30] = (float)c->nQuarkNew;
+}
+
+/* The device kernel's per-point arithmetic, restated operation by operation. */
+float orc32_had_point(const float k[ORC32_NCONST], int procMask, float u0, float u1, float u2,
+                      float w[6], float kin[4])
+{
+    return orc32_had_point_om(k, procMask, u0, 1.0f - u0, u1, 1.0f - u1, u2, 1.0f - u2, w, kin);
+}
+
+float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, float om0, float u1,
+                         float om1, float u2, float om2, float w[6], float kin[4])
+{
+    const float LT = k[0];
+    float r = orc_expf(-(u0 * LT));                    /* tauMin / tau */
+    float yMax = 0.5f * (LT * om0);
+    /* ln x1 = y - yMax = -2 yMax (1 - u1), ln x2 = -y - yMax = -2 yMax u1: no cancellation */
+    float twoYMax = yMax + yMax;
+    float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
+    float x1 = orc_expf(lx1), x2 = orc_expf(lx2);
+    float zMax = sqrtf(one_minus_exp(-(u0 * LT), r));
+    float omz = r / (1.0f + zMax);                      /* 1 - zMax, cancellation-free */
+    float twoz = zMax + zMax;
+    float A = fmaf(twoz, om2, omz);                     /* 1 - z */
+    float B = fmaf(twoz, u2, omz);                      /* 1 + z */
+    float a = 0.5f * A, b = 0.5f * B;                   /* -t/s, -u/s */
+    float ab = a * b;
+    float inv_ab = 1.0f / ab;
+    float ia = b * inv_ab, ib = a * inv_ab;
+    float sH = k[1] / r;
+    float pT2 = sH * ab;
+
+    float alpS;
+    if (k[4] == 0.0f) {
+        alpS = k[5];
+    } else {
+        float Q2 = k[3] * pT2;
+        if (Q2 < k[6]) Q2 = k[6];
+        float lq = orc_logf(Q2);
+        float lam, cf;
+        if (Q2 > k[7] && k[10] != 0.0f) { lam = k[11]; cf = k[15]; }
+        else if (Q2 > k[8])             { lam = k[12]; cf = k[16]; }
+        else if (Q2 > k[9])             { lam = k[13]; cf = k[17]; }
+        else                            { lam = k[14]; cf = k[18]; }
+        alpS = cf / (lq - lam);
+    }
+
+    /* parton densities at x1 and x2 */
+    float l1a = orc_logf(one_minus_exp(lx1, x1)), l1b = orc_logf(one_minus_exp(lx2, x2));
+    float g1 = orc_expf(fmaf(k[21], l1a, fmaf(k[20], lx1, k[19])));
+    float g2 = orc_expf(fmaf(k[21], l1b, fmaf(k[20], lx2, k[19])));
+    float S1 = orc_expf(fmaf(k[24], l1a, fmaf(k[23], lx1, k[22])));
+    float S2 = orc_expf(fmaf(k[24], l1b, fmaf(k[23], lx2, k[22])));
+    float V1 = orc_expf(fmaf(k[27], l1a, fmaf(k[26], lx1, k[25])));
+    float V2 = orc_expf(fmaf(k[27], l1b, fmaf(k[26], lx2, k[25])));
+    float Vd1 = k[28] * V1, Vd2 = k[28] * V2;
+    float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+    float Q1 = fmaf(k[29], S1, V1 + Vd1), Q2s = fmaf(k[29], S2, V2 + Vd2);
+    float SS = S1 * S2;
+    float Lgg = g1 * g2;
+    float Lqg = fmaf(Q1, g2, g1 * Q2s);
+    float LQQ = Q1 * Q2s;
+    float Lsame = fmaf(U1, U2, fmaf(D1, D2, (k[29] - 2.0f) * SS));
+    float Lopp = fmaf(U1 + D1, S2, fmaf(S1, U2 + D2, (k[29] - 4.0f) * SS));
+
+    /* brackets: d sigma / dt = pi alpS^2 / sH^2 * bracket */
+    float q2 = fmaf(a, a, b * b);
+    float common = k[2] * (alpS * alpS) * (LT * om0) * zMax * r;
+    float ws[6];
+    ws[0] = Lgg * (2.25f * fmaf(b * ia, ia, fmaf(a * ib, ib, 3.0f - ab)));
+    ws[1] = Lgg * (k[30] * q2 * fmaf(inv_ab, 0.16666666666666667f, -0.375f));
+    ws[2] = Lqg * (fmaf(b, b, 1.0f) * fmaf(ia, ia, 0.44444444444444444f * ib));
+    {
+        float sigT = 0.44444444444444444f * fmaf(b, b, 1.0f) * (ia * ia);
+        float sigU = 0.44444444444444444f * fmaf(a, a, 1.0f) * (ib * ib);
+        float sigTU = -0.29629629629629630f * inv_ab;
+        float sigST = 0.29629629629629630f * (b * b) * ia;
+        float csame = 0.5f * (sigU + sigTU - sigT);
+        ws[3] = fmaf(sigT, LQQ, fmaf(csame, Lsame, sigST * Lopp));
+    }
+    ws[4] = Lopp * (q2 * fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
+    ws[5] = Lopp * (k[30] * 0.44444444444444444f * q2);
+    float tot = 0.0f;
+    for (int i = 0; i < 6; ++i) {
+        float wi = (procMask & (1 << i)) ? common * ws[i] : 0.0f;
+        if (!(wi > 0.0f)) wi = 0.0f;
+        w[i] = wi;
+        tot += wi;
+    }
+    if (kin) { kin[0] = sH; kin[1] = -(sH * a); kin[2] = x1; kin[3] = x2; }
+    return tot;
+}

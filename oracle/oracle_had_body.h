@@ -1,0 +1,197 @@
+/*
+ * oracle_had_body.h -- Pythia-form hadronic point, instantiated by oracle_physics.c twice:
+ * REAL = double (suffix _d) and REAL = long double (suffix _ld).  TEST INFRASTRUCTURE ONLY.
+ *
+ * The FP64 instantiation is Pythia's arithmetic as written; near the pT cut at large sHat it
+ * loses up to ~1e-10 relative in 1 - z, 1 - x and 1 - pT2Min/p2Abs (all computed by
+ * subtraction, as Pythia does).  The 80-bit instantiation runs the same steps with ~1e-19
+ * rounding and is the per-point reference for the FP64 kernels.
+ *
+ * Requires: REAL, FN(name), M_EXP, M_LOG, M_POW, M_SQRT, M_FMIN, M_FMAX.
+ */
+
+/* sigmaHat, Pythia's literal expressions (GeV^-2), massless:
+ *   111 Sigma2gg2gg::sigmaKin           SigmaQCD.cc:115-129
+ *   112 Sigma2gg2qqbar::sigmaKin        SigmaQCD.cc:170-189 (threshold open)
+ *   113 Sigma2qg2qg::sigmaKin           SigmaQCD.cc:216-226
+ *   114 Sigma2qq2qq::sigmaKin/sigmaHat  SigmaQCD.cc:257-286
+ *   115 Sigma2qqbar2gg::sigmaKin        SigmaQCD.cc:315-325
+ *   116 Sigma2qqbar2qqbarNew::sigmaKin  SigmaQCD.cc:364-378 (threshold open) */
+static REAL FN(sigma_hat)(int code, REAL sH, REAL tH, REAL uH, REAL alpS, int id1, int id2, int nQuarkNew)
+{
+    const REAL pi = (REAL)3.14159265358979323846264338327950288L;
+    REAL sH2 = sH * sH, tH2 = tH * tH, uH2 = uH * uH;
+    REAL pre = (pi / sH2) * (alpS * alpS);
+    switch (code) {
+    case ORC_GG2GG: {
+        REAL sigTS = ((REAL)9 / 4) * (tH2 / sH2 + 2 * tH / sH + 3 + 2 * sH / tH + sH2 / tH2);
+        REAL sigUS = ((REAL)9 / 4) * (uH2 / sH2 + 2 * uH / sH + 3 + 2 * sH / uH + sH2 / uH2);
+        REAL sigTU = ((REAL)9 / 4) * (tH2 / uH2 + 2 * tH / uH + 3 + 2 * uH / tH + uH2 / tH2);
+        return pre * (REAL)0.5 * (sigTS + sigUS + sigTU);
+    }
+    case ORC_GG2QQBAR: {
+        REAL sigTS = ((REAL)1 / 6) * uH / tH - ((REAL)3 / 8) * uH2 / sH2;
+        REAL sigUS = ((REAL)1 / 6) * tH / uH - ((REAL)3 / 8) * tH2 / sH2;
+        return pre * nQuarkNew * (sigTS + sigUS);
+    }
+    case ORC_QG2QG: {
+        REAL sigTS = uH2 / tH2 - ((REAL)4 / 9) * uH / sH;
+        REAL sigTU = sH2 / tH2 - ((REAL)4 / 9) * sH / uH;
+        return pre * (sigTS + sigTU);
+    }
+    case ORC_QQ2QQ: {
+        REAL sigT  = ((REAL)4 / 9) * (sH2 + uH2) / tH2;
+        REAL sigU  = ((REAL)4 / 9) * (sH2 + tH2) / uH2;
+        REAL sigTU = -((REAL)8 / 27) * sH2 / (tH * uH);
+        REAL sigST = -((REAL)8 / 27) * uH2 / (sH * tH);
+        REAL sigSum;
+        if (id2 == id1)       sigSum = (REAL)0.5 * (sigT + sigU + sigTU);
+        else if (id2 == -id1) sigSum = sigT + sigST;
+        else                  sigSum = sigT;
+        return pre * sigSum;
+    }
+    case ORC_QQBAR2GG: {
+        REAL sigTS = ((REAL)32 / 27) * uH / tH - ((REAL)8 / 3) * uH2 / sH2;
+        REAL sigUS = ((REAL)32 / 27) * tH / uH - ((REAL)8 / 3) * tH2 / sH2;
+        return pre * (REAL)0.5 * (sigTS + sigUS);
+    }
+    case ORC_QQBAR2QQBARNEW:
+        return pre * nQuarkNew * ((REAL)4 / 9) * (tH2 + uH2) / sH2;
+    default:
+        return 0;
+    }
+}
+
+/* AlphaStrong::alphaS, orders 0/1 (StandardModel.cc:188-215). */
+static REAL FN(alphas)(const orc_alphas *a, REAL scale2)
+{
+    const REAL pi = (REAL)3.14159265358979323846264338327950288L;
+    if (a->order == 0) return (REAL)a->valueRef;
+    if (scale2 < (REAL)a->scale2Min) scale2 = (REAL)a->scale2Min;
+    if (scale2 > (REAL)a->mt2 && a->nfmax >= 6) return 12 * pi / (21 * M_LOG(scale2 / (REAL)a->L6s));
+    if (scale2 > (REAL)a->mb2) return 12 * pi / (23 * M_LOG(scale2 / (REAL)a->L5s));
+    if (scale2 > (REAL)a->mc2) return 12 * pi / (25 * M_LOG(scale2 / (REAL)a->L4s));
+    return 12 * pi / (27 * M_LOG(scale2 / (REAL)a->L3s));
+}
+
+/* Toy xf(x): u = uv + sea, d = dv + sea, other quarks / antiquarks = sea, 21 = gluon. */
+static REAL FN(xf)(const orc_toypdf *p, int id, REAL x)
+{
+    if (x <= 0 || x >= 1) return 0;
+    int aid = id < 0 ? -id : id;
+    if (id == 21) return (REAL)p->Ag * M_POW(x, (REAL)p->ag) * M_POW(1 - x, (REAL)p->bg);
+    if (aid == 0 || aid > p->nQuarkIn) return 0;
+    REAL sea = (REAL)p->As * M_POW(x, (REAL)p->as) * M_POW(1 - x, (REAL)p->bs);
+    if (id == 2) return sea + (REAL)p->Auv * M_POW(x, (REAL)p->av) * M_POW(1 - x, (REAL)p->bv);
+    if (id == 1) return sea + (REAL)p->Adv * M_POW(x, (REAL)p->av) * M_POW(1 - x, (REAL)p->bv);
+    return sea;
+}
+
+static void FN(fill_xf)(const orc_hadcfg *c, REAL x, REAL *xf)
+{
+    int nQ = c->nQuarkIn;
+    for (int id = -nQ; id <= nQ; ++id) xf[id + nQ] = (id == 0) ? 0 : FN(xf)(&c->pdf, id, x);
+    xf[2 * nQ + 1] = FN(xf)(&c->pdf, 21, x);
+}
+
+/* sigmaPDF for one code: Pythia's incoming-pair list for the code's flux (initFlux,
+ * SigmaProcess.cc:124-193), Kfactor * sigmaHatWrap * xfA * xfB summed (sigmaPDF :423-480). */
+static REAL FN(sigma_pdf)(const orc_hadcfg *c, int code, REAL sH, REAL tH, REAL uH, REAL alpS,
+                          const REAL *xfA, const REAL *xfB)
+{
+    int nQ = c->nQuarkIn, ig = 2 * nQ + 1;
+    REAL K = (REAL)c->kfactor * (REAL)ORC_CONVERT2MB;
+#define XF(arr, id) ((id) == 21 ? (arr)[ig] : (arr)[(id) + nQ])
+    REAL sum = 0;
+    if (code == ORC_GG2GG || code == ORC_GG2QQBAR) {                    /* "gg" */
+        sum = K * FN(sigma_hat)(code, sH, tH, uH, alpS, 21, 21, c->nQuarkNew) * XF(xfA, 21) * XF(xfB, 21);
+    } else if (code == ORC_QG2QG) {                                     /* "qg" */
+        for (int id = -nQ; id <= nQ; ++id) {
+            if (id == 0) continue;
+            sum += K * FN(sigma_hat)(code, sH, tH, uH, alpS, id, 21, c->nQuarkNew) * XF(xfA, id) * XF(xfB, 21);
+            sum += K * FN(sigma_hat)(code, sH, tH, uH, alpS, 21, id, c->nQuarkNew) * XF(xfA, 21) * XF(xfB, id);
+        }
+    } else if (code == ORC_QQ2QQ) {                                     /* "qq" */
+        for (int i1 = -nQ; i1 <= nQ; ++i1) {
+            if (i1 == 0) continue;
+            for (int i2 = -nQ; i2 <= nQ; ++i2) {
+                if (i2 == 0) continue;
+                sum += K * FN(sigma_hat)(code, sH, tH, uH, alpS, i1, i2, c->nQuarkNew) * XF(xfA, i1) * XF(xfB, i2);
+            }
+        }
+    } else if (code == ORC_QQBAR2GG || code == ORC_QQBAR2QQBARNEW) {   /* "qqbarSame" */
+        for (int id = -nQ; id <= nQ; ++id) {
+            if (id == 0) continue;
+            sum += K * FN(sigma_hat)(code, sH, tH, uH, alpS, id, -id, c->nQuarkNew) * XF(xfA, id) * XF(xfB, -id);
+        }
+    }
+#undef XF
+    return sum;
+}
+
+/* One point: selectTau (PhaseSpace.cc:1256-1337, iTau = 0), y flat in [-yMax, yMax]
+ * (limitY :1183-1200, wtY = 2 yMax), limitZ/selectZ (:1205-1250, :1453-1568, iZ = 0 with its
+ * neg/pos split), store2Kin (SigmaProcess.cc:846-925: massless pT2 = tH uH / sH,
+ * Q2Ren = renormMultFac pT2), weight = sigmaPDF * wtTau * wtY * wtZ (PhaseSpace.cc:1074). */
+static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[6], REAL kin[8], int literalU)
+{
+    for (int i = 0; i < 6; ++i) w[i] = 0;
+    REAL u0 = (REAL)uin[0], u1 = (REAL)uin[1], u2 = (REAL)uin[2];
+    REAL s = (REAL)c->eCM * (REAL)c->eCM;
+    REAL pT2Min = (REAL)c->pTHatMin * (REAL)c->pTHatMin;
+    REAL tauMin = 4 * pT2Min / s;                       /* (mT3Min + mT4Min)^2 / s, massless */
+    REAL tauMax = 1;
+    REAL intTau0 = M_LOG(tauMax / tauMin);
+    REAL tau = tauMin * M_POW(tauMax / tauMin, u0);
+    REAL wtTau = intTau0;
+
+    REAL yMax = -(REAL)0.5 * M_LOG(tau);
+    REAL y = yMax * (2 * u1 - 1);
+    REAL wtY = 2 * yMax;
+    REAL x1 = M_SQRT(tau) * M_EXP(y);
+    REAL x2 = M_SQRT(tau) * M_EXP(-y);
+
+    REAL sH = tau * s;
+    REAL mHat = M_SQRT(sH);
+    REAL p2Abs = (REAL)0.25 * sH * sH / sH;
+    REAL pAbs = M_SQRT(p2Abs);
+    REAL zArg = 1 - pT2Min / p2Abs;
+    REAL zMax = zArg > 0 ? M_SQRT(zArg) : 0;
+    REAL zMin = 0;
+    REAL zNegMin = -zMax, zNegMax = -zMin, zPosMin = zMin, zPosMax = zMax;
+    REAL area0Neg = zNegMax - zNegMin, area0Pos = zPosMax - zPosMin;
+    REAL area0 = area0Neg + area0Pos;
+    REAL z;
+    if (u2 * area0 < area0Neg) z = zNegMin + (u2 * area0 / area0Neg) * area0Neg;
+    else                       z = zPosMin + ((u2 * area0 - area0Neg) / area0Pos) * area0Pos;
+    if (z < 0) z = M_FMIN(zNegMax, M_FMAX(zNegMin, z));
+    else       z = M_FMIN(zPosMax, M_FMAX(zPosMin, z));
+    REAL wtZ = mHat * pAbs * area0;                    /* zCoef[0] = 1 */
+
+    REAL sH34 = -(REAL)0.5 * sH;
+    REAL tHuH = sH34 * sH34 * (1 - z) * (1 + z);
+    REAL tH, uH;
+    if (z < 0) { tH = sH34 + mHat * pAbs * z; uH = tHuH / tH; }
+    else       { uH = sH34 - mHat * pAbs * z; tH = tHuH / uH; }
+    if (literalU) uH = -(sH + tH);
+
+    REAL pT2 = tH * uH / sH;
+    REAL Q2Ren = (REAL)c->renormFac * (REAL)c->renormFac * pT2;
+    REAL alpS = FN(alphas)(&c->as, Q2Ren);
+
+    REAL xfA[16], xfB[16];
+    FN(fill_xf)(c, x1, xfA);
+    FN(fill_xf)(c, x2, xfB);
+
+    REAL jac = wtTau * wtY * wtZ;
+    REAL total = 0;
+    for (int i = 0; i < 6; ++i) {
+        if (!(c->procMask & (1 << i))) continue;
+        REAL wi = FN(sigma_pdf)(c, 111 + i, sH, tH, uH, alpS, xfA, xfB) * jac;
+        if (wi < 0) wi = 0;                              /* PhaseSpace.cc:1135 */
+        w[i] = wi;
+        total += wi;
+    }
+    kin[0] = sH; kin[1] = tH; kin[2] = uH; kin[3] = x1; kin[4] = x2; kin[5] = pT2; kin[6] = alpS; kin[7] = jac;
+    return total;
+}
