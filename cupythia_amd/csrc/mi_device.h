@@ -131,7 +131,7 @@ __device__ __forceinline__ float own_logf(float x)
 }
 
 // ------------------------------------------------------------------ constants of the integrand
-// Layout identical to the host-side fill (mi_host.cpp: fill_had_consts).
+// Filled on the host by fill_consts (mi_api.hip).
 template <typename T>
 struct HadConst {
     T LT;          // ln(1 / tauMin)
@@ -150,6 +150,10 @@ struct HadConst {
 };
 static_assert(sizeof(HadConst<float>) == 32 * sizeof(float), "HadConst layout");
 
+// Intermediate values the adjoint pass needs from a point (see mi_adjoint.hip).
+template <typename T>
+struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
+
 // ------------------------------------------------------------------ FP32 hadronic point
 // Inputs: (u, 1-u) for each dimension (exact pairs); writes ws[i] = weight of code 111+i
 // (0 if masked out), returns the total.  Operation-for-operation identical to the host
@@ -157,7 +161,7 @@ static_assert(sizeof(HadConst<float>) == 32 * sizeof(float), "HadConst layout");
 __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
                                                float u0, float om0, float u1, float om1,
                                                float u2, float om2, float* ws, float* sHo, float* tHo,
-                                               float* x1o, float* x2o)
+                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr)
 {
     const float LT = k.LT;
     const float targ = -(u0 * LT);
@@ -178,7 +182,7 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     const float sH = k.fourPT2 / r;
     const float pT2 = sH * ab;
 
-    float alpS;
+    float alpS, cfo = 0.0f;
     if (k.as_order == 0.0f) {
         alpS = k.as_fixed;
     } else {
@@ -191,6 +195,7 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
         else if (Q2 > k.mc2)             { lam = k.lnL4s; cf = k.c4; }
         else                             { lam = k.lnL3s; cf = k.c3; }
         alpS = cf / (lq - lam);
+        cfo = cf;
     }
 
     const float l1a = own_logf(one_minus_expf(lx1, x1)), l1b = own_logf(one_minus_expf(lx2, x2));
@@ -235,6 +240,10 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
         tot += wi;
     }
     if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    if (aux) {
+        aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
+        aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
+    }
     return tot;
 }
 
@@ -244,7 +253,8 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
 __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int mask,
                                                 double u0, double om0, double u1, double om1,
                                                 double u2, double om2, double* ws, double* sHo,
-                                                double* tHo, double* x1o, double* x2o)
+                                                double* tHo, double* x1o, double* x2o,
+                                                PointAux<double>* aux = nullptr)
 {
     const double LT = k.LT;
     const double targ = -(u0 * LT);
@@ -265,7 +275,7 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
     const double sH = k.fourPT2 / r;
     const double pT2 = sH * ab;
 
-    double alpS;
+    double alpS, cfo = 0.0;
     if (k.as_order == 0.0) {
         alpS = k.as_fixed;
     } else {
@@ -278,6 +288,7 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
         else if (Q2 > k.mc2)            { lam = k.lnL4s; cf = k.c4; }
         else                            { lam = k.lnL3s; cf = k.c3; }
         alpS = cf / (lq - lam);
+        cfo = cf;
     }
 
     const double l1a = log(-expm1(lx1)), l1b = log(-expm1(lx2));
@@ -322,6 +333,10 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
         tot += wi;
     }
     if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    if (aux) {
+        aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
+        aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
+    }
     return tot;
 }
 

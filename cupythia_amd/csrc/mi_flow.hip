@@ -1,0 +1,946 @@
+// mi_flow.hip -- spline-coupling normalizing flow layered on the VEGAS map: forward
+// (normalizing, y -> z, log|det dz/dy|), inverse (sampling, z -> y), backward (parameter
+// gradients of sum_i g_i log q(y_i)), variance-loss training with Adam, and the flow sampler of
+// the integration -- all on the device.
+//
+// Architecture: 3-D, 6 rational-quadratic spline couplings on [0,1] (Durkan, Bekasov, Murray,
+// Papamakarios, "Neural Spline Flows", NeurIPS 2019) with 16 bins; layer l transforms
+// coordinate l % 3 conditioned on the other two through an MLP 2 -> 64 -> 64 -> 49 (ReLU):
+// 16 width and 16 height logits (softmax, minimum 1e-3) and 17 knot derivatives
+// (1e-3 + softplus).  Weights are stored as BF16 and the two 64-wide layers run as
+// 64x64x64 tiles on the matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation); the
+// 2 -> 64 layer and the splines run on VALU in FP32.
+//
+// Execution: one wave (64 threads) per block owns 64 points per tile and walks tiles with a
+// grid stride; the layer's packed weights (17.6 KB) are staged in LDS once per block; one
+// launch per layer, coordinates pass through HBM between layers (16 B per point per layer).
+//
+// Training (i-flow / neural importance sampling with the variance loss): points y = T^-1(z)
+// from the current flow, w = f(V(y)) J_V(y) |dy/dz|; the gradient of E_q[w^2] is
+// -E_q[w^2 grad log q(y)] (score form, samples held fixed), so the loss surrogate is
+// sum_i g_i log q(y_i) with g_i = -w_i^2 / (N <w>^2).  The initial flow is the identity (last
+// layers zero), so flow + VEGAS starts as VEGAS; the weight f / q stays the unbiased estimator.
+#include "mi_kernel_common.h"
+
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+namespace mi {
+
+constexpr int FL = MI_FLOW_LAYERS, FK = MI_FLOW_BINS, FH = MI_FLOW_HIDDEN, FO = MI_FLOW_NOUT;
+constexpr int FPL = MI_FLOW_NPARAMS / MI_FLOW_LAYERS;   // 7537 parameters per layer
+constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
+static_assert(PB3 + FO == FPL, "flow parameter layout");
+constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
+constexpr int HS = 72;      // LDS row stride (bf16) of hidden activations
+constexpr int OS = 68;      // LDS row stride (fp32) of outputs / gradients
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Per-layer weights in the layout the kernels copy into LDS.
+struct PackedLayer {
+    float W1[2 * FH];          // BF16-rounded values, [j][i]
+    float b1[FH];
+    float b2[FH];
+    float b3[64];              // 49 used, rest 0
+    uint16_t W2[FH * FH];      // BF16 [out][in]
+    uint16_t W3[64 * FH];      // BF16 [o][h], rows 49..63 zero
+};
+static_assert(sizeof(PackedLayer) % 16 == 0, "PackedLayer copies as uint4");
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);            // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t tobits(uint16_t v) { return v; }
+__device__ __forceinline__ uint16_t tobits(float v) { return f2bf(v); }
+
+template <typename E>
+__device__ __forceinline__ bf16x8 frag8(const E *p, int stride)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, tobits(p[j * stride]));
+    return r;
+}
+
+struct Acc { f32x16 c[2][2]; };
+
+__device__ __forceinline__ void acc_zero(Acc &a)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) a.c[i][j][v] = 0.0f;
+}
+
+// acc += A (64 x 64) * B (64 x 64), element (m, k) of A at A[m*a_sm + k*a_sk], (k, n) of B at
+// B[k*b_sk + n*b_sn].  32x32x16 MFMA operand layout: lane l holds A[l%32][8*(l/32) .. +7] and
+// B[8*(l/32) .. +7][l%32]; the result element (m, n) of a 32x32 block sits in lane
+// n + 32*((m/4)%2), register 4*(m/8) + m%4.
+template <typename EA, typename EB>
+__device__ __forceinline__ void wgemm(Acc &acc, const EA *A, int a_sm, int a_sk, const EB *B, int b_sk, int b_sn)
+{
+    const int l = threadIdx.x & 63, r = l & 31, hh = l >> 5;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k0 = 16 * kk + 8 * hh;
+        const bf16x8 a0 = frag8(A + r * a_sm + k0 * a_sk, a_sk);
+        const bf16x8 a1 = frag8(A + (32 + r) * a_sm + k0 * a_sk, a_sk);
+        const bf16x8 b0 = frag8(B + k0 * b_sk + r * b_sn, b_sk);
+        const bf16x8 b1 = frag8(B + k0 * b_sk + (32 + r) * b_sn, b_sk);
+        acc.c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc.c[0][0], 0, 0, 0);
+        acc.c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc.c[0][1], 0, 0, 0);
+        acc.c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc.c[1][0], 0, 0, 0);
+        acc.c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc.c[1][1], 0, 0, 0);
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void acc_each(const Acc &a, F fn)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int m = 32 * mi + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+                const int n = 32 * ni + (l & 31);
+                fn(m, n, a.c[mi][ni][v]);
+            }
+}
+
+__device__ __forceinline__ void stage_layer(PackedLayer &dst, const PackedLayer *src)
+{
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    uint4 *d = reinterpret_cast<uint4 *>(&dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(PackedLayer) / 16); i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ int cond0(int d) { return d == 0 ? 1 : 0; }
+__device__ __forceinline__ int cond1(int d) { return d == 2 ? 1 : 2; }
+
+// MLP of one 64-point tile: row p of H1 from lane p's conditioner (c0, c1), then the two MFMA
+// layers; writes O[p][0..63] (49 used) as FP32.
+__device__ __forceinline__ void mlp_tile(const PackedLayer &P, float c0, float c1, uint16_t *H1, uint16_t *H2,
+                                         float *Ob)
+{
+    const int p = threadIdx.x;
+#pragma unroll 8
+    for (int j = 0; j < FH; ++j) {
+        const float h = fmaxf(__builtin_fmaf(P.W1[2 * j], c0, __builtin_fmaf(P.W1[2 * j + 1], c1, P.b1[j])), 0.0f);
+        H1[p * HS + j] = f2bf(h);
+    }
+    __syncthreads();
+    Acc a;
+    acc_zero(a);
+    wgemm(a, H1, HS, 1, P.W2, 1, FH);
+    acc_each(a, [&](int m, int n, float v) { H2[m * HS + n] = f2bf(fmaxf(v + P.b2[n], 0.0f)); });
+    __syncthreads();
+    acc_zero(a);
+    wgemm(a, H2, HS, 1, P.W3, 1, FH);
+    acc_each(a, [&](int m, int n, float v) { Ob[m * OS + n] = v + P.b3[n]; });
+    __syncthreads();
+}
+
+// Spline parameters of one point from its 49 outputs: softmax widths / heights, softplus knot
+// derivatives; locate the bin of x (forward) or y (inverse).
+struct Bin { int k; float xk, wk, yk, hk, dk, dk1; };
+
+__device__ __forceinline__ float softplusf(float x) { return fmaxf(x, 0.0f) + log1pf(expf(-fabsf(x))); }
+
+__device__ __forceinline__ Bin locate(const float *o, float v, bool by_y)
+{
+    float mw = o[0], mh = o[FK];
+#pragma unroll
+    for (int i = 1; i < FK; ++i) { mw = fmaxf(mw, o[i]); mh = fmaxf(mh, o[FK + i]); }
+    float sw = 0.0f, sh = 0.0f;
+#pragma unroll
+    for (int i = 0; i < FK; ++i) { sw += expf(o[i] - mw); sh += expf(o[FK + i] - mh); }
+    const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
+    Bin b;
+    b.k = 0; b.xk = 0.0f; b.yk = 0.0f;
+    b.wk = kMinW + cw * (expf(o[0] - mw) / sw);
+    b.hk = kMinH + ch * (expf(o[FK] - mh) / sh);
+    float cx = 0.0f, cy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < FK; ++i) {
+        const float wi = kMinW + cw * (expf(o[i] - mw) / sw);
+        const float hi = kMinH + ch * (expf(o[FK + i] - mh) / sh);
+        if (i > 0 && (by_y ? v >= cy : v >= cx)) { b.k = i; b.xk = cx; b.yk = cy; b.wk = wi; b.hk = hi; }
+        cx += wi;
+        cy += hi;
+    }
+    b.dk = kMinD + softplusf(o[2 * FK + b.k]);
+    b.dk1 = kMinD + softplusf(o[2 * FK + b.k + 1]);
+    return b;
+}
+
+// Rational-quadratic map of bin b at x: returns S(x), writes log S'(x).
+__device__ __forceinline__ float rq_forward(const Bin &b, float x, float *logd)
+{
+    const float xi = (x - b.xk) / b.wk;
+    const float s = b.hk / b.wk;
+    const float t = xi * (1.0f - xi);
+    const float den = s + (b.dk1 + b.dk - 2.0f * s) * t;
+    const float y = b.yk + b.hk * (s * xi * xi + b.dk * t) / den;
+    const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
+    *logd = 2.0f * logf(s) + logf(n2) - 2.0f * logf(den);
+    return y;
+}
+
+// Inverse of the bin's map at y (root of the quadratic), with log S'(x) at the root.
+__device__ __forceinline__ float rq_inverse(const Bin &b, float y, float *logd)
+{
+    const float s = b.hk / b.wk;
+    const float dy = y - b.yk;
+    const float A = b.dk1 + b.dk - 2.0f * s;
+    const float qa = b.hk * (s - b.dk) + dy * A;
+    const float qb = b.hk * b.dk - dy * A;
+    const float qc = -s * dy;
+    const float disc = fmaxf(qb * qb - 4.0f * qa * qc, 0.0f);
+    float xi = (2.0f * qc) / (-qb - sqrtf(disc));
+    xi = fminf(fmaxf(xi, 0.0f), 1.0f);
+    const float t = xi * (1.0f - xi);
+    const float den = s + A * t;
+    const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
+    *logd = 2.0f * logf(s) + logf(n2) - 2.0f * logf(den);
+    return b.xk + xi * b.wk;
+}
+
+// ------------------------------------------------------------------------------------ forward / inverse
+// One layer over n points: xin / xout are n x 3 (may alias), ld (n) accumulates +log S' (forward)
+// or -log S' (inverse).
+template <bool INVERSE>
+__global__ __launch_bounds__(64) void k_flow_layer(const PackedLayer *__restrict__ layers, int l, uint64_t n,
+                                                   const float *__restrict__ xin, float *__restrict__ xout,
+                                                   float *__restrict__ ld)
+{
+    __shared__ PackedLayer P;
+    __shared__ uint16_t H1[64 * HS], H2[64 * HS];
+    __shared__ float Ob[64 * OS];
+    stage_layer(P, layers + l);
+    const int d = l % 3, ca = cond0(d), cb = cond1(d);
+    const uint64_t ntiles = (n + 63) / 64;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t i = tile * 64 + threadIdx.x;
+        const bool live = i < n;
+        float x[3] = {0.5f, 0.5f, 0.5f};
+        if (live) { x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2]; }
+        mlp_tile(P, x[ca], x[cb], H1, H2, Ob);
+        const float *o = Ob + threadIdx.x * OS;
+        float lj;
+        if (!INVERSE) {
+            const Bin b = locate(o, x[d], false);
+            x[d] = rq_forward(b, x[d], &lj);
+        } else {
+            const Bin b = locate(o, x[d], true);
+            x[d] = rq_inverse(b, x[d], &lj);
+            lj = -lj;
+        }
+        if (live) {
+            xout[3 * i] = x[0]; xout[3 * i + 1] = x[1]; xout[3 * i + 2] = x[2];
+            ld[i] += lj;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------ backward
+// Adjoint of one layer for L = sum_i (abar_out_i . x_out_i + g_i log S'_i) summed with the
+// later layers' contributions: abar (n x 3) holds dL/d(output coords) on entry and dL/d(input
+// coords) on exit; the layer's parameter gradient of this block goes to gp[blockIdx.x][FPL].
+__global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__restrict__ layers, int l, uint64_t n,
+                                                       const float *__restrict__ xin, const float *__restrict__ g,
+                                                       float *__restrict__ abar, float *__restrict__ gp)
+{
+    __shared__ PackedLayer P;
+    __shared__ uint16_t H1[64 * HS], H2[64 * HS];
+    __shared__ float Ob[64 * OS], Db[64 * OS];
+    __shared__ float Cc[64 * 2];
+    stage_layer(P, layers + l);
+    const int d = l % 3, ca = cond0(d), cb = cond1(d);
+    const int lane = threadIdx.x;
+    Acc gW2, gW3;
+    acc_zero(gW2);
+    acc_zero(gW3);
+    float gb1 = 0.0f, gb2 = 0.0f, gb3 = 0.0f, gw10 = 0.0f, gw11 = 0.0f;
+    const uint64_t ntiles = (n + 63) / 64;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t i = tile * 64 + lane;
+        const bool live = i < n;
+        float x[3] = {0.5f, 0.5f, 0.5f}, ab[3] = {0.0f, 0.0f, 0.0f}, gi = 0.0f;
+        if (live) {
+            x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2];
+            ab[0] = abar[3 * i]; ab[1] = abar[3 * i + 1]; ab[2] = abar[3 * i + 2];
+            gi = g[i];
+        }
+        Cc[2 * lane] = x[ca];
+        Cc[2 * lane + 1] = x[cb];
+        mlp_tile(P, x[ca], x[cb], H1, H2, Ob);
+        // ---- spline backward (lane = point), dO written over O
+        {
+            float *o = Ob + lane * OS;
+            float mw = o[0], mh = o[FK];
+            for (int q = 1; q < FK; ++q) { mw = fmaxf(mw, o[q]); mh = fmaxf(mh, o[FK + q]); }
+            float sw = 0.0f, sh = 0.0f;
+            for (int q = 0; q < FK; ++q) { sw += expf(o[q] - mw); sh += expf(o[FK + q] - mh); }
+            const Bin b = locate(o, x[d], false);
+            const float xi = (x[d] - b.xk) / b.wk, s = b.hk / b.wk, t = xi * (1.0f - xi);
+            const float A = b.dk1 + b.dk - 2.0f * s, den = s + A * t;
+            const float P2 = s * xi * xi + b.dk * t, R = P2 / den;
+            const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
+            const float aS = live ? ab[d] : 0.0f, gL = live ? gi : 0.0f;
+            const float den2 = den * den, omx = 1.0f - xi, om2t = 1.0f - 2.0f * t;
+            // d R / d (xi, s, dk, dk1)
+            const float Pxi = 2.0f * s * xi + b.dk * (1.0f - 2.0f * xi), denxi = A * (1.0f - 2.0f * xi);
+            const float R_xi = (Pxi * den - P2 * denxi) / den2;
+            const float R_s = (xi * xi * den - P2 * om2t) / den2;
+            const float R_dk = t * (den - P2) / den2;
+            const float R_dk1 = -P2 * t / den2;
+            // d log S' / d (xi, s, dk, dk1)
+            const float n2xi = 2.0f * b.dk1 * xi + 2.0f * s * (1.0f - 2.0f * xi) - 2.0f * b.dk * omx;
+            const float L_xi = n2xi / n2 - 2.0f * denxi / den;
+            const float L_s = 2.0f / s + 2.0f * t / n2 - 2.0f * om2t / den;
+            const float L_dk = omx * omx / n2 - 2.0f * t / den;
+            const float L_dk1 = xi * xi / n2 - 2.0f * t / den;
+            const float hR = aS * b.hk;
+            const float Gxi = hR * R_xi + gL * L_xi, Gs = hR * R_s + gL * L_s;
+            const float Gdk = hR * R_dk + gL * L_dk, Gdk1 = hR * R_dk1 + gL * L_dk1;
+            const float dx = Gxi / b.wk;                  // dL / d x_in[d]
+            const float dxk = -dx;
+            const float dwk = -(Gxi * xi + Gs * s) / b.wk;
+            const float dhk = aS * R + Gs / b.wk;
+            const float dyk = aS;
+            // widths / heights: cumulative sums then softmax
+            float dotw = 0.0f, doth = 0.0f;
+            float gw[FK], gh[FK];
+            const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
+            for (int q = 0; q < FK; ++q) {
+                const float gwq = (q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f);
+                const float ghq = (q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f);
+                const float pw = expf(o[q] - mw) / sw, ph = expf(o[FK + q] - mh) / sh;
+                gw[q] = cw * gwq;
+                gh[q] = ch * ghq;
+                dotw += pw * gw[q];
+                doth += ph * gh[q];
+            }
+            float dO[FO];
+            for (int q = 0; q < FK; ++q) {
+                const float pw = expf(o[q] - mw) / sw, ph = expf(o[FK + q] - mh) / sh;
+                dO[q] = pw * (gw[q] - dotw);
+                dO[FK + q] = ph * (gh[q] - doth);
+            }
+            for (int q = 0; q <= FK; ++q) {
+                const float gq = (q == b.k ? Gdk : 0.0f) + (q == b.k + 1 ? Gdk1 : 0.0f);
+                const float sig = 1.0f / (1.0f + expf(-o[2 * FK + q]));
+                dO[2 * FK + q] = gq * sig;
+            }
+            __syncthreads();
+            for (int q = 0; q < FO; ++q) o[q] = live ? dO[q] : 0.0f;
+            for (int q = FO; q < 64; ++q) o[q] = 0.0f;
+            ab[d] = live ? dx : 0.0f;
+        }
+        __syncthreads();
+        // ---- MLP backward
+        for (int p = 0; p < 64; ++p) gb3 += (lane < FO) ? Ob[p * OS + lane] : 0.0f;
+        wgemm(gW3, Ob, 1, OS, H2, HS, 1);                         // dW3 += dO^T H2
+        Acc a;
+        acc_zero(a);
+        wgemm(a, Ob, OS, 1, P.W3, FH, 1);                          // dH2 = dO W3
+        acc_each(a, [&](int m, int nn, float v) { Db[m * OS + nn] = (H2[m * HS + nn] != 0) ? v : 0.0f; });
+        __syncthreads();
+        for (int p = 0; p < 64; ++p) gb2 += Db[p * OS + lane];
+        wgemm(gW2, Db, 1, OS, H1, HS, 1);                          // dW2 += dH2^T H1
+        acc_zero(a);
+        wgemm(a, Db, OS, 1, P.W2, FH, 1);                          // dH1 = dH2 W2
+        __syncthreads();
+        acc_each(a, [&](int m, int nn, float v) { Ob[m * OS + nn] = (H1[m * HS + nn] != 0) ? v : 0.0f; });
+        __syncthreads();
+        for (int p = 0; p < 64; ++p) {
+            const float v = Ob[p * OS + lane];
+            gb1 += v;
+            gw10 += v * Cc[2 * p];
+            gw11 += v * Cc[2 * p + 1];
+        }
+        float dc0 = 0.0f, dc1 = 0.0f;
+        for (int j = 0; j < FH; ++j) {
+            const float v = Ob[lane * OS + j];
+            dc0 += v * P.W1[2 * j];
+            dc1 += v * P.W1[2 * j + 1];
+        }
+        if (live) {
+            ab[ca] += dc0;
+            ab[cb] += dc1;
+            abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
+        }
+        __syncthreads();
+    }
+    float *out = gp + (size_t)blockIdx.x * FPL;
+    out[PW1 + 2 * lane] = gw10;
+    out[PW1 + 2 * lane + 1] = gw11;
+    out[PB1 + lane] = gb1;
+    out[PB2 + lane] = gb2;
+    if (lane < FO) out[PB3 + lane] = gb3;
+    acc_each(gW2, [&](int m, int nn, float v) { out[PW2 + m * FH + nn] = v; });
+    acc_each(gW3, [&](int m, int nn, float v) { if (m < FO) out[PW3 + m * FH + nn] = v; });
+}
+
+// ------------------------------------------------------------------------------------ small kernels
+__global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__restrict__ out)
+{
+    const int l = blockIdx.x;
+    const float *p = params + (size_t)l * FPL;
+    PackedLayer &q = out[l];
+    for (int i = threadIdx.x; i < 2 * FH; i += blockDim.x) q.W1[i] = bf2f(f2bf(p[PW1 + i]));
+    for (int i = threadIdx.x; i < FH; i += blockDim.x) { q.b1[i] = p[PB1 + i]; q.b2[i] = p[PB2 + i]; }
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) q.b3[i] = i < FO ? p[PB3 + i] : 0.0f;
+    for (int i = threadIdx.x; i < FH * FH; i += blockDim.x) q.W2[i] = f2bf(p[PW2 + i]);
+    for (int i = threadIdx.x; i < 64 * FH; i += blockDim.x) q.W3[i] = (i / FH) < FO ? f2bf(p[PW3 + i]) : (uint16_t)0;
+}
+
+// z: three 24-bit uniforms of Philox stream `sid` of events ctr0 + i.
+__global__ void k_flow_sample_z(uint64_t key, uint64_t ctr0, uint32_t sid, uint64_t n, float *__restrict__ z,
+                                float *__restrict__ ld)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const U4 r = philox_event(key, ctr0 + i, sid);
+    z[3 * i] = (float)k24(r.x) * (1.0f / 16777216.0f);
+    z[3 * i + 1] = (float)k24(r.y) * (1.0f / 16777216.0f);
+    z[3 * i + 2] = (float)k24(r.z) * (1.0f / 16777216.0f);
+    ld[i] = 0.0f;
+}
+
+// Weight of a flow point: y -> VEGAS map -> FP32 integrand, times J_V and |dy/dz| = exp(ld).
+// Writes w (optional) and one kRec record per block.
+__global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int mask, const float *__restrict__ gtab,
+                                                        int nbins, uint64_t n, const float *__restrict__ y,
+                                                        const float *__restrict__ ld, float *__restrict__ wout,
+                                                        double *__restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *stab = reinterpret_cast<float *>(smem);
+    for (int i = threadIdx.x; i < 9 * nbins; i += blockDim.x) stab[i] = gtab[i];
+    __syncthreads();
+    double v[kRec];
+#pragma unroll
+    for (int q = 0; q < kRec; ++q) v[q] = 0.0;
+    const float nbf = (float)nbins;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float u[3], om[3], jac = 1.0f;
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+            const float yy = fminf(fmaxf(y[3 * i + dd], 0.0f), 1.0f);
+            const float t = yy * nbf;
+            const int b = min((int)t, nbins - 1);
+            const float frac = t - (float)b;
+            const float *q = stab + 3 * (dd * nbins + b);
+            u[dd] = __builtin_fmaf(frac, q[1], q[0]);
+            om[dd] = __builtin_fmaf(-frac, q[1], q[2]);
+            jac = jac * (nbf * q[1]);
+        }
+        float ws[kMaxProc];
+        const float f = had_point_f32(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr, nullptr,
+                                      nullptr);
+        const float wt = f * jac * expf(ld[i]);
+        if (wout) wout[i] = wt;
+        const double dw = (double)wt;
+        v[0] += dw;
+        v[7] = fma(dw, dw, v[7]);
+        v[14] = fmax(v[14], dw);
+        v[21] += (wt > 0.0f) ? 1.0 : 0.0;
+        const float scale = jac * expf(ld[i]);
+#pragma unroll
+        for (int p = 0; p < kMaxProc; ++p) {
+            const double dp = (double)(ws[p] * scale);
+            v[1 + p] += dp;
+            v[8 + p] = fma(dp, dp, v[8 + p]);
+            v[15 + p] = fmax(v[15 + p], dp);
+        }
+    }
+    block_record(v, partials + (size_t)blockIdx.x * kRec);
+}
+
+// g_i = -w_i^2 / (n <w>^2) (variance-loss score coefficient), abar = 0; stats = sum w, sum w^2.
+__global__ void k_flow_coef(const float *__restrict__ w, const double *__restrict__ rec, uint64_t n,
+                            float *__restrict__ g, float *__restrict__ abar, double *__restrict__ hist, int step)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double mean = rec[0] / (double)n;
+    if (i == 0) { hist[2 * step] = rec[0]; hist[2 * step + 1] = rec[7]; }
+    if (i >= n) return;
+    const double r = (double)w[i] / mean;
+    g[i] = (float)(-(r * r) / (double)n);
+    abar[3 * i] = 0.0f; abar[3 * i + 1] = 0.0f; abar[3 * i + 2] = 0.0f;
+}
+
+// Column sums of nblk x ncol floats (fixed order) into out (FP32 gradient).
+__global__ void k_colsum_f(const float *__restrict__ parts, uint32_t nblk, uint32_t ncol, float *__restrict__ out)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < nblk; ++b) s += parts[(size_t)b * ncol + c];
+    out[c] = s;
+}
+
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Adds a kRec record (sum slots add, max slots max) into an accumulating record.
+__global__ void k_add_record(double *__restrict__ acc, const double *__restrict__ rec, int first)
+{
+    const int i = threadIdx.x;
+    if (i >= kRec) return;
+    if (first) { acc[i] = rec[i]; return; }
+    acc[i] = (i >= 14 && i <= 20) ? fmax(acc[i], rec[i]) : acc[i] + rec[i];
+}
+
+}  // namespace mi
+
+// ============================================================================ host
+using namespace mi;
+
+namespace mi {
+
+struct FlowState {
+    float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    PackedLayer *packed = nullptr;
+    // work buffers sized for `cap` points
+    uint64_t cap = 0;
+    float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
+    float *ld = nullptr, *w = nullptr, *g = nullptr, *abar = nullptr;
+    float *gp = nullptr;        // per-block layer gradients
+    uint32_t gp_blocks = 0;
+    double *hist = nullptr;     // training history (sum w, sum w^2 per step)
+    int hist_cap = 0;
+    double *acc = nullptr;      // accumulating record
+    int step = 0;
+    bool ready = false;
+};
+
+}  // namespace mi
+
+namespace {
+
+int ferr(mi_ctx *ctx, int code, const char *msg)
+{
+    ctx->err = msg;
+    return code;
+}
+
+#define FCHK(call)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                          \
+            return MI_ERR_HIP;                                                                      \
+        }                                                                                           \
+    } while (0)
+
+// Host Philox4x32-10 (the same counter map as the device) for the initial weights.
+void philox_host(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t o[4])
+{
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+int flow_blocks(mi_ctx *ctx) { return (ctx->num_cu > 0 ? ctx->num_cu : 256) * 2; }
+
+int flow_alloc(mi_ctx *ctx, uint64_t npts)
+{
+    FlowState &F = *ctx->flow;
+    if (npts > F.cap) {
+        (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
+        F.xs = nullptr; F.ld = F.w = F.g = F.abar = nullptr; F.cap = 0;
+        FCHK(hipMalloc(&F.xs, sizeof(float) * 3 * (FL + 1) * npts));
+        FCHK(hipMalloc(&F.ld, sizeof(float) * npts));
+        FCHK(hipMalloc(&F.w, sizeof(float) * npts));
+        FCHK(hipMalloc(&F.g, sizeof(float) * npts));
+        FCHK(hipMalloc(&F.abar, sizeof(float) * 3 * npts));
+        F.cap = npts;
+    }
+    const uint32_t nb = (uint32_t)flow_blocks(ctx);
+    if (nb > F.gp_blocks) {
+        (void)hipFree(F.gp);
+        F.gp = nullptr;
+        FCHK(hipMalloc(&F.gp, sizeof(float) * (size_t)FPL * nb));
+        F.gp_blocks = nb;
+    }
+    return MI_OK;
+}
+
+int check_flow(mi_ctx *ctx)
+{
+    if (!ctx) return MI_ERR_ARG;
+    if (!ctx->flow || !ctx->flow->ready) return ferr(ctx, MI_ERR_STATE, "flow not initialised (mi_flow_init)");
+    return MI_OK;
+}
+
+hipError_t run_layers(mi_ctx *ctx, bool inverse, uint64_t n, float *x_first, float *ld, bool keep_all)
+{
+    // x_first holds the input; with keep_all the layer inputs of the normalizing direction land in
+    // F.xs (x^(0) .. x^(FL)), otherwise the result overwrites x_first.
+    FlowState &F = *ctx->flow;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 63) / 64, (uint64_t)flow_blocks(ctx) * 4);
+    for (int s = 0; s < FL; ++s) {
+        const int l = inverse ? FL - 1 - s : s;
+        const float *src;
+        float *dst;
+        if (keep_all) {
+            src = F.xs + (size_t)3 * F.cap * (inverse ? l + 1 : l);
+            dst = F.xs + (size_t)3 * F.cap * (inverse ? l : l + 1);
+        } else {
+            src = x_first;
+            dst = x_first;
+        }
+        if (inverse)
+            hipLaunchKernelGGL(k_flow_layer<true>, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n, src, dst, ld);
+        else
+            hipLaunchKernelGGL(k_flow_layer<false>, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n, src, dst, ld);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mi_flow_release(mi_ctx *ctx)
+{
+    if (!ctx || !ctx->flow) return;
+    FlowState &F = *ctx->flow;
+    (void)hipFree(F.params); (void)hipFree(F.m); (void)hipFree(F.v); (void)hipFree(F.grad); (void)hipFree(F.packed);
+    (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
+    (void)hipFree(F.gp); (void)hipFree(F.hist); (void)hipFree(F.acc);
+    delete ctx->flow;
+    ctx->flow = nullptr;
+}
+
+int mi_flow_set_params(mi_ctx *ctx, const float *params_host)
+{
+    if (!ctx) return MI_ERR_ARG;
+    if (!params_host) return ferr(ctx, MI_ERR_ARG, "params is NULL");
+    FCHK(hipSetDevice(ctx->device));
+    if (!ctx->flow) ctx->flow = new FlowState();
+    FlowState &F = *ctx->flow;
+    if (!F.params) {
+        FCHK(hipMalloc(&F.params, sizeof(float) * MI_FLOW_NPARAMS));
+        FCHK(hipMalloc(&F.m, sizeof(float) * MI_FLOW_NPARAMS));
+        FCHK(hipMalloc(&F.v, sizeof(float) * MI_FLOW_NPARAMS));
+        FCHK(hipMalloc(&F.grad, sizeof(float) * MI_FLOW_NPARAMS));
+        FCHK(hipMalloc(&F.packed, sizeof(PackedLayer) * FL));
+        FCHK(hipMalloc(&F.acc, sizeof(double) * kRec));
+    }
+    FCHK(hipMemcpyAsync(F.params, params_host, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyHostToDevice, ctx->stream));
+    FCHK(hipMemsetAsync(F.m, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
+    FCHK(hipMemsetAsync(F.v, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
+    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(ctx->stream));
+    F.step = 0;
+    F.ready = true;
+    return MI_OK;
+}
+
+int mi_flow_get_params(mi_ctx *ctx, float *params_host)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (!params_host) return ferr(ctx, MI_ERR_ARG, "params is NULL");
+    FCHK(hipSetDevice(ctx->device));
+    FCHK(hipMemcpyAsync(params_host, ctx->flow->params, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyDeviceToHost,
+                        ctx->stream));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
+int mi_flow_init(mi_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return MI_ERR_ARG;
+    std::vector<float> p(MI_FLOW_NPARAMS, 0.0f);
+    const float dinit = (float)std::log(std::expm1(1.0 - (double)kMinD));   // softplus^-1(1 - min): identity
+    uint64_t ctr = 0;
+    auto uni = [&](void) {
+        uint32_t o[4];
+        philox_host((uint32_t)ctr, (uint32_t)(ctr >> 32), 7u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+        ++ctr;
+        return (float)(o[0] >> 8) * (1.0f / 16777216.0f);
+    };
+    for (int l = 0; l < FL; ++l) {
+        float *q = p.data() + (size_t)l * FPL;
+        for (int i = 0; i < 2 * FH; ++i) q[PW1 + i] = (2.0f * uni() - 1.0f) / std::sqrt(2.0f);
+        for (int i = 0; i < FH * FH; ++i) q[PW2 + i] = (2.0f * uni() - 1.0f) / std::sqrt((float)FH);
+        for (int i = 0; i <= FK; ++i) q[PB3 + 2 * FK + i] = dinit;
+    }
+    int rc = mi_flow_set_params(ctx, p.data());
+    if (rc) return rc;
+    if (ctx->grid.ndim != 3) {
+        rc = mi_vegas_init(ctx, 3, 64);
+        if (rc) return rc;
+    }
+    return MI_OK;
+}
+
+int mi_flow_forward(mi_ctx *ctx, uint64_t n, const float *y_dev, float *z_dev, float *logdet_dev)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (n == 0) return MI_OK;
+    if (!y_dev || !z_dev || !logdet_dev) return ferr(ctx, MI_ERR_ARG, "NULL array");
+    FCHK(hipSetDevice(ctx->device));
+    FCHK(hipMemcpyAsync(z_dev, y_dev, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    FCHK(hipMemsetAsync(logdet_dev, 0, sizeof(float) * n, ctx->stream));
+    FCHK(run_layers(ctx, false, n, z_dev, logdet_dev, false));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
+int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, float *logdet_dev)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (n == 0) return MI_OK;
+    if (!y_dev || !z_dev || !logdet_dev) return ferr(ctx, MI_ERR_ARG, "NULL array");
+    FCHK(hipSetDevice(ctx->device));
+    FCHK(hipMemcpyAsync(y_dev, z_dev, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    FCHK(hipMemsetAsync(logdet_dev, 0, sizeof(float) * n, ctx->stream));
+    FCHK(run_layers(ctx, true, n, y_dev, logdet_dev, false));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
+// Gradient of sum_i g_i logdet_fwd(y_i) w.r.t. the parameters, for y already in F.xs[0].
+static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
+{
+    FlowState &F = *ctx->flow;
+    // forward pass keeping every layer input
+    FCHK(hipMemsetAsync(F.ld, 0, sizeof(float) * n, ctx->stream));
+    FCHK(run_layers(ctx, false, n, nullptr, F.ld, true));
+    FCHK(hipMemsetAsync(F.abar, 0, sizeof(float) * 3 * n, ctx->stream));
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 63) / 64, (uint64_t)F.gp_blocks);
+    for (int l = FL - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n,
+                           F.xs + (size_t)3 * F.cap * l, g_dev, F.abar, F.gp);
+        FCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_colsum_f, dim3((FPL + 255) / 256), dim3(256), 0, ctx->stream, F.gp, blocks, (uint32_t)FPL,
+                           F.grad + (size_t)l * FPL);
+        FCHK(hipGetLastError());
+    }
+    return MI_OK;
+}
+
+int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g_dev, float *grad_dev)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (n == 0) return ferr(ctx, MI_ERR_ARG, "n must be > 0");
+    if (!y_dev || !g_dev || !grad_dev) return ferr(ctx, MI_ERR_ARG, "NULL array");
+    FCHK(hipSetDevice(ctx->device));
+    rc = flow_alloc(ctx, n);
+    if (rc) return rc;
+    FlowState &F = *ctx->flow;
+    FCHK(hipMemcpyAsync(F.xs, y_dev, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    rc = flow_backward_internal(ctx, n, g_dev);
+    if (rc) return rc;
+    FCHK(hipMemcpyAsync(grad_dev, F.grad, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyDeviceToDevice, ctx->stream));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
+// Sample n points of the current flow (+ VEGAS) for events ctr0 ..: y in F.xs[0], ld, w, and
+// the estimate record of the batch in out_rec (device).
+static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, uint32_t sid,
+                               double *out_rec, bool keep_all)
+{
+    FlowState &F = *ctx->flow;
+    float *z = F.xs + (size_t)3 * F.cap * FL;
+    hipLaunchKernelGGL(k_flow_sample_z, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, run->key, ctr0,
+                       sid, n, z, F.ld);
+    FCHK(hipGetLastError());
+    if (keep_all) {
+        FCHK(run_layers(ctx, true, n, nullptr, F.ld, true));
+    } else {
+        FCHK(hipMemcpyAsync(F.xs, z, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
+        FCHK(run_layers(ctx, true, n, F.xs, F.ld, false));
+    }
+    HadConst<float> k;
+    fill_had_consts_f32(run->phys, k);
+    const int occ = 2;
+    const uint32_t nblk = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock,
+                                                      (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ);
+    if ((size_t)nblk * kRec > ctx->partials_cap) return ferr(ctx, MI_ERR_STATE, "partials buffer too small");
+    hipLaunchKernelGGL(k_flow_weight, dim3(nblk), dim3(kBlock), sizeof(float) * 9 * ctx->grid.nbins, ctx->stream, k,
+                       run->phys.proc_mask, ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials);
+    FCHK(hipGetLastError());
+    FCHK(launch_reduce_records(ctx->stream, ctx->partials, nblk, out_rec));
+    return MI_OK;
+}
+
+int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, double lr, mi_flow_info *out)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (!run) return ferr(ctx, MI_ERR_ARG, "run is NULL");
+    if (run->phys.mode != MI_MODE_HADRONIC_3D) return ferr(ctx, MI_ERR_ARG, "the flow needs the hadronic mode");
+    if (steps < 1 || steps > 100000) return ferr(ctx, MI_ERR_ARG, "steps must be 1..100000");
+    if (batch < 64 || batch > (1ULL << 24)) return ferr(ctx, MI_ERR_RANGE, "batch must be 64..2^24");
+    if (!(lr > 0.0) || !(lr < 1.0)) return ferr(ctx, MI_ERR_ARG, "lr must be in (0, 1)");
+    if (ctx->grid.ndim != 3) return ferr(ctx, MI_ERR_STATE, "the flow composes with a 3-D VEGAS grid");
+    FCHK(hipSetDevice(ctx->device));
+    rc = flow_alloc(ctx, batch);
+    if (rc) return rc;
+    FlowState &F = *ctx->flow;
+    if (steps > F.hist_cap) {
+        (void)hipFree(F.hist);
+        F.hist = nullptr;
+        FCHK(hipMalloc(&F.hist, sizeof(double) * 2 * steps));
+        F.hist_cap = steps;
+    }
+    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 2;
+    if ((size_t)need * kRec > ctx->partials_cap) {
+        (void)hipFree(ctx->partials);
+        ctx->partials = nullptr;
+        ctx->partials_cap = 0;
+        FCHK(hipMalloc(&ctx->partials, sizeof(double) * need * kRec));
+        ctx->partials_cap = (size_t)need * kRec;
+    }
+    FCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int s = 0; s < steps; ++s) {
+        rc = flow_sample_weights(ctx, run, run->counter0 + (uint64_t)s * batch, batch, 2u, ctx->red, true);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_flow_coef, dim3((uint32_t)((batch + 255) / 256)), dim3(256), 0, ctx->stream, F.w, ctx->red,
+                           batch, F.g, F.abar, F.hist, s);
+        FCHK(hipGetLastError());
+        // backward from the stored layer inputs (x^(0) = y ... x^(FL) = z from the sampling pass)
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((batch + 63) / 64, (uint64_t)F.gp_blocks);
+        for (int l = FL - 1; l >= 0; --l) {
+            hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, batch,
+                               F.xs + (size_t)3 * F.cap * l, F.g, F.abar, F.gp);
+            FCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_colsum_f, dim3((FPL + 255) / 256), dim3(256), 0, ctx->stream, F.gp, blocks,
+                               (uint32_t)FPL, F.grad + (size_t)l * FPL);
+            FCHK(hipGetLastError());
+        }
+        F.step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, (float)F.step), c2 = 1.0f - std::pow(b2, (float)F.step);
+        hipLaunchKernelGGL(k_adam, dim3((MI_FLOW_NPARAMS + 255) / 256), dim3(256), 0, ctx->stream, F.params, F.m, F.v,
+                           F.grad, MI_FLOW_NPARAMS, (float)lr, b1, b2, 1e-8f, c1, c2);
+        FCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed);
+        FCHK(hipGetLastError());
+    }
+    FCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<double> h(2 * (size_t)steps);
+    FCHK(hipMemcpyAsync(h.data(), F.hist, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost, ctx->stream));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    FCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (out) {
+        std::memset(out, 0, sizeof *out);
+        const double nb = (double)batch;
+        auto loss = [&](int s) {
+            const double m = h[2 * s] / nb, m2 = h[2 * s + 1] / nb;
+            return m > 0.0 ? m2 / (m * m) - 1.0 : 0.0;
+        };
+        out->steps = steps;
+        out->loss_first = loss(0);
+        out->loss_last = loss(steps - 1);
+        out->cv_vegas = std::sqrt(std::max(0.0, out->loss_first));
+        out->cv_flow = std::sqrt(std::max(0.0, out->loss_last));
+        out->kernel_ms = ms;
+    }
+    return MI_OK;
+}
+
+// Integration with the flow sampler: chunks of up to 2^22 events (counters counter0 ..),
+// z from Philox stream 0, y = T^-1(z), weight through the VEGAS map; records accumulated on
+// the device in a fixed order.
+int mi_flow_integrate_internal(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out)
+{
+    int rc = check_flow(ctx);
+    if (rc) return rc;
+    if (run->phys.mode != MI_MODE_HADRONIC_3D) return ferr(ctx, MI_ERR_ARG, "the flow sampler needs the hadronic mode");
+    if (run->precision != MI_PREC_FP32) return ferr(ctx, MI_ERR_ARG, "the flow sampler evaluates FP32 weights");
+    if (ctx->grid.ndim != 3) return ferr(ctx, MI_ERR_STATE, "the flow composes with a 3-D VEGAS grid");
+    const uint64_t chunk = std::min<uint64_t>(run->n_points, 1ULL << 22);
+    rc = flow_alloc(ctx, chunk);
+    if (rc) return rc;
+    FlowState &F = *ctx->flow;
+    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 2;
+    if ((size_t)need * kRec > ctx->partials_cap) {
+        (void)hipFree(ctx->partials);
+        ctx->partials = nullptr;
+        ctx->partials_cap = 0;
+        FCHK(hipMalloc(&ctx->partials, sizeof(double) * need * kRec));
+        ctx->partials_cap = (size_t)need * kRec;
+    }
+    FCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    int first = 1;
+    for (uint64_t off = 0; off < run->n_points; off += chunk) {
+        const uint64_t n = std::min<uint64_t>(chunk, run->n_points - off);
+        rc = flow_sample_weights(ctx, run, run->counter0 + off, n, 0u, ctx->red, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_add_record, dim3(1), dim3(64), 0, ctx->stream, F.acc, ctx->red, first);
+        FCHK(hipGetLastError());
+        first = 0;
+    }
+    FCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    double r[kRec];
+    FCHK(hipMemcpyAsync(r, F.acc, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    FCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    FCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::memset(out, 0, sizeof *out);
+    const double nn = (double)run->n_points;
+    out->code[0] = 0;
+    for (int i = 0; i < MI_MAX_PROC; ++i) out->code[1 + i] = 111 + i;
+    for (int i = 0; i <= MI_MAX_PROC; ++i) {
+        if (i > 0 && !((run->phys.proc_mask >> (i - 1)) & 1)) continue;
+        const double avg = r[i] / nn;
+        out->sumW[i] = r[i];
+        out->sumW2[i] = r[7 + i];
+        out->wMax[i] = r[14 + i];
+        out->sigmaGen[i] = avg;
+        double d2 = 0.0;
+        if (avg != 0.0) d2 = (r[7 + i] / nn - avg * avg) / nn / (avg * avg);
+        out->sigmaErr[i] = (d2 > 0.0 ? std::sqrt(d2) : 0.0) * avg;
+    }
+    out->nTry = out->nSel = out->nAcc = (int64_t)run->n_points;
+    out->nNonZero = (int64_t)r[21];
+    out->kernel_ms = ms;
+    if (!std::isfinite(out->sigmaGen[0])) return ferr(ctx, MI_ERR_NUMERIC, "non-finite sigma");
+    return MI_OK;
+}
+
+}  // extern "C"

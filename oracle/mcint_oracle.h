@@ -182,6 +182,13 @@ void orc_vegas1d_compat(int K, int blocks, int threads, uint64_t Tper, int n_ite
    nthreads cores; sums = {sum w, sum w^2}. */
 void orc_had_mc(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, int nthreads, double sums[2]);
 
+/* ------------------------------------------------------------------ adjoint ---- */
+/* Central finite differences (relative step 1e-5) of the long-double point, per event, with the
+   configuration rebuilt at theta +- h.  out[0] = sum w, out[1..4] = sum dw/d{alphaS(MZ),
+   renorm_fac, gluon a, gluon b}, out[5..8] = sums of squares.
+   Pinned: FD of a closed form (fixed alphaS: d ln sigma / d ln alphaS = 2 exactly). */
+void orc_adjoint(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, double out[9]);
+
 #ifdef __cplusplus
 }
 #endif
