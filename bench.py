@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tts", action="store_true", help="also measure time-to-solution (VEGAS, flow+VEGAS)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc: 3 integration launches and one 1 GiB device copy (the "
+                         "calibration of FETCH_SIZE / WRITE_SIZE), nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the integration shards by "
@@ -108,6 +111,16 @@ def main():
     ctx = mi.Context(0)
     phys = mi.physics_default(1)
     n = args.points
+    if args.pmc_probe:
+        for k in range(3):
+            ctx.integrate(mi.make_run(phys, n, key=42, counter0=k * n))
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")      # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": src.numel() * 4}))
+        ctx.close()
+        return
     for w in range(args.warmup):
         ctx.integrate(mi.make_run(phys, n, key=42, counter0=w * n))
     torch.cuda.synchronize()

@@ -237,6 +237,18 @@ class Context:
                     wused.value, SigmaEstimate(est))
         return ev
 
+    # -------------------------------------------------------------- reweighting
+    def reweight_scales(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, w: torch.Tensor,
+                        factors) -> torch.Tensor:
+        """Weights at renormalisation-scale factors (n x nvar), one pass over the events."""
+        fac = (C.c_double * len(factors))(*factors)
+        n = sHat.numel()
+        out = torch.empty((n, len(factors)), dtype=torch.float64, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_reweight_scales(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat), _ptr(w),
+                                                len(factors), fac, _ptr(out)), "mi_reweight_scales")
+        return out
+
     # -------------------------------------------------------------- adjoint
     def adjoint(self, run: A.MiRun) -> A.MiSigmaGrad:
         g = A.MiSigmaGrad()

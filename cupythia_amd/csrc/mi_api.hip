@@ -603,4 +603,32 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
     return MI_OK;
 }
 
+// ---------------------------------------------------------------------------- reweighting
+int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev,
+                       const double *tHat_dev, const double *w_dev, int nvar, const double *factors_host,
+                       double *out_dev)
+{
+    if (!ctx) return MI_ERR_ARG;
+    if (!phys) return fail(ctx, MI_ERR_ARG, "phys is NULL");
+    if (nvar < 1 || nvar > 64) return fail(ctx, MI_ERR_RANGE, "nvar must be 1..64");
+    if (!factors_host) return fail(ctx, MI_ERR_ARG, "factors is NULL");
+    for (int v = 0; v < nvar; ++v)
+        if (!(factors_host[v] > 0.0)) return fail(ctx, MI_ERR_ARG, "scale factors must be > 0");
+    if (n && (!sHat_dev || !tHat_dev || !w_dev || !out_dev)) return fail(ctx, MI_ERR_ARG, "NULL array");
+    if (!(phys->alphaS_value > 0.0) || !(phys->renorm_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "bad alphaS / renorm_fac");
+    if (n == 0) return MI_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HadConst<double> k;
+    mi_physics p = *phys;
+    if (p.eCM <= 2.0 * p.pTHatMin || p.pTHatMin <= 0.0) { p.eCM = 13600.0; p.pTHatMin = 20.0; }
+    if (p.nQuarkIn < 2 || p.nQuarkIn > 5) p.nQuarkIn = 5;
+    fill_had_consts_f64(p, k);
+    int rc = ensure(ctx, &ctx->hist, &ctx->hist_cap, (size_t)nvar);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hist, factors_host, sizeof(double) * nvar, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, launch_reweight(ctx->stream, k, n, sHat_dev, tHat_dev, w_dev, nvar, ctx->hist, out_dev));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
 }  // extern "C"

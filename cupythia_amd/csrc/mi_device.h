@@ -41,8 +41,10 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        // one 32x32 -> 64 multiply each (v_mad_u64_u32) instead of separate lo / hi products
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -77,12 +79,14 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
 __device__ __forceinline__ double u01(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
 
 // ------------------------------------------------------------------ own FP32 exp / log
+// Branch-free forms (selects instead of branches, so a wave never runs both sides of a divergent
+// test); the results are identical to the straightforward branching definitions.
 __device__ __forceinline__ float own_expf(float x)
 {
-    if (x < -86.0f) return 0.0f;
-    if (x > 88.0f) x = 88.0f;
-    const float n = rintf(x * 1.4426950408889634f);
-    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    const bool under = x < -86.0f;                         // result would leave the normal range: 0
+    const float xc = fminf(fmaxf(x, -86.0f), 88.0f);
+    const float n = rintf(xc * 1.4426950408889634f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, xc);
     r = __builtin_fmaf(n, -1.4286068203094172e-6f, r);
     float p = 1.9841269841269841e-4f;
     p = __builtin_fmaf(p, r, 1.3888888888888889e-3f);
@@ -92,32 +96,34 @@ __device__ __forceinline__ float own_expf(float x)
     p = __builtin_fmaf(p, r, 0.5f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    return p * __int_as_float(((int)n + 127) << 23);
+    const float v = p * __int_as_float(((int)n + 127) << 23);
+    return under ? 0.0f : v;
 }
 
-// 1 - e^x (x <= 0) without cancellation near 0; ex = e^x already computed.
+// 1 - e^x (x <= 0) without cancellation near 0; ex = e^x already computed: Taylor series of
+// -(e^x - 1) to x^8 for x > -0.35, 1 - ex below.
 __device__ __forceinline__ float one_minus_expf(float x, float ex)
 {
-    if (x > -0.35f) {
-        float p = __builtin_fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
-        p = __builtin_fmaf(x, p, 1.3888888888888889e-3f);
-        p = __builtin_fmaf(x, p, 8.3333333333333333e-3f);
-        p = __builtin_fmaf(x, p, 4.1666666666666667e-2f);
-        p = __builtin_fmaf(x, p, 1.6666666666666667e-1f);
-        p = __builtin_fmaf(x, p, 0.5f);
-        p = __builtin_fmaf(x, p, 1.0f);
-        return -(x * p);
-    }
-    return 1.0f - ex;
+    float p = __builtin_fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
+    p = __builtin_fmaf(x, p, 1.3888888888888889e-3f);
+    p = __builtin_fmaf(x, p, 8.3333333333333333e-3f);
+    p = __builtin_fmaf(x, p, 4.1666666666666667e-2f);
+    p = __builtin_fmaf(x, p, 1.6666666666666667e-1f);
+    p = __builtin_fmaf(x, p, 0.5f);
+    p = __builtin_fmaf(x, p, 1.0f);
+    const float series = -(x * p);
+    return (x > -0.35f) ? series : 1.0f - ex;
 }
 
 __device__ __forceinline__ float own_logf(float x)
 {
-    if (!(x > 0.0f)) return -1.0e30f;
-    const int ix = __float_as_int(x);
+    const bool bad = !(x > 0.0f);                          // only 1 - x == 0 reaches here
+    const int ix = __float_as_int(bad ? 1.0f : x);
     int e = ((ix >> 23) & 0xff) - 127;
     float m = __int_as_float((ix & 0x007fffff) | 0x3f800000);
-    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    const bool big = m > 1.41421356f;
+    m = big ? m * 0.5f : m;
+    e = big ? e + 1 : e;
     const float f = m - 1.0f;
     const float s = f / (m + 1.0f);
     const float s2 = s * s;
@@ -127,7 +133,8 @@ __device__ __forceinline__ float own_logf(float x)
     const float two_s = s + s;
     const float lm = __builtin_fmaf(two_s * s2, p, two_s);
     const float fe = (float)e;
-    return __builtin_fmaf(fe, 0.693145751953125f, __builtin_fmaf(fe, 1.4286068203094172e-6f, lm));
+    const float v = __builtin_fmaf(fe, 0.693145751953125f, __builtin_fmaf(fe, 1.4286068203094172e-6f, lm));
+    return bad ? -1.0e30f : v;
 }
 
 // ------------------------------------------------------------------ constants of the integrand
@@ -187,13 +194,11 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
         alpS = k.as_fixed;
     } else {
         float Q2 = k.xiR2 * pT2;
-        if (Q2 < k.scale2Min) Q2 = k.scale2Min;
+        Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
         const float lq = own_logf(Q2);
-        float lam, cf;
-        if (Q2 > k.mt2 && k.nf6 != 0.0f) { lam = k.lnL6s; cf = k.c6; }
-        else if (Q2 > k.mb2)             { lam = k.lnL5s; cf = k.c5; }
-        else if (Q2 > k.mc2)             { lam = k.lnL4s; cf = k.c4; }
-        else                             { lam = k.lnL3s; cf = k.c3; }
+        const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0f, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
+        const float lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
+        const float cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
         alpS = cf / (lq - lam);
         cfo = cf;
     }

@@ -90,7 +90,7 @@ template <typename EA, typename EB>
 __device__ __forceinline__ void wgemm(Acc &acc, const EA *A, int a_sm, int a_sk, const EB *B, int b_sk, int b_sn)
 {
     const int l = threadIdx.x & 63, r = l & 31, hh = l >> 5;
-#pragma unroll
+#pragma unroll 1
     for (int kk = 0; kk < 4; ++kk) {
         const int k0 = 16 * kk + 8 * hh;
         const bf16x8 a0 = frag8(A + r * a_sm + k0 * a_sk, a_sk);
@@ -269,12 +269,12 @@ __global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__rest
     __shared__ uint16_t H1[64 * HS], H2[64 * HS];
     __shared__ float Ob[64 * OS], Db[64 * OS];
     __shared__ float Cc[64 * 2];
+    __shared__ float gW2[FH * FH], gW3[64 * FH];              // this block's weight gradients
     stage_layer(P, layers + l);
     const int d = l % 3, ca = cond0(d), cb = cond1(d);
     const int lane = threadIdx.x;
-    Acc gW2, gW3;
-    acc_zero(gW2);
-    acc_zero(gW3);
+    for (int q = lane; q < FH * FH; q += 64) { gW2[q] = 0.0f; gW3[q] = 0.0f; }
+    __syncthreads();
     float gb1 = 0.0f, gb2 = 0.0f, gb3 = 0.0f, gw10 = 0.0f, gw11 = 0.0f;
     const uint64_t ntiles = (n + 63) / 64;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -323,46 +323,46 @@ __global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__rest
             const float dwk = -(Gxi * xi + Gs * s) / b.wk;
             const float dhk = aS * R + Gs / b.wk;
             const float dyk = aS;
-            // widths / heights: cumulative sums then softmax
-            float dotw = 0.0f, doth = 0.0f;
-            float gw[FK], gh[FK];
+            // widths / heights: cumulative sums then softmax; dO overwrites O in place (two passes:
+            // the softmax projections first, then each output read before it is replaced)
             const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
+            float dotw = 0.0f, doth = 0.0f;
             for (int q = 0; q < FK; ++q) {
-                const float gwq = (q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f);
-                const float ghq = (q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f);
-                const float pw = expf(o[q] - mw) / sw, ph = expf(o[FK + q] - mh) / sh;
-                gw[q] = cw * gwq;
-                gh[q] = ch * ghq;
-                dotw += pw * gw[q];
-                doth += ph * gh[q];
+                const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
+                const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
+                dotw += (expf(o[q] - mw) / sw) * gwq;
+                doth += (expf(o[FK + q] - mh) / sh) * ghq;
             }
-            float dO[FO];
             for (int q = 0; q < FK; ++q) {
+                const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
+                const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
                 const float pw = expf(o[q] - mw) / sw, ph = expf(o[FK + q] - mh) / sh;
-                dO[q] = pw * (gw[q] - dotw);
-                dO[FK + q] = ph * (gh[q] - doth);
+                o[q] = live ? pw * (gwq - dotw) : 0.0f;
+                o[FK + q] = live ? ph * (ghq - doth) : 0.0f;
             }
             for (int q = 0; q <= FK; ++q) {
                 const float gq = (q == b.k ? Gdk : 0.0f) + (q == b.k + 1 ? Gdk1 : 0.0f);
                 const float sig = 1.0f / (1.0f + expf(-o[2 * FK + q]));
-                dO[2 * FK + q] = gq * sig;
+                o[2 * FK + q] = live ? gq * sig : 0.0f;
             }
-            __syncthreads();
-            for (int q = 0; q < FO; ++q) o[q] = live ? dO[q] : 0.0f;
             for (int q = FO; q < 64; ++q) o[q] = 0.0f;
             ab[d] = live ? dx : 0.0f;
         }
         __syncthreads();
         // ---- MLP backward
         for (int p = 0; p < 64; ++p) gb3 += (lane < FO) ? Ob[p * OS + lane] : 0.0f;
-        wgemm(gW3, Ob, 1, OS, H2, HS, 1);                         // dW3 += dO^T H2
         Acc a;
+        acc_zero(a);
+        wgemm(a, Ob, 1, OS, H2, HS, 1);                            // dW3 += dO^T H2
+        acc_each(a, [&](int m, int nn, float v) { gW3[m * FH + nn] += v; });
         acc_zero(a);
         wgemm(a, Ob, OS, 1, P.W3, FH, 1);                          // dH2 = dO W3
         acc_each(a, [&](int m, int nn, float v) { Db[m * OS + nn] = (H2[m * HS + nn] != 0) ? v : 0.0f; });
         __syncthreads();
         for (int p = 0; p < 64; ++p) gb2 += Db[p * OS + lane];
-        wgemm(gW2, Db, 1, OS, H1, HS, 1);                          // dW2 += dH2^T H1
+        acc_zero(a);
+        wgemm(a, Db, 1, OS, H1, HS, 1);                            // dW2 += dH2^T H1
+        acc_each(a, [&](int m, int nn, float v) { gW2[m * FH + nn] += v; });
         acc_zero(a);
         wgemm(a, Db, OS, 1, P.W2, FH, 1);                          // dH1 = dH2 W2
         __syncthreads();
@@ -393,8 +393,9 @@ __global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__rest
     out[PB1 + lane] = gb1;
     out[PB2 + lane] = gb2;
     if (lane < FO) out[PB3 + lane] = gb3;
-    acc_each(gW2, [&](int m, int nn, float v) { out[PW2 + m * FH + nn] = v; });
-    acc_each(gW3, [&](int m, int nn, float v) { if (m < FO) out[PW3 + m * FH + nn] = v; });
+    __syncthreads();
+    for (int q = lane; q < FH * FH; q += 64) out[PW2 + q] = gW2[q];
+    for (int q = lane; q < FO * FH; q += 64) out[PW3 + q] = gW3[q];
 }
 
 // ------------------------------------------------------------------------------------ small kernels

@@ -123,6 +123,10 @@ hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk);
 hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64_t *offsets,
                                  const mi_events &out);
 
+// Renormalisation-scale reweighting of events (mi_kernels.hip).
+hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
+                           const double *w, int nvar, const double *fac, double *out);
+
 // Adjoint (mi_adjoint.hip): partials of kRec doubles per block, see that file for the layout.
 struct AdjLaunch {
     HadLaunch h;

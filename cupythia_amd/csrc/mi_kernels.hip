@@ -497,3 +497,46 @@ int occ_integrate_c1(bool vegas, bool hist, int nbins)
 }
 
 }  // namespace mi
+
+namespace mi {
+
+// ------------------------------------------------------------------------------------ scale reweighting
+// Same-sample renormalisation-scale variations of weighted events (cuPythia/pipeline/reweight.cu:36-51):
+// w_k = w * (alphaS(xi_k^2 pT^2) / alphaS(xi0^2 pT^2))^2 with pT^2 = tHat uHat / sHat (massless),
+// alphaS in Pythia's order-0/1 form (FP64).
+__device__ __forceinline__ double alphas64(const HadConst<double> &k, double Q2)
+{
+    if (k.as_order == 0.0) return k.as_fixed;
+    Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
+    const double lq = log(Q2);
+    const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
+    const double lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
+    const double cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
+    return cf / (lq - lam);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reweight(HadConst<double> k, uint64_t n, const double *__restrict__ sH,
+                                                     const double *__restrict__ tH, const double *__restrict__ w,
+                                                     int nvar, const double *__restrict__ fac, double *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = sH[i], t = tH[i], u = -(s + t);
+    const double pT2 = t * u / s;
+    const double a0 = alphas64(k, k.xiR2 * pT2);
+    for (int v = 0; v < nvar; ++v) {
+        const double r = alphas64(k, fac[v] * fac[v] * pT2) / a0;
+        out[i * nvar + v] = w[i] * (r * r);
+    }
+}
+
+hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
+                           const double *w, int nvar, const double *fac, double *out)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reweight, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, k, n, sH, tH, w,
+                       nvar, fac, out);
+    return hipGetLastError();
+}
+
+}  // namespace mi

@@ -220,6 +220,15 @@ int mi_vegas_set_grid(mi_ctx *ctx, const double *edges_host, int ndim, int nbins
 int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_events *out,
                         uint64_t *n_acc, double *wmax_used, mi_sigma_est *est);
 
+/* ------------------------------------------------------------ reweighting ---- */
+/* Renormalisation-scale variations of events in one pass: out[i * nvar + k] =
+ * w[i] * (alphaS(f_k^2 pT^2) / alphaS(renorm_fac^2 pT^2))^2, pT^2 = tHat uHat / sHat (massless),
+ * alphaS per phys (order 0/1, FP64).  Replaces the one-pass multi-weight stage of
+ * cuPythia/pipeline/reweight.cu:36-51 and generate.cu:49-54.  nvar <= 64. */
+int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev,
+                       const double *tHat_dev, const double *w_dev, int nvar, const double *factors_host,
+                       double *out_dev);
+
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
  * same points, FP64 accumulation.  Hadronic mode with alphaS_order 1 (or 0 for the fixed-
