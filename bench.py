@@ -29,7 +29,7 @@ UNIT = "weighted events/s"
 # five processes).  Philox's integer work is excluded (reported by rocprof instead).
 FLOPS_PER_POINT = 180
 PEAK_FP32_TFLOPS = 157.3          # MI355X vector FP32 (vendor figure, counts packed FP32)
-PMC_FILE = os.path.join(ROOT, "profiles", "pmc_integrate_had_c2.txt")
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_integrate_had_2p28.txt")
 
 
 def traffic_from_profile():
@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=1 << 28)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tts", action="store_true", help="also measure time-to-solution (VEGAS, flow+VEGAS)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -146,7 +146,8 @@ def main():
                                "FP32 ME + FP64 sums, Philox4x32-10 key 42",
                    "points_per_step": n, "precision": "fp32 ME / fp64 accumulation",
                    "sigma_mb": est.sigmaGen(), "sigma_err_mb": est.sigmaErr()},
-        "roofline": {"bound": "valu", "kernel": "k_integrate_had<float,flat>", "achieved": achieved,
+        "roofline": {"bound": "mfma", "engine": "fp32 vector ALU (compute roof; the FP32 matrix and vector peaks are both 157.3)",
+                     "kernel": "k_integrate_had<float,flat>", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(), "flops_per_point": FLOPS_PER_POINT,
                      "avg_kernel_ms": avg_kernel_s * 1e3},

@@ -95,6 +95,7 @@ PROTOTYPES = {
     "mi_vegas_get_grid": ([VP, P(D), P(I), P(I)], I),
     "mi_vegas_set_grid": ([VP, P(D), I, I], I),
     "mi_unweight_compact": ([VP, P(MiRun), D, P(MiEvents), P(U64), P(D), P(MiSigmaEst)], I),
+    "mi_write_lhe": ([VP, C.c_char_p, P(MiPhysics), P(MiEvents), U64, U64, P(MiSigmaEst)], I),
     "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, I, P(D), VP], I),
     "mi_adjoint": ([VP, P(MiRun), P(MiSigmaGrad)], I),
     "mi_flow_init": ([VP, U64], I),

@@ -237,6 +237,15 @@ class Context:
                     wused.value, SigmaEstimate(est))
         return ev
 
+    def write_lhe(self, path: str, phys: A.MiPhysics, ev: "Events", key: int) -> None:
+        """LHEF 3.0 file of compacted events (flavours, colour flow, lab momenta; pb)."""
+        raw = A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
+                         _ptr(ev.index), ev.n)
+        est = ev.estimate.raw if ev.estimate is not None else A.MiSigmaEst()
+        self.sync_stream()
+        self._check(self.lib.mi_write_lhe(self.h, path.encode(), C.byref(phys), C.byref(raw), ev.n, key,
+                                          C.byref(est)), "mi_write_lhe")
+
     # -------------------------------------------------------------- reweighting
     def reweight_scales(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, w: torch.Tensor,
                         factors) -> torch.Tensor:

@@ -220,6 +220,16 @@ int mi_vegas_set_grid(mi_ctx *ctx, const double *edges_host, int ndim, int nbins
 int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_events *out,
                         uint64_t *n_acc, double *wmax_used, mi_sigma_est *est);
 
+/* -------------------------------------------------------------- LHE output ---- */
+/* LHEF 3.0 file of n compacted events (device SoA, e.g. from mi_unweight_compact): incoming
+ * flavours drawn from the process's flux list with weight xf1 xf2 dsigma/dt, outgoing flavours and
+ * colour flow per Pythia's setIdColAcol, azimuth uniform -- all from Philox streams 3 and 4 of
+ * the event index under `key`; momenta in the lab; weights and cross sections (est->sigmaGen per
+ * process code) in pb.  Replaces cuPythia/pipeline/lhe_writer.cu:46-70 and
+ * cuPythia/07_unweighting_lhe/unweight_lhe.cu:74-102. */
+int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phys, const mi_events *ev, uint64_t n,
+                 uint64_t key, const mi_sigma_est *est);
+
 /* ------------------------------------------------------------ reweighting ---- */
 /* Renormalisation-scale variations of events in one pass: out[i * nvar + k] =
  * w[i] * (alphaS(f_k^2 pT^2) / alphaS(renorm_fac^2 pT^2))^2, pT^2 = tHat uHat / sHat (massless),
