@@ -20,7 +20,7 @@
 //
 // FP32 arithmetic here is bit-reproducible on the host: only correctly rounded IEEE operations,
 // explicit fmaf, no contraction (the library is compiled with -ffp-contract=off), and the own
-// exp/log below (Cody-Waite reduction + Taylor polynomial; atanh series).  That is what makes
+// exp/log below (Cody-Waite reduction + fitted polynomial; table-driven log).  That is what makes
 // the unweighting decisions, and hence the compacted event indices, bit-exact.
 #pragma once
 
@@ -80,60 +80,132 @@ __device__ __forceinline__ double u01(uint64_t x) { return (double)(x >> 11) * (
 
 // ------------------------------------------------------------------ own FP32 exp / log
 // Branch-free forms (selects instead of branches, so a wave never runs both sides of a divergent
-// test); the results are identical to the straightforward branching definitions.
+// test).  Every operation is correctly rounded (fma, mul, add, rint, ldexp, int <-> float), so
+// the host restatement reproduces each result bit for bit.
+//
+// exp: Cody-Waite reduction x = n ln2 + r with ln2 split hi/lo, |r| <= ln2/2; e^r by a degree-6
+// polynomial fitted for minimum relative error by Lawson's iteratively reweighted least squares
+// (coefficients rounded to FP32 one by one, refitting the rest; relative error 3.1e-9);
+// scaling by 2^n with ldexp.  Results that would leave the normal range are flushed to 0.
 __device__ __forceinline__ float own_expf(float x)
 {
-    const bool under = x < -86.0f;                         // result would leave the normal range: 0
+    const bool under = x < -86.0f;
     const float xc = fminf(fmaxf(x, -86.0f), 88.0f);
     const float n = rintf(xc * 1.4426950408889634f);
     float r = __builtin_fmaf(n, -0.693145751953125f, xc);
     r = __builtin_fmaf(n, -1.4286068203094172e-6f, r);
-    float p = 1.9841269841269841e-4f;
-    p = __builtin_fmaf(p, r, 1.3888888888888889e-3f);
-    p = __builtin_fmaf(p, r, 8.3333333333333333e-3f);
-    p = __builtin_fmaf(p, r, 4.1666666666666667e-2f);
-    p = __builtin_fmaf(p, r, 1.6666666666666667e-1f);
-    p = __builtin_fmaf(p, r, 0.5f);
+    float p = 0.001382148f;
+    p = __builtin_fmaf(p, r, 0.008368663f);
+    p = __builtin_fmaf(p, r, 0.041668255f);
+    p = __builtin_fmaf(p, r, 0.16666521f);
+    p = __builtin_fmaf(p, r, 0.49999994f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    const float v = p * __int_as_float(((int)n + 127) << 23);
+    const float v = __builtin_ldexpf(p, (int)n);
     return under ? 0.0f : v;
 }
 
-// 1 - e^x (x <= 0) without cancellation near 0; ex = e^x already computed: Taylor series of
-// -(e^x - 1) to x^8 for x > -0.35, 1 - ex below.
+// 1 - e^x (x <= 0) without cancellation near 0; ex = e^x already computed.  For x > -0.35:
+// -x q(x) with q(x) ~ (e^x - 1)/x of degree 5 (same fitting method, relative error 2.5e-10);
+// below that 1 - ex, which is >= 0.295 so the subtraction is benign.
 __device__ __forceinline__ float one_minus_expf(float x, float ex)
 {
-    float p = __builtin_fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
-    p = __builtin_fmaf(x, p, 1.3888888888888889e-3f);
-    p = __builtin_fmaf(x, p, 8.3333333333333333e-3f);
-    p = __builtin_fmaf(x, p, 4.1666666666666667e-2f);
-    p = __builtin_fmaf(x, p, 1.6666666666666667e-1f);
-    p = __builtin_fmaf(x, p, 0.5f);
+    float p = __builtin_fmaf(x, 0.0011900244f, 0.008254852f);
+    p = __builtin_fmaf(x, p, 0.041652385f);
+    p = __builtin_fmaf(x, p, 0.16666552f);
+    p = __builtin_fmaf(x, p, 0.49999997f);
     p = __builtin_fmaf(x, p, 1.0f);
     const float series = -(x * p);
     return (x > -0.35f) ? series : 1.0f - ex;
 }
 
+// log: table-driven reduction after P. T. P. Tang, "Table-driven implementation of the logarithm
+// function in IEEE floating-point arithmetic", ACM TOMS 16 (1990) 378-400.  x = 2^e m, m in [1, 2);
+// entry j = top 7 mantissa bits holds G_j = RN(1 / (1 + (j + 1/2)/128)) and
+// L_j = RN(-log(G_j) - k_j ln2), k_j = 1 for j >= 53 (those entries fold m into [0.707, 1), so the
+// result keeps relative accuracy just below x = 1); k_j is added to e by the carry of
+// ix + (75 << 16).  r = m G_j - 1 (one fma, |r| <= 2^-8), log1p(r) ~ r - r^2/2 + r^3/3
+// (truncation < 6e-11), log x = (e + k_j) ln2 + L_j + log1p(r).  No division.
+static __constant__ float2 kLogTab[128] = {
+    {0x1.fe01fep-1f, 0x1.ff00acp-9f}, {0x1.fa11cap-1f, 0x1.7dc49ep-7f},
+    {0x1.f6310ap-1f, 0x1.3cea5ep-6f}, {0x1.f25f64p-1f, 0x1.b9fc0ap-6f},
+    {0x1.ee9c8p-1f, 0x1.1b0d9p-5f}, {0x1.eae808p-1f, 0x1.58a5b6p-5f},
+    {0x1.e741aap-1f, 0x1.95c836p-5f}, {0x1.e3a918p-1f, 0x1.d276b2p-5f},
+    {0x1.e01e02p-1f, 0x1.075982p-4f}, {0x1.dca01ep-1f, 0x1.253f62p-4f},
+    {0x1.d92f22p-1f, 0x1.42edcep-4f}, {0x1.d5cac8p-1f, 0x1.60658ap-4f},
+    {0x1.d272cap-1f, 0x1.7da76ap-4f}, {0x1.cf26e6p-1f, 0x1.9ab422p-4f},
+    {0x1.cbe6dap-1f, 0x1.b78c7ep-4f}, {0x1.c8b266p-1f, 0x1.d4313ap-4f},
+    {0x1.c5894ep-1f, 0x1.f0a304p-4f}, {0x1.c26b54p-1f, 0x1.06715p-3f},
+    {0x1.bf583ep-1f, 0x1.14785cp-3f}, {0x1.bc4fd6p-1f, 0x1.2266f4p-3f},
+    {0x1.b951e2p-1f, 0x1.303d74p-3f}, {0x1.b65e2ep-1f, 0x1.3dfc2cp-3f},
+    {0x1.b37484p-1f, 0x1.4ba372p-3f}, {0x1.b094b4p-1f, 0x1.59338ap-3f},
+    {0x1.adbe88p-1f, 0x1.66acd4p-3f}, {0x1.aaf1d2p-1f, 0x1.740f94p-3f},
+    {0x1.a82e66p-1f, 0x1.815c06p-3f}, {0x1.a5741p-1f, 0x1.8e929p-3f},
+    {0x1.a2c2a8p-1f, 0x1.9bb366p-3f}, {0x1.a01a02p-1f, 0x1.a8becep-3f},
+    {0x1.9d79f2p-1f, 0x1.b5b518p-3f}, {0x1.9ae24ep-1f, 0x1.c29688p-3f},
+    {0x1.9852fp-1f, 0x1.cf635ap-3f}, {0x1.95cbbp-1f, 0x1.dc1bcep-3f},
+    {0x1.934c68p-1f, 0x1.e8c026p-3f}, {0x1.90d4f2p-1f, 0x1.f550ap-3f},
+    {0x1.8e6528p-1f, 0x1.00e6c4p-2f}, {0x1.8bfce8p-1f, 0x1.071b86p-2f},
+    {0x1.899c1p-1f, 0x1.0d46b4p-2f}, {0x1.87427cp-1f, 0x1.13687p-2f},
+    {0x1.84f00cp-1f, 0x1.1980d4p-2f}, {0x1.82a4ap-1f, 0x1.1f8ffap-2f},
+    {0x1.806018p-1f, 0x1.259602p-2f}, {0x1.7e2256p-1f, 0x1.2b9302p-2f},
+    {0x1.7beb3ap-1f, 0x1.31871ap-2f}, {0x1.79baa6p-1f, 0x1.377268p-2f},
+    {0x1.779082p-1f, 0x1.3d54f8p-2f}, {0x1.756cacp-1f, 0x1.432ef2p-2f},
+    {0x1.734f0cp-1f, 0x1.490068p-2f}, {0x1.713786p-1f, 0x1.4ec976p-2f},
+    {0x1.6f2602p-1f, 0x1.548a2ap-2f}, {0x1.6d1a62p-1f, 0x1.5a42acp-2f},
+    {0x1.6b149p-1f, 0x1.5ff308p-2f}, {0x1.691474p-1f, -0x1.602d0ap-2f},
+    {0x1.6719f4p-1f, -0x1.5a8cbp-2f}, {0x1.6524f8p-1f, -0x1.54f43p-2f},
+    {0x1.63356cp-1f, -0x1.4f638p-2f}, {0x1.614b36p-1f, -0x1.49da7ep-2f},
+    {0x1.5f6644p-1f, -0x1.44592p-2f}, {0x1.5d867cp-1f, -0x1.3edf46p-2f},
+    {0x1.5babccp-1f, -0x1.396ce2p-2f}, {0x1.59d62p-1f, -0x1.3401e4p-2f},
+    {0x1.58056p-1f, -0x1.2e9e2cp-2f}, {0x1.56397cp-1f, -0x1.2941bp-2f},
+    {0x1.54725ep-1f, -0x1.23ec58p-2f}, {0x1.52aff6p-1f, -0x1.1e9e18p-2f},
+    {0x1.50f22ep-1f, -0x1.1956d4p-2f}, {0x1.4f38f6p-1f, -0x1.14167ep-2f},
+    {0x1.4d843cp-1f, -0x1.0edd06p-2f}, {0x1.4bd3eep-1f, -0x1.09aa58p-2f},
+    {0x1.4a27fap-1f, -0x1.047e5ep-2f}, {0x1.488052p-1f, -0x1.feb222p-3f},
+    {0x1.46dce4p-1f, -0x1.f474b6p-3f}, {0x1.453d9ep-1f, -0x1.ea4448p-3f},
+    {0x1.43a274p-1f, -0x1.e020d2p-3f}, {0x1.420b52p-1f, -0x1.d60a16p-3f},
+    {0x1.40782ep-1f, -0x1.cc0012p-3f}, {0x1.3ee8f4p-1f, -0x1.c2028ap-3f},
+    {0x1.3d5d9ap-1f, -0x1.b81178p-3f}, {0x1.3bd60ep-1f, -0x1.ae2caap-3f},
+    {0x1.3a5244p-1f, -0x1.a4540cp-3f}, {0x1.38d22ep-1f, -0x1.9a877ep-3f},
+    {0x1.3755bep-1f, -0x1.90c6e2p-3f}, {0x1.35dce6p-1f, -0x1.871214p-3f},
+    {0x1.34679ap-1f, -0x1.7d68fep-3f}, {0x1.32f5cep-1f, -0x1.73cb8ap-3f},
+    {0x1.318776p-1f, -0x1.6a39ap-3f}, {0x1.301c82p-1f, -0x1.60b30cp-3f},
+    {0x1.2eb4eap-1f, -0x1.5737ccp-3f}, {0x1.2d50ap-1f, -0x1.4dc7b8p-3f},
+    {0x1.2bef98p-1f, -0x1.4462b4p-3f}, {0x1.2a91cap-1f, -0x1.3b08bcp-3f},
+    {0x1.293726p-1f, -0x1.31b996p-3f}, {0x1.27dfa4p-1f, -0x1.28753ep-3f},
+    {0x1.268b38p-1f, -0x1.1f3b94p-3f}, {0x1.2539d8p-1f, -0x1.160c8p-3f},
+    {0x1.23eb7ap-1f, -0x1.0ce7fp-3f}, {0x1.22a012p-1f, -0x1.03cdcp-3f},
+    {0x1.215798p-1f, -0x1.f57bc8p-4f}, {0x1.201202p-1f, -0x1.e3708cp-4f},
+    {0x1.1ecf44p-1f, -0x1.d1797cp-4f}, {0x1.1d8f56p-1f, -0x1.bf968p-4f},
+    {0x1.1c523p-1f, -0x1.adc782p-4f}, {0x1.1b17c6p-1f, -0x1.9c0c2cp-4f},
+    {0x1.19e012p-1f, -0x1.8a647ep-4f}, {0x1.18ab08p-1f, -0x1.78d02p-4f},
+    {0x1.1778a2p-1f, -0x1.674f0ep-4f}, {0x1.1648d6p-1f, -0x1.55e10ep-4f},
+    {0x1.151b9ap-1f, -0x1.4485dcp-4f}, {0x1.13f0e8p-1f, -0x1.333d74p-4f},
+    {0x1.12c8b8p-1f, -0x1.2207acp-4f}, {0x1.11a302p-1f, -0x1.10e462p-4f},
+    {0x1.107fbcp-1f, -0x1.ffa694p-5f}, {0x1.0f5eep-1f, -0x1.dda8b8p-5f},
+    {0x1.0e4066p-1f, -0x1.bbced4p-5f}, {0x1.0d2446p-1f, -0x1.9a188ep-5f},
+    {0x1.0c0a78p-1f, -0x1.78858ap-5f}, {0x1.0af2f8p-1f, -0x1.5715ep-5f},
+    {0x1.09ddbap-1f, -0x1.35c8b2p-5f}, {0x1.08cabcp-1f, -0x1.149e56p-5f},
+    {0x1.07b9f2p-1f, -0x1.e72bccp-6f}, {0x1.06ab5ap-1f, -0x1.a55f62p-6f},
+    {0x1.059eeap-1f, -0x1.63d616p-6f}, {0x1.04949cp-1f, -0x1.228f82p-6f},
+    {0x1.038c6cp-1f, -0x1.c3177cp-7f}, {0x1.02865p-1f, -0x1.4192bcp-7f},
+    {0x1.018244p-1f, -0x1.8121bcp-8f}, {0x1.00804p-1f, -0x1.003fd6p-9f},
+};
+
 __device__ __forceinline__ float own_logf(float x)
 {
     const bool bad = !(x > 0.0f);                          // only 1 - x == 0 reaches here
-    const int ix = __float_as_int(bad ? 1.0f : x);
-    int e = ((ix >> 23) & 0xff) - 127;
-    float m = __int_as_float((ix & 0x007fffff) | 0x3f800000);
-    const bool big = m > 1.41421356f;
-    m = big ? m * 0.5f : m;
-    e = big ? e + 1 : e;
-    const float f = m - 1.0f;
-    const float s = f / (m + 1.0f);
-    const float s2 = s * s;
-    float p = __builtin_fmaf(s2, 0.11111111111111111f, 0.14285714285714286f);
-    p = __builtin_fmaf(s2, p, 0.2f);
-    p = __builtin_fmaf(s2, p, 0.33333333333333333f);
-    const float two_s = s + s;
-    const float lm = __builtin_fmaf(two_s * s2, p, two_s);
-    const float fe = (float)e;
-    const float v = __builtin_fmaf(fe, 0.693145751953125f, __builtin_fmaf(fe, 1.4286068203094172e-6f, lm));
+    const uint32_t ix = __float_as_uint(bad ? 1.0f : x);
+    const float fe = (float)((ix + 0x004B0000u) >> 23) - 127.0f;
+    const uint32_t j = (ix >> 16) & 127u;
+    const float m = __uint_as_float((ix & 0x007fffffu) | 0x3f800000u);
+    const float2 t = kLogTab[j];
+    const float r = __builtin_fmaf(m, t.x, -1.0f);
+    const float r2 = r * r;
+    const float p = __builtin_fmaf(r2, __builtin_fmaf(r, 0.333333343f, -0.5f), r);
+    const float u = t.y + __builtin_fmaf(fe, 1.4286068203094172e-6f, p);
+    const float v = __builtin_fmaf(fe, 0.693145751953125f, u);
     return bad ? -1.0e30f : v;
 }
 
@@ -178,20 +250,22 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     const float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
     const float x1 = own_expf(lx1), x2 = own_expf(lx2);
     const float zMax = sqrtf(one_minus_expf(targ, r));
-    const float omz = r / (1.0f + zMax);
+    // one division gives both 1 / r and 1 / (1 + zMax)
+    const float opz = 1.0f + zMax;
+    const float R1 = 1.0f / (r * opz);
+    const float omz = (r * r) * R1;                        // r / (1 + zMax) = 1 - zMax
+    const float sH = (k.fourPT2 * opz) * R1;               // 4 pTmin^2 / r
     const float twoz = zMax + zMax;
     const float A = __builtin_fmaf(twoz, om2, omz);        // 1 - z
     const float B = __builtin_fmaf(twoz, u2, omz);         // 1 + z
     const float a = 0.5f * A, b = 0.5f * B;
     const float ab = a * b;
-    const float inv_ab = 1.0f / ab;
-    const float ia = b * inv_ab, ib = a * inv_ab;
-    const float sH = k.fourPT2 / r;
     const float pT2 = sH * ab;
 
-    float alpS, cfo = 0.0f;
+    float alpS, inv_ab, cfo = 0.0f;
     if (k.as_order == 0.0f) {
         alpS = k.as_fixed;
+        inv_ab = 1.0f / ab;
     } else {
         float Q2 = k.xiR2 * pT2;
         Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
@@ -199,9 +273,14 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
         const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0f, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
         const float lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
         const float cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
-        alpS = cf / (lq - lam);
+        // one division gives both 1 / (a b) and alphaS = cf / (lq - lam)
+        const float den = lq - lam;
+        const float R2 = 1.0f / (ab * den);
+        inv_ab = den * R2;
+        alpS = (cf * ab) * R2;
         cfo = cf;
     }
+    const float ia = b * inv_ab, ib = a * inv_ab;
 
     const float l1a = own_logf(one_minus_expf(lx1, x1)), l1b = own_logf(one_minus_expf(lx2, x2));
     const float g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));

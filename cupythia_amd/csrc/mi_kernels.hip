@@ -9,6 +9,7 @@
 // through LDS; one record per block goes to global memory; a single-block kernel folds the
 // block records in a fixed order.  The result is bit-identical from run to run.
 #include "mi_kernel_common.h"
+#include <type_traits>
 
 namespace mi {
 
@@ -97,6 +98,12 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
     return hipGetLastError();
 }
 
+// Running maximum of non-negative weights: FP32 as the integer order of the bit patterns, FP64 as fmax.
+__device__ __forceinline__ uint32_t run_max(uint32_t m, float w) { return max(m, __float_as_uint(w)); }
+__device__ __forceinline__ double run_max(double m, double w) { return fmax(m, w); }
+template <typename T> __device__ __forceinline__ double max_value(uint32_t m) { return (double)__uint_as_float(m); }
+template <typename T> __device__ __forceinline__ double max_value(double m) { return m; }
+
 // ------------------------------------------------------------------------------------ hadronic integration
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
 template <typename T, bool VEGAS, bool HIST>
@@ -121,6 +128,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
     double v[kRec];
 #pragma unroll
     for (int i = 0; i < kRec; ++i) v[i] = 0.0;
+    // Running maxima as the weights' own type: the weights are >= 0 (never NaN), so for FP32 the
+    // maximum is an integer max of the bit patterns.  Converted to FP64 once at the end, which is
+    // exact and order-preserving.  The nonzero count is an integer.
+    using MaxT = typename std::conditional<sizeof(T) == 4, uint32_t, double>::type;
+    MaxT mx[kMaxProc + 1];
+#pragma unroll
+    for (int i = 0; i <= kMaxProc; ++i) mx[i] = 0;
+    uint32_t nnz = 0;
 
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
     for (uint32_t j = 0; j < ppt; ++j) {
@@ -133,14 +148,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
         const double dt = (double)tot;
         v[0] += dt;
         v[7] = fma(dt, dt, v[7]);
-        v[14] = fmax(v[14], dt);
-        v[21] += (tot > (T)0) ? 1.0 : 0.0;
+        mx[0] = run_max(mx[0], tot);
+        nnz += (tot > (T)0) ? 1u : 0u;
 #pragma unroll
         for (int p = 0; p < kMaxProc; ++p) {
             const double d = (double)ws[p];
             v[1 + p] += d;
             v[8 + p] = fma(d, d, v[8 + p]);
-            v[15 + p] = fmax(v[15 + p], d);
+            mx[1 + p] = run_max(mx[1 + p], ws[p]);
         }
         if (HIST) {
             const double w2 = dt * dt;
@@ -150,6 +165,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
             atomicAdd(&h[2 * nbins + bins[2]], w2);
         }
     }
+#pragma unroll
+    for (int i = 0; i <= kMaxProc; ++i) v[14 + i] = max_value<T>(mx[i]);
+    v[21] = (double)nnz;
     block_record(v, partials + (size_t)blockIdx.x * kRec);
     if (HIST) {
         __syncthreads();

@@ -143,6 +143,8 @@ uint64_t orc_c1_unweight_compat(uint64_t seed, uint64_t n, double s, double alpS
    against the FP64 Pythia-form oracle above. */
 float orc_expf(float x);
 float orc_logf(float x);
+/* the 128-entry table of orc_logf (and of the device log): G_j, L_j */
+void orc32_logtab(float G[128], float L[128]);
 /* constant block shared by host and device (see orc32_consts_fill for the layout). */
 #define ORC32_NCONST 32
 void  orc32_consts_fill(const orc_hadcfg *c, float k[ORC32_NCONST]);

@@ -87,6 +87,7 @@ def lib():
         L.orc_c1_unweight_compat.restype = u64
         L.orc_expf.argtypes = [C.c_float]; L.orc_expf.restype = C.c_float
         L.orc_logf.argtypes = [C.c_float]; L.orc_logf.restype = C.c_float
+        L.orc32_logtab.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float)]; L.orc32_logtab.restype = None
         L.orc32_consts_fill.argtypes = [P(OrcHadCfg), P(C.c_float)]
         L.orc32_had_point.argtypes = [P(C.c_float), i, C.c_float, C.c_float, C.c_float,
                                       P(C.c_float), P(C.c_float)]

@@ -8,12 +8,16 @@
  * operations (+ - * / sqrt fma rint and exact bit manipulation), the same operation order,
  * no contraction (this file is compiled with -ffp-contract=off), and this project's own
  * exp/log:
- *   exp: Cody-Waite reduction x = n ln2 + r (ln2 split hi/lo), Taylor polynomial of e^r to
- *        degree 7 on |r| <= ln2/2 (truncation 5e-9 relative), scaling by 2^n via the
- *        exponent field (W. J. Cody, W. Waite, "Software Manual for the Elementary
- *        Functions", 1980).
- *   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd
- *        series to s^9 (|s| <= 0.1716, truncation 2e-9 relative).
+ *   exp: Cody-Waite reduction x = n ln2 + r (ln2 split hi/lo; W. J. Cody, W. Waite, "Software
+ *        Manual for the Elementary Functions", 1980), e^r on |r| <= ln2/2 by a degree-6
+ *        polynomial fitted for minimum relative error with Lawson's iteratively reweighted
+ *        least squares (coefficients rounded to FP32 one at a time, the rest refitted;
+ *        relative error 3.1e-9), scaling by 2^n with ldexpf.
+ *   1 - e^x near 0: -x q(x), q ~ (e^x - 1)/x of degree 5 by the same fit (2.5e-10).
+ *   log: table-driven (P. T. P. Tang, ACM TOMS 16 (1990) 378-400): x = 2^e m, m in [1, 2),
+ *        j = top 7 bits of m, G_j = RN(1/(1 + (j + 1/2)/128)), L_j = RN(-log G_j - k_j ln2)
+ *        with k_j = [j >= 53] folding the upper entries into [0.707, 1); r = fma(m, G_j, -1),
+ *        |r| <= 2^-8, log1p r ~ r - r^2/2 + r^3/3; log x = (e + k_j) ln2 + L_j + log1p r.
  * The accuracy of the whole FP32 weight is pinned against the FP64 Pythia-form oracle
  * (orc_had_point) in tests/test_oracle_physics.py.
  */
@@ -35,51 +39,54 @@ float orc_expf(float x)
     float n = rintf(x * 1.4426950408889634f);
     float r = fmaf(n, -0.693145751953125f, x);
     r = fmaf(n, -1.4286068203094172e-6f, r);
-    float p = 1.9841269841269841e-4f;
-    p = fmaf(p, r, 1.3888888888888889e-3f);
-    p = fmaf(p, r, 8.3333333333333333e-3f);
-    p = fmaf(p, r, 4.1666666666666667e-2f);
-    p = fmaf(p, r, 1.6666666666666667e-1f);
-    p = fmaf(p, r, 0.5f);
+    float p = 0.001382148f;
+    p = fmaf(p, r, 0.008368663f);
+    p = fmaf(p, r, 0.041668255f);
+    p = fmaf(p, r, 0.16666521f);
+    p = fmaf(p, r, 0.49999994f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
-    return p * i2f(((int32_t)n + 127) << 23);
+    return ldexpf(p, (int)n);
 }
 
-/* 1 - e^x for x <= 0: Taylor series of -(e^x - 1) to x^8 when |x| <= 0.35 (no cancellation
- * near x = 0), plain 1 - exp(x) otherwise (then 1 - e^x >= 0.295 and the subtraction is benign). */
+/* 1 - e^x for x <= 0: -x q(x) when x > -0.35 (no cancellation near x = 0), plain 1 - exp(x)
+ * otherwise (then 1 - e^x >= 0.295 and the subtraction is benign). */
 static float one_minus_exp(float x, float ex)
 {
     if (x > -0.35f) {
-        float p = fmaf(x, 2.4801587301587302e-5f, 1.9841269841269841e-4f);
-        p = fmaf(x, p, 1.3888888888888889e-3f);
-        p = fmaf(x, p, 8.3333333333333333e-3f);
-        p = fmaf(x, p, 4.1666666666666667e-2f);
-        p = fmaf(x, p, 1.6666666666666667e-1f);
-        p = fmaf(x, p, 0.5f);
+        float p = fmaf(x, 0.0011900244f, 0.008254852f);
+        p = fmaf(x, p, 0.041652385f);
+        p = fmaf(x, p, 0.16666552f);
+        p = fmaf(x, p, 0.49999997f);
         p = fmaf(x, p, 1.0f);
         return -(x * p);
     }
     return 1.0f - ex;
 }
 
+void orc32_logtab(float G[128], float L[128])
+{
+    for (int j = 0; j < 128; ++j) {
+        G[j] = (float)(1.0 / (1.0 + (j + 0.5) / 128.0));
+        L[j] = (float)(-log((double)G[j]) - (j >= 53 ? log(2.0) : 0.0));
+    }
+}
+
+static float logG[128], logL[128];
+__attribute__((constructor)) static void init_logtab(void) { orc32_logtab(logG, logL); }
+
 float orc_logf(float x)
 {
     if (!(x > 0.0f)) return -1.0e30f;           /* only reached for 1 - x == 0 (x rounds to 1) */
-    int32_t ix = f2i(x);
-    int32_t e = ((ix >> 23) & 0xff) - 127;
-    float m = i2f((ix & 0x007fffff) | 0x3f800000);
-    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
-    float f = m - 1.0f;
-    float s = f / (m + 1.0f);
-    float s2 = s * s;
-    float p = fmaf(s2, 0.11111111111111111f, 0.14285714285714286f);
-    p = fmaf(s2, p, 0.2f);
-    p = fmaf(s2, p, 0.33333333333333333f);
-    float two_s = s + s;
-    float lm = fmaf(two_s * s2, p, two_s);
-    float fe = (float)e;
-    return fmaf(fe, 0.693145751953125f, fmaf(fe, 1.4286068203094172e-6f, lm));
+    uint32_t ix = (uint32_t)f2i(x);
+    float fe = (float)((ix + 0x004B0000u) >> 23) - 127.0f;
+    uint32_t j = (ix >> 16) & 127u;
+    float m = i2f((int32_t)((ix & 0x007fffffu) | 0x3f800000u));
+    float r = fmaf(m, logG[j], -1.0f);
+    float r2 = r * r;
+    float p = fmaf(r2, fmaf(r, 0.333333343f, -0.5f), r);
+    float u = logL[j] + fmaf(fe, 1.4286068203094172e-6f, p);
+    return fmaf(fe, 0.693145751953125f, u);
 }
 
 /* Constant block (all FP32):
@@ -137,20 +144,22 @@ float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, fl
     float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
     float x1 = orc_expf(lx1), x2 = orc_expf(lx2);
     float zMax = sqrtf(one_minus_exp(-(u0 * LT), r));
-    float omz = r / (1.0f + zMax);                      /* 1 - zMax, cancellation-free */
+    /* one division for 1/r and 1/(1 + zMax) */
+    float opz = 1.0f + zMax;
+    float R1 = 1.0f / (r * opz);
+    float omz = (r * r) * R1;                           /* r/(1 + zMax) = 1 - zMax, cancellation-free */
+    float sH = (k[1] * opz) * R1;                       /* 4 pTmin^2 / r */
     float twoz = zMax + zMax;
     float A = fmaf(twoz, om2, omz);                     /* 1 - z */
     float B = fmaf(twoz, u2, omz);                      /* 1 + z */
     float a = 0.5f * A, b = 0.5f * B;                   /* -t/s, -u/s */
     float ab = a * b;
-    float inv_ab = 1.0f / ab;
-    float ia = b * inv_ab, ib = a * inv_ab;
-    float sH = k[1] / r;
     float pT2 = sH * ab;
 
-    float alpS;
+    float alpS, inv_ab;
     if (k[4] == 0.0f) {
         alpS = k[5];
+        inv_ab = 1.0f / ab;
     } else {
         float Q2 = k[3] * pT2;
         if (Q2 < k[6]) Q2 = k[6];
@@ -160,8 +169,13 @@ float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, fl
         else if (Q2 > k[8])             { lam = k[12]; cf = k[16]; }
         else if (Q2 > k[9])             { lam = k[13]; cf = k[17]; }
         else                            { lam = k[14]; cf = k[18]; }
-        alpS = cf / (lq - lam);
+        /* one division for 1/(a b) and cf/(lq - lam) */
+        float den = lq - lam;
+        float R2 = 1.0f / (ab * den);
+        inv_ab = den * R2;
+        alpS = (cf * ab) * R2;
     }
+    float ia = b * inv_ab, ib = a * inv_ab;
 
     /* parton densities at x1 and x2 */
     float l1a = orc_logf(one_minus_exp(lx1, x1)), l1b = orc_logf(one_minus_exp(lx2, x2));

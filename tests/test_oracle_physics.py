@@ -205,6 +205,27 @@ def test_fp32_mirror_accuracy(O):
     assert np.allclose(k32[:, 0], kld[:, 0], rtol=1e-5)
 
 
+def test_log_table_in_device_source_matches_oracle(O):
+    """The device log's table is a literal in mi_device.h; the oracle computes it from its
+    defining formula.  Both must agree bit for bit for the FP32 mirror to be exact."""
+    import ctypes as C
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "cupythia_amd", "csrc", "mi_device.h")).read()
+    body = src[src.index("kLogTab[128] = {"):]
+    body = body[:body.index("};")]
+    vals = [float.fromhex(t[:-1]) for t in re.findall(r"-?0x[0-9a-fA-F.]+p[-+]?\d+f", body)]
+    assert len(vals) == 256
+    G, L = (C.c_float * 128)(), (C.c_float * 128)()
+    O.lib().orc32_logtab(G, L)
+    assert np.array_equal(np.array(vals[0::2], np.float32), np.array(G[:], np.float32))
+    assert np.array_equal(np.array(vals[1::2], np.float32), np.array(L[:], np.float32))
+    # every entry reduces its mantissa interval to |m G_j - 1| <= 2^-8
+    for j in range(128):
+        for m in (1 + j / 128, 1 + (j + 1) / 128):
+            assert abs(m * G[j] - 1) <= 2.0 ** -8
+
+
 def test_own_exp_log_accuracy(O):
     L = O.lib()
     xs = np.linspace(-80, 80, 20001, dtype=np.float32)
@@ -215,6 +236,17 @@ def test_own_exp_log_accuracy(O):
     lg = np.array([L.orc_logf(float(y)) for y in ys])
     refl = np.log(ys.astype(np.float64))
     assert np.max(np.abs(lg - refl)) < 1e-5 and np.max(np.abs(lg - refl) / np.maximum(np.abs(refl), 1)) < 4e-7
+    # within 2^-8 of 1 (log(1 - x) for small x) the error is absolute, ~ulp(2^-8)/2: what matters
+    # there, since the value enters b log(1 - x) + a log x + log A of size O(1)
+    zs = (1.0 - np.float32(2.0) ** -np.linspace(8, 24, 4001)).astype(np.float32)
+    zs = np.concatenate([zs, (1.0 + np.float32(2.0) ** -np.linspace(8, 23, 4001)).astype(np.float32)])
+    lz = np.array([L.orc_logf(float(z)) for z in zs])
+    refz = np.log(zs.astype(np.float64))
+    assert np.max(np.abs(lz - refz)) < 5e-10
+    # exp: the Cody-Waite + fitted polynomial keeps ~1 ulp over the whole range used
+    xs2 = np.linspace(-86, 0, 40001, dtype=np.float32)
+    e2 = np.array([L.orc_expf(float(x)) for x in xs2])
+    assert np.max(np.abs(e2 - np.exp(xs2.astype(np.float64))) / np.exp(xs2.astype(np.float64))) < 2.5e-7
 
 
 def test_hadronic_quadrature_converges_and_matches_mc(O):
