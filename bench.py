@@ -9,7 +9,9 @@ nothing crosses PCIe inside the timed region except the 24-double result of each
 
 Prints one JSON line: weighted events/s (value), the roofline of the integration kernel (its
 duration measured with a hipEvent pair on its own stream, inside the library), the CPU baseline
-(the Pythia-form FP64 oracle on the host cores, bounded sample), and, with --tts, the
+(the Pythia-form FP64 oracle on the host cores, bounded sample), the other workloads of the
+path (VEGAS 12 x 2^24 + 2^26-candidate unweighting, the adjoint pass at 2^26, the flow trained on
+2^20-point batches and integrating 2^26 points, with its matrix-core roofline), and the
 time-to-solution at 0.1% relative error of plain VEGAS and flow + VEGAS.
 """
 import argparse
@@ -68,11 +70,13 @@ def time_to_solution(ctx, mi, target=1e-3, flow=False):
     info = ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
     gpu_ms = info.kernel_ms
     sampler = mi.MI_SAMPLER_VEGAS
+    extra = {"adapt_ms": info.kernel_ms}
     if flow:
         ctx.flow_init(7)
         fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), 200, 1 << 16, 2e-3)
         gpu_ms += fi.kernel_ms
         sampler = mi.MI_SAMPLER_FLOW
+        extra.update(train_ms=fi.kernel_ms, cv_first_batch=fi.cv_vegas, cv_last_batch=fi.cv_flow)
     # pilot for the variance, then the final pass sized for the target
     pilot = ctx.integrate(mi.make_run(phys, 1 << 20, key=44, sampler=sampler))
     gpu_ms += pilot.kernel_ms
@@ -83,7 +87,63 @@ def time_to_solution(ctx, mi, target=1e-3, flow=False):
     gpu_ms += fin.kernel_ms
     wall = time.perf_counter() - t0
     return {"gpu_ms": gpu_ms, "wall_s": wall, "points_final": n, "sigma": fin.sigmaGen(),
-            "rel_err": fin.sigmaErr() / fin.sigmaGen()}
+            "rel_err": fin.sigmaErr() / fin.sigmaGen(), "pilot_rel_err": rel, "final_ms": fin.kernel_ms, **extra}
+
+
+# Flow: per layer the MLP 2 -> 64 -> 64 -> 49 is 2*64 + 64*64 + 64*49 = 7360 multiply-adds per
+# point; 6 layers -> 88,320 flops per point for one pass (the matrix-core part, 64x64x64 BF16
+# tiles, pads the 49 outputs to 64).  Training = one inverse pass + backward (~2x forward).
+FLOW_FLOPS_PER_POINT = 2 * 6 * 7360
+PEAK_BF16_TFLOPS = 2500.0         # MI355X dense BF16 matrix peak
+
+
+def workloads(ctx, mi):
+    """The other configurations of the path, timed on the device (sum of kernel times):
+    VEGAS adaptation 12 x 2^24 + 2^26-candidate unweighting, the adjoint pass at 2^26 points,
+    and the flow (training on 2^20-point batches, then a 2^26-point integration)."""
+    import torch
+    phys = mi.physics_default(1)
+    out = {}
+    # VEGAS + unweighting (3-D grid, 64 bins, alpha 1.5)
+    ctx.vegas_init(3, 64)
+    vi = ctx.vegas_adapt(mi.make_run(phys, 1 << 24, key=42), 12, 1.5)
+    run = mi.make_run(phys, 1 << 26, key=42, counter0=1 << 32, sampler=mi.MI_SAMPLER_VEGAS)
+    ev = ctx.unweight_compact(run)
+    torch.cuda.synchronize()
+    nb = ev.n * (5 * 8 + 4 + 8)
+    out["vegas_unweight"] = {
+        "vegas_points": 12 << 24, "vegas_ms": vi.kernel_ms, "vegas_points_per_s": (12 << 24) / (vi.kernel_ms / 1e3),
+        "sigma_mb": vi.sigma, "sigma_err_mb": vi.sigma_err, "chi2_dof": vi.chi2_dof, "eta_last": vi.last_eta,
+        "unweight_candidates": 1 << 26, "accepted": ev.n, "unweight_ms": ev.estimate.kernel_ms,
+        "candidates_per_s": (1 << 26) / (ev.estimate.kernel_ms / 1e3), "event_bytes": nb,
+        "note": "unweight_ms covers the exact-max pass, count, scan and write launches"}
+    # adjoint
+    g = ctx.adjoint(mi.make_run(phys, 1 << 26, key=42))
+    out["adjoint"] = {"points": 1 << 26, "kernel_ms": g.kernel_ms, "points_per_s": (1 << 26) / (g.kernel_ms / 1e3),
+                      "sigma_mb": g.sigma, "d_alphaS_MZ": g.d_alphaS_MZ, "d_renorm_fac": g.d_renorm_fac,
+                      "d_pdf_g_a": g.d_pdf_g_a, "d_pdf_g_b": g.d_pdf_g_b}
+    # flow on top of the adapted grid
+    ctx.flow_init(7)
+    steps, batch = 40, 1 << 20
+    fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), steps, batch, 2e-3)
+    fe = ctx.integrate(mi.make_run(phys, 1 << 26, key=47, sampler=mi.MI_SAMPLER_FLOW))
+    ve = ctx.integrate(mi.make_run(phys, 1 << 26, key=47, sampler=mi.MI_SAMPLER_VEGAS))
+    n = 1 << 26
+    cvf = fe.sigmaErr() / fe.sigmaGen() * math.sqrt(n)
+    cvv = ve.sigmaErr() / ve.sigmaGen() * math.sqrt(n)
+    achieved = FLOW_FLOPS_PER_POINT * n / (fe.kernel_ms / 1e3) / 1e12
+    train_tf = 3 * FLOW_FLOPS_PER_POINT * steps * batch / (fi.kernel_ms / 1e3) / 1e12
+    out["flow"] = {
+        "train_steps": steps, "train_batch": batch, "train_ms": fi.kernel_ms,
+        "train_ms_per_step": fi.kernel_ms / steps, "train_tflops_at_3x_forward": train_tf,
+        "cv_first_batch": fi.cv_vegas, "cv_last_batch": fi.cv_flow,
+        "integrate_points": n, "integrate_ms": fe.kernel_ms, "points_per_s": n / (fe.kernel_ms / 1e3),
+        "sigma_mb": fe.sigmaGen(), "sigma_err_mb": fe.sigmaErr(), "cv_flow": cvf,
+        "vegas_same_points": {"ms": ve.kernel_ms, "sigma_mb": ve.sigmaGen(), "sigma_err_mb": ve.sigmaErr(), "cv": cvv},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "flops_per_point": FLOW_FLOPS_PER_POINT,
+                     "note": "flow-sampler integration: all launches (6 spline-coupling layers, weights, reductions)"}}
+    return out
 
 
 def main():
@@ -94,7 +154,11 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 28)
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tts", action="store_true", help="also measure time-to-solution (VEGAS, flow+VEGAS)")
+    ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution comparison")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the VEGAS / unweighting, adjoint and flow workloads")
+    ap.add_argument("--only", choices=["tts", "workloads"],
+                    help="run only that part (for profiling) and print its JSON")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc: 3 integration launches and one 1 GiB device copy (the "
                          "calibration of FETCH_SIZE / WRITE_SIZE), nothing else")
@@ -111,6 +175,15 @@ def main():
     ctx = mi.Context(0)
     phys = mi.physics_default(1)
     n = args.points
+    if args.only == "tts":
+        print(json.dumps({"time_to_solution": {"vegas": time_to_solution(ctx, mi, flow=False),
+                                               "flow_vegas": time_to_solution(ctx, mi, flow=True)}}))
+        ctx.close()
+        return
+    if args.only == "workloads":
+        print(json.dumps({"workloads": workloads(ctx, mi)}))
+        ctx.close()
+        return
     if args.pmc_probe:
         for k in range(3):
             ctx.integrate(mi.make_run(phys, n, key=42, counter0=k * n))
@@ -161,7 +234,9 @@ def main():
         nthr = min(16, os.cpu_count() or 1)
         rateN, dtN, _ = cpu_baseline(args.cpu_sample * 4, nthr)
         out["cpu_baseline"]["all_cores"] = {"value": rateN, "cores": nthr, "seconds": dtN}
-    if args.tts:
+    if not args.no_workloads:
+        out["workloads"] = workloads(ctx, mi)
+    if not args.no_tts:
         tv = time_to_solution(ctx, mi, flow=False)
         out["time_to_solution"] = {"vegas": tv}
         try:
