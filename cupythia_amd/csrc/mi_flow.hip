@@ -34,8 +34,11 @@ constexpr int FPL = MI_FLOW_NPARAMS / MI_FLOW_LAYERS;   // 7537 parameters per l
 constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
 static_assert(PB3 + FO == FPL, "flow parameter layout");
 constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
-constexpr int HS = 72;      // LDS row stride (bf16) of hidden activations
-constexpr int OS = 68;      // LDS row stride (fp32) of outputs / gradients
+constexpr int HS = 72;      // LDS row stride (bf16) of 64-wide hidden activations (144 B rows)
+constexpr int OS = 52;      // LDS row stride (fp32) of the 49 MLP outputs (208 B rows, 16 B aligned)
+constexpr int GS = 68;      // LDS row stride (fp32) of 64-wide gradient rows
+constexpr int FWF = 4;      // waves per block of the forward / inverse layer kernel
+constexpr int FWB = 3;      // waves per block of the backward layer kernel
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -51,23 +54,43 @@ struct PackedLayer {
 };
 static_assert(sizeof(PackedLayer) % 16 == 0, "PackedLayer copies as uint4");
 
-__device__ __forceinline__ uint16_t f2bf(float f)
+// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32 converts two at a time).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);            // round to nearest even
-    return (uint16_t)(u >> 16);
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
-__device__ __forceinline__ uint16_t tobits(uint16_t v) { return v; }
-__device__ __forceinline__ uint16_t tobits(float v) { return f2bf(v); }
 
-template <typename E>
-__device__ __forceinline__ bf16x8 frag8(const E *p, int stride)
+// Orders the LDS accesses of one wave around a point where lanes start reading what other lanes
+// of the same wave wrote (the LDS executes one wave's operations in issue order; this keeps the
+// compiler from moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Eight operand elements p[0], p[stride], ... as one MFMA fragment (FP32 sources are rounded).
+__device__ __forceinline__ bf16x8 frag8(const uint16_t *p, int stride)
 {
     bf16x8 r;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, tobits(p[j * stride]));
+    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, p[j * stride]);
     return r;
+}
+__device__ __forceinline__ bf16x8 frag8(const float *p, int stride)
+{
+    uint4 w;
+    w.x = pack_bf16x2(p[0], p[stride]);
+    w.y = pack_bf16x2(p[2 * stride], p[3 * stride]);
+    w.z = pack_bf16x2(p[4 * stride], p[5 * stride]);
+    w.w = pack_bf16x2(p[6 * stride], p[7 * stride]);
+    return __builtin_bit_cast(bf16x8, w);
 }
 
 struct Acc { f32x16 c[2][2]; };
@@ -120,6 +143,86 @@ __device__ __forceinline__ void acc_each(const Acc &a, F fn)
             }
 }
 
+// acc holds C = A W^T for A (64 x 64, row m at A + m*as, k contiguous) and W (64 x 64, row n at
+// W + n*ws, k contiguous), computed with the operands swapped so that the result comes out
+// transposed: lane l owns row m = 32*mb + (l & 31) and the columns n = 32*nb + 8q + 4*(l/32) + r,
+// q, r = 0..3, in acc.c[mb][nb][4q + r] -- four consecutive columns per group, which the stores
+// below write as one 8-byte (bf16) or 16-byte (fp32) LDS access.
+template <typename EW>
+__device__ __forceinline__ void wgemm_nt(Acc &acc, const uint16_t *A, int as, const EW *W, int ws)
+{
+    const int l = threadIdx.x & 63, r = l & 31, hh = l >> 5;
+#pragma unroll 1
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k0 = 16 * kk + 8 * hh;
+        const bf16x8 a0 = frag8(A + r * as + k0, 1);
+        const bf16x8 a1 = frag8(A + (32 + r) * as + k0, 1);
+        const bf16x8 w0 = frag8(W + r * ws + k0, 1);
+        const bf16x8 w1 = frag8(W + (32 + r) * ws + k0, 1);
+        acc.c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a0, acc.c[0][0], 0, 0, 0);
+        acc.c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a0, acc.c[0][1], 0, 0, 0);
+        acc.c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a1, acc.c[1][0], 0, 0, 0);
+        acc.c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a1, acc.c[1][1], 0, 0, 0);
+    }
+}
+
+// Starts a wgemm_nt accumulation at the bias: C[m][n] = b[n].
+__device__ __forceinline__ void acc_bias_nt(Acc &a, const float *b)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 v = *reinterpret_cast<const float4 *>(b + 32 * nb + 8 * q + 4 * (l >> 5));
+                a.c[mb][nb][4 * q] = v.x; a.c[mb][nb][4 * q + 1] = v.y;
+                a.c[mb][nb][4 * q + 2] = v.z; a.c[mb][nb][4 * q + 3] = v.w;
+            }
+}
+
+// H[m][n] = bf16(relu(C[m][n])) from a wgemm_nt result (bias included in C).
+__device__ __forceinline__ void store_nt_bf16_relu(const Acc &a, uint16_t *H)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 32 * mb + (l & 31), n = 32 * nb + 8 * q + 4 * (l >> 5);
+                uint32_t w[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    w[t] = pack_bf16x2(fmaxf(a.c[mb][nb][4 * q + 2 * t], 0.0f), fmaxf(a.c[mb][nb][4 * q + 2 * t + 1], 0.0f));
+                *reinterpret_cast<uint2 *>(H + m * HS + n) = make_uint2(w[0], w[1]);
+            }
+}
+
+// O[m][n] = C[m][n] for the column groups with n + 3 < ncol (row stride os, a multiple of 4).
+__device__ __forceinline__ void store_nt_f32(const Acc &a, float *O, int os, int ncol)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 32 * mb + (l & 31), n = 32 * nb + 8 * q + 4 * (l >> 5);
+                if (n + 3 < ncol) {
+                    float4 v;
+                    v.x = a.c[mb][nb][4 * q];
+                    v.y = a.c[mb][nb][4 * q + 1];
+                    v.z = a.c[mb][nb][4 * q + 2];
+                    v.w = a.c[mb][nb][4 * q + 3];
+                    *reinterpret_cast<float4 *>(O + m * os + n) = v;
+                }
+            }
+}
+
 __device__ __forceinline__ void stage_layer(PackedLayer &dst, const PackedLayer *src)
 {
     const uint4 *s = reinterpret_cast<const uint4 *>(src);
@@ -131,54 +234,79 @@ __device__ __forceinline__ void stage_layer(PackedLayer &dst, const PackedLayer 
 __device__ __forceinline__ int cond0(int d) { return d == 0 ? 1 : 0; }
 __device__ __forceinline__ int cond1(int d) { return d == 2 ? 1 : 2; }
 
-// MLP of one 64-point tile: row p of H1 from lane p's conditioner (c0, c1), then the two MFMA
-// layers; writes O[p][0..63] (49 used) as FP32.
-__device__ __forceinline__ void mlp_tile(const PackedLayer &P, float c0, float c1, uint16_t *H1, uint16_t *H2,
-                                         float *Ob)
+// First layer of one 64-point tile (lane = point): H1[lane][j] = bf16(relu(W1[j] . c + b1[j])),
+// four hidden units per 8-byte store.
+__device__ __forceinline__ void layer1(const PackedLayer &P, float c0, float c1, uint16_t *H1)
 {
-    const int p = threadIdx.x;
-#pragma unroll 8
-    for (int j = 0; j < FH; ++j) {
-        const float h = fmaxf(__builtin_fmaf(P.W1[2 * j], c0, __builtin_fmaf(P.W1[2 * j + 1], c1, P.b1[j])), 0.0f);
-        H1[p * HS + j] = f2bf(h);
+    const int p = threadIdx.x & 63;
+#pragma unroll 4
+    for (int j = 0; j < FH; j += 4) {
+        uint32_t w[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int j0 = j + 2 * t, j1 = j0 + 1;
+            const float h0 = fmaxf(__builtin_fmaf(P.W1[2 * j0], c0, __builtin_fmaf(P.W1[2 * j0 + 1], c1, P.b1[j0])), 0.0f);
+            const float h1 = fmaxf(__builtin_fmaf(P.W1[2 * j1], c0, __builtin_fmaf(P.W1[2 * j1 + 1], c1, P.b1[j1])), 0.0f);
+            w[t] = pack_bf16x2(h0, h1);
+        }
+        *reinterpret_cast<uint2 *>(H1 + p * HS + j) = make_uint2(w[0], w[1]);
     }
-    __syncthreads();
-    Acc a;
-    acc_zero(a);
-    wgemm(a, H1, HS, 1, P.W2, 1, FH);
-    acc_each(a, [&](int m, int n, float v) { H2[m * HS + n] = f2bf(fmaxf(v + P.b2[n], 0.0f)); });
-    __syncthreads();
-    acc_zero(a);
-    wgemm(a, H2, HS, 1, P.W3, 1, FH);
-    acc_each(a, [&](int m, int n, float v) { Ob[m * OS + n] = v + P.b3[n]; });
-    __syncthreads();
 }
 
-// Spline parameters of one point from its 49 outputs: softmax widths / heights, softplus knot
-// derivatives; locate the bin of x (forward) or y (inverse).
+// MLP of one 64-point tile: H2 = relu(H1 W2^T + b2) (bf16), O = H2 W3^T + b3 (fp32, the first
+// ocol columns, row stride os).  H2 may alias H1 and O may alias both: each buffer is written
+// only after the GEMM that reads the previous one has issued all its LDS reads.
+__device__ __forceinline__ void mlp_tile(const PackedLayer &P, float c0, float c1, uint16_t *H1, uint16_t *H2,
+                                         float *O, int os, int ocol)
+{
+    layer1(P, c0, c1, H1);
+    wave_sync();
+    Acc a;
+    acc_bias_nt(a, P.b2);
+    wgemm_nt(a, H1, HS, P.W2, FH);
+    wave_sync();
+    store_nt_bf16_relu(a, H2);
+    wave_sync();
+    acc_bias_nt(a, P.b3);
+    wgemm_nt(a, H2, HS, P.W3, FH);
+    wave_sync();
+    store_nt_f32(a, O, os, ocol);
+    wave_sync();
+}
+
+// Spline parameters of one point from its 49 outputs: softmax widths / heights (each exponential
+// once, one reciprocal per softmax), softplus knot derivatives; locate the bin of x (forward) or
+// y (inverse).
 struct Bin { int k; float xk, wk, yk, hk, dk, dk1; };
 
-__device__ __forceinline__ float softplusf(float x) { return fmaxf(x, 0.0f) + log1pf(expf(-fabsf(x))); }
+__device__ __forceinline__ float softplusf(float x) { return fmaxf(x, 0.0f) + log1pf(__expf(-fabsf(x))); }
 
-__device__ __forceinline__ Bin locate(const float *o, float v, bool by_y)
+__device__ __forceinline__ void softmax16(const float *o, float *p)
 {
-    float mw = o[0], mh = o[FK];
+    float m = o[0];
 #pragma unroll
-    for (int i = 1; i < FK; ++i) { mw = fmaxf(mw, o[i]); mh = fmaxf(mh, o[FK + i]); }
-    float sw = 0.0f, sh = 0.0f;
+    for (int i = 1; i < FK; ++i) m = fmaxf(m, o[i]);
+    float s = 0.0f;
 #pragma unroll
-    for (int i = 0; i < FK; ++i) { sw += expf(o[i] - mw); sh += expf(o[FK + i] - mh); }
+    for (int i = 0; i < FK; ++i) { p[i] = __expf(o[i] - m); s += p[i]; }
+    const float r = 1.0f / s;
+#pragma unroll
+    for (int i = 0; i < FK; ++i) p[i] *= r;
+}
+
+__device__ __forceinline__ Bin locate(const float *o, const float *pw, const float *ph, float v, bool by_y)
+{
     const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
     Bin b;
     b.k = 0; b.xk = 0.0f; b.yk = 0.0f;
-    b.wk = kMinW + cw * (expf(o[0] - mw) / sw);
-    b.hk = kMinH + ch * (expf(o[FK] - mh) / sh);
-    float cx = 0.0f, cy = 0.0f;
+    b.wk = kMinW + cw * pw[0];
+    b.hk = kMinH + ch * ph[0];
+    float cx = b.wk, cy = b.hk;
 #pragma unroll
-    for (int i = 0; i < FK; ++i) {
-        const float wi = kMinW + cw * (expf(o[i] - mw) / sw);
-        const float hi = kMinH + ch * (expf(o[FK + i] - mh) / sh);
-        if (i > 0 && (by_y ? v >= cy : v >= cx)) { b.k = i; b.xk = cx; b.yk = cy; b.wk = wi; b.hk = hi; }
+    for (int i = 1; i < FK; ++i) {
+        const float wi = kMinW + cw * pw[i];
+        const float hi = kMinH + ch * ph[i];
+        if (by_y ? v >= cy : v >= cx) { b.k = i; b.xk = cx; b.yk = cy; b.wk = wi; b.hk = hi; }
         cx += wi;
         cy += hi;
     }
@@ -190,13 +318,16 @@ __device__ __forceinline__ Bin locate(const float *o, float v, bool by_y)
 // Rational-quadratic map of bin b at x: returns S(x), writes log S'(x).
 __device__ __forceinline__ float rq_forward(const Bin &b, float x, float *logd)
 {
-    const float xi = (x - b.xk) / b.wk;
-    const float s = b.hk / b.wk;
+    const float iw = 1.0f / b.wk;
+    const float xi = (x - b.xk) * iw;
+    const float s = b.hk * iw;
     const float t = xi * (1.0f - xi);
     const float den = s + (b.dk1 + b.dk - 2.0f * s) * t;
-    const float y = b.yk + b.hk * (s * xi * xi + b.dk * t) / den;
+    const float iden = 1.0f / den;
+    const float y = b.yk + b.hk * (s * xi * xi + b.dk * t) * iden;
     const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
-    *logd = 2.0f * logf(s) + logf(n2) - 2.0f * logf(den);
+    const float sd = s * iden;
+    *logd = __logf(sd * sd * n2);
     return y;
 }
 
@@ -215,37 +346,44 @@ __device__ __forceinline__ float rq_inverse(const Bin &b, float y, float *logd)
     const float t = xi * (1.0f - xi);
     const float den = s + A * t;
     const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
-    *logd = 2.0f * logf(s) + logf(n2) - 2.0f * logf(den);
+    const float sd = s / den;
+    *logd = __logf(sd * sd * n2);
     return b.xk + xi * b.wk;
 }
 
 // ------------------------------------------------------------------------------------ forward / inverse
 // One layer over n points: xin / xout are n x 3 (may alias), ld (n) accumulates +log S' (forward)
-// or -log S' (inverse).
+// or -log S' (inverse).  FWF waves per block share the staged weights; each wave walks its own
+// 64-point tiles with one LDS buffer in which H1, H2 and the outputs replace each other.
 template <bool INVERSE>
-__global__ __launch_bounds__(64) void k_flow_layer(const PackedLayer *__restrict__ layers, int l, uint64_t n,
-                                                   const float *__restrict__ xin, float *__restrict__ xout,
-                                                   float *__restrict__ ld)
+__global__ __launch_bounds__(64 * FWF) void k_flow_layer(const PackedLayer *__restrict__ layers, int l, uint64_t n,
+                                                         const float *__restrict__ xin, float *__restrict__ xout,
+                                                         float *__restrict__ ld)
 {
-    __shared__ PackedLayer P;
-    __shared__ uint16_t H1[64 * HS], H2[64 * HS];
-    __shared__ float Ob[64 * OS];
+    __shared__ __align__(16) PackedLayer P;
+    __shared__ __align__(16) float wbuf[FWF][64 * OS];
     stage_layer(P, layers + l);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float *O = wbuf[wave];
+    uint16_t *H = reinterpret_cast<uint16_t *>(O);
     const int d = l % 3, ca = cond0(d), cb = cond1(d);
     const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t i = tile * 64 + threadIdx.x;
+    for (uint64_t tile = (uint64_t)blockIdx.x * FWF + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWF) {
+        const uint64_t i = tile * 64 + lane;
         const bool live = i < n;
         float x[3] = {0.5f, 0.5f, 0.5f};
         if (live) { x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2]; }
-        mlp_tile(P, x[ca], x[cb], H1, H2, Ob);
-        const float *o = Ob + threadIdx.x * OS;
+        mlp_tile(P, x[ca], x[cb], H, H, O, OS, OS);
+        const float *o = O + lane * OS;
+        float pw[FK], ph[FK];
+        softmax16(o, pw);
+        softmax16(o + FK, ph);
         float lj;
         if (!INVERSE) {
-            const Bin b = locate(o, x[d], false);
+            const Bin b = locate(o, pw, ph, x[d], false);
             x[d] = rq_forward(b, x[d], &lj);
         } else {
-            const Bin b = locate(o, x[d], true);
+            const Bin b = locate(o, pw, ph, x[d], true);
             x[d] = rq_inverse(b, x[d], &lj);
             lj = -lj;
         }
@@ -253,31 +391,40 @@ __global__ __launch_bounds__(64) void k_flow_layer(const PackedLayer *__restrict
             xout[3 * i] = x[0]; xout[3 * i + 1] = x[1]; xout[3 * i + 2] = x[2];
             ld[i] += lj;
         }
-        __syncthreads();
+        wave_sync();                                   // row reads done before the next tile's H1
     }
 }
 
 // ------------------------------------------------------------------------------------ backward
 // Adjoint of one layer for L = sum_i (abar_out_i . x_out_i + g_i log S'_i) summed with the
 // later layers' contributions: abar (n x 3) holds dL/d(output coords) on entry and dL/d(input
-// coords) on exit; the layer's parameter gradient of this block goes to gp[blockIdx.x][FPL].
-__global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__restrict__ layers, int l, uint64_t n,
-                                                       const float *__restrict__ xin, const float *__restrict__ g,
-                                                       float *__restrict__ abar, float *__restrict__ gp)
+// coords) on exit.  FWB waves per block, each with its own tiles; the weight gradients dW2, dW3
+// accumulate in the matrix-core accumulators of the wave across all its tiles, and each wave
+// writes its parameter gradient to row (blockIdx.x * FWB + wave) of gp (rows x FPL).
+__global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *__restrict__ layers, int l, uint64_t n,
+                                                             const float *__restrict__ xin, const float *__restrict__ g,
+                                                             float *__restrict__ abar, float *__restrict__ gp)
 {
-    __shared__ PackedLayer P;
-    __shared__ uint16_t H1[64 * HS], H2[64 * HS];
-    __shared__ float Ob[64 * OS], Db[64 * OS];
-    __shared__ float Cc[64 * 2];
-    __shared__ float gW2[FH * FH], gW3[64 * FH];              // this block's weight gradients
+    __shared__ __align__(16) PackedLayer P;
+    __shared__ __align__(16) uint16_t H1s[FWB][64 * HS];
+    __shared__ __align__(16) uint16_t H2s[FWB][64 * HS];      // H2, then dH2 (bf16)
+    __shared__ __align__(16) float Gs[FWB][64 * GS];          // MLP outputs, then dO, then dH1
+    __shared__ float Ccs[FWB][128];
     stage_layer(P, layers + l);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t *H1 = H1s[wave], *H2 = H2s[wave];
+    float *G = Gs[wave], *Cc = Ccs[wave];
     const int d = l % 3, ca = cond0(d), cb = cond1(d);
-    const int lane = threadIdx.x;
-    for (int q = lane; q < FH * FH; q += 64) { gW2[q] = 0.0f; gW3[q] = 0.0f; }
-    __syncthreads();
-    float gb1 = 0.0f, gb2 = 0.0f, gb3 = 0.0f, gw10 = 0.0f, gw11 = 0.0f;
+    Acc gW2, gW3;
+    acc_zero(gW2);
+    acc_zero(gW3);
+    // bias / first-layer sums of the two columns lane & 31 and 32 + (lane & 31) this lane holds
+    // in the standard accumulator layout (the other half of the rows sits in lane ^ 32)
+    float gb1[2] = {0.0f, 0.0f}, gw10[2] = {0.0f, 0.0f}, gw11[2] = {0.0f, 0.0f}, gb2[2] = {0.0f, 0.0f};
+    float gb3 = 0.0f;
+    const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
     const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (uint64_t tile = (uint64_t)blockIdx.x * FWB + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWB) {
         const uint64_t i = tile * 64 + lane;
         const bool live = i < n;
         float x[3] = {0.5f, 0.5f, 0.5f}, ab[3] = {0.0f, 0.0f, 0.0f}, gi = 0.0f;
@@ -288,95 +435,101 @@ __global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__rest
         }
         Cc[2 * lane] = x[ca];
         Cc[2 * lane + 1] = x[cb];
-        mlp_tile(P, x[ca], x[cb], H1, H2, Ob);
-        // ---- spline backward (lane = point), dO written over O
+        mlp_tile(P, x[ca], x[cb], H1, H2, G, GS, 64);
+        // ---- spline backward (lane = point), dO written over the outputs
         {
-            float *o = Ob + lane * OS;
-            float mw = o[0], mh = o[FK];
-            for (int q = 1; q < FK; ++q) { mw = fmaxf(mw, o[q]); mh = fmaxf(mh, o[FK + q]); }
-            float sw = 0.0f, sh = 0.0f;
-            for (int q = 0; q < FK; ++q) { sw += expf(o[q] - mw); sh += expf(o[FK + q] - mh); }
-            const Bin b = locate(o, x[d], false);
-            const float xi = (x[d] - b.xk) / b.wk, s = b.hk / b.wk, t = xi * (1.0f - xi);
+            float *o = G + lane * GS;
+            float pw[FK], ph[FK];
+            softmax16(o, pw);
+            softmax16(o + FK, ph);
+            const Bin b = locate(o, pw, ph, x[d], false);
+            const float iw = 1.0f / b.wk;
+            const float xi = (x[d] - b.xk) * iw, s = b.hk * iw, t = xi * (1.0f - xi);
             const float A = b.dk1 + b.dk - 2.0f * s, den = s + A * t;
-            const float P2 = s * xi * xi + b.dk * t, R = P2 / den;
+            const float iden = 1.0f / den, iden2 = iden * iden;
+            const float P2 = s * xi * xi + b.dk * t, R = P2 * iden;
             const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
+            const float in2 = 1.0f / n2;
             const float aS = live ? ab[d] : 0.0f, gL = live ? gi : 0.0f;
-            const float den2 = den * den, omx = 1.0f - xi, om2t = 1.0f - 2.0f * t;
+            const float omx = 1.0f - xi, om2t = 1.0f - 2.0f * t;
             // d R / d (xi, s, dk, dk1)
             const float Pxi = 2.0f * s * xi + b.dk * (1.0f - 2.0f * xi), denxi = A * (1.0f - 2.0f * xi);
-            const float R_xi = (Pxi * den - P2 * denxi) / den2;
-            const float R_s = (xi * xi * den - P2 * om2t) / den2;
-            const float R_dk = t * (den - P2) / den2;
-            const float R_dk1 = -P2 * t / den2;
+            const float R_xi = (Pxi * den - P2 * denxi) * iden2;
+            const float R_s = (xi * xi * den - P2 * om2t) * iden2;
+            const float R_dk = t * (den - P2) * iden2;
+            const float R_dk1 = -P2 * t * iden2;
             // d log S' / d (xi, s, dk, dk1)
             const float n2xi = 2.0f * b.dk1 * xi + 2.0f * s * (1.0f - 2.0f * xi) - 2.0f * b.dk * omx;
-            const float L_xi = n2xi / n2 - 2.0f * denxi / den;
-            const float L_s = 2.0f / s + 2.0f * t / n2 - 2.0f * om2t / den;
-            const float L_dk = omx * omx / n2 - 2.0f * t / den;
-            const float L_dk1 = xi * xi / n2 - 2.0f * t / den;
+            const float L_xi = n2xi * in2 - 2.0f * denxi * iden;
+            const float L_s = 2.0f / s + 2.0f * t * in2 - 2.0f * om2t * iden;
+            const float L_dk = omx * omx * in2 - 2.0f * t * iden;
+            const float L_dk1 = xi * xi * in2 - 2.0f * t * iden;
             const float hR = aS * b.hk;
-            const float Gxi = hR * R_xi + gL * L_xi, Gs = hR * R_s + gL * L_s;
+            const float Gxi = hR * R_xi + gL * L_xi, Gs_ = hR * R_s + gL * L_s;
             const float Gdk = hR * R_dk + gL * L_dk, Gdk1 = hR * R_dk1 + gL * L_dk1;
-            const float dx = Gxi / b.wk;                  // dL / d x_in[d]
+            const float dx = Gxi * iw;                    // dL / d x_in[d]
             const float dxk = -dx;
-            const float dwk = -(Gxi * xi + Gs * s) / b.wk;
-            const float dhk = aS * R + Gs / b.wk;
+            const float dwk = -(Gxi * xi + Gs_ * s) * iw;
+            const float dhk = aS * R + Gs_ * iw;
             const float dyk = aS;
-            // widths / heights: cumulative sums then softmax; dO overwrites O in place (two passes:
-            // the softmax projections first, then each output read before it is replaced)
-            const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
+            // widths / heights: cumulative sums, then the softmax Jacobian
             float dotw = 0.0f, doth = 0.0f;
+#pragma unroll
             for (int q = 0; q < FK; ++q) {
                 const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
                 const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
-                dotw += (expf(o[q] - mw) / sw) * gwq;
-                doth += (expf(o[FK + q] - mh) / sh) * ghq;
+                dotw += pw[q] * gwq;
+                doth += ph[q] * ghq;
             }
+            // knot-derivative logits (read before the row is overwritten)
+            const float zk = o[2 * FK + b.k], zk1 = o[2 * FK + b.k + 1];
+            const float sgk = 1.0f / (1.0f + __expf(-zk)), sgk1 = 1.0f / (1.0f + __expf(-zk1));
+#pragma unroll
             for (int q = 0; q < FK; ++q) {
                 const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
                 const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
-                const float pw = expf(o[q] - mw) / sw, ph = expf(o[FK + q] - mh) / sh;
-                o[q] = live ? pw * (gwq - dotw) : 0.0f;
-                o[FK + q] = live ? ph * (ghq - doth) : 0.0f;
+                o[q] = live ? pw[q] * (gwq - dotw) : 0.0f;
+                o[FK + q] = live ? ph[q] * (ghq - doth) : 0.0f;
             }
-            for (int q = 0; q <= FK; ++q) {
-                const float gq = (q == b.k ? Gdk : 0.0f) + (q == b.k + 1 ? Gdk1 : 0.0f);
-                const float sig = 1.0f / (1.0f + expf(-o[2 * FK + q]));
-                o[2 * FK + q] = live ? gq * sig : 0.0f;
+#pragma unroll
+            for (int q = 2 * FK; q < 64; ++q) o[q] = 0.0f;
+            if (live) {
+                o[2 * FK + b.k] = Gdk * sgk;
+                o[2 * FK + b.k + 1] = Gdk1 * sgk1;
             }
-            for (int q = FO; q < 64; ++q) o[q] = 0.0f;
             ab[d] = live ? dx : 0.0f;
         }
-        __syncthreads();
+        wave_sync();
         // ---- MLP backward
-        for (int p = 0; p < 64; ++p) gb3 += (lane < FO) ? Ob[p * OS + lane] : 0.0f;
+#pragma unroll 8
+        for (int p = 0; p < 64; ++p) gb3 += G[p * GS + lane];                // columns >= 49 are 0
+        wgemm(gW3, G, 1, GS, H2, HS, 1);                                     // dW3 += dO^T H2
         Acc a;
         acc_zero(a);
-        wgemm(a, Ob, 1, OS, H2, HS, 1);                            // dW3 += dO^T H2
-        acc_each(a, [&](int m, int nn, float v) { gW3[m * FH + nn] += v; });
+        wgemm(a, G, GS, 1, P.W3, FH, 1);                                     // dH2 = dO W3
+        wave_sync();
+        acc_each(a, [&](int m, int nn, float v) {                            // masked by H2 > 0, over H2
+            const float dv = (H2[m * HS + nn] != 0) ? v : 0.0f;
+            gb2[nn >> 5] += dv;
+            H2[m * HS + nn] = f2bf(dv);
+        });
+        wave_sync();
+        wgemm(gW2, H2, 1, HS, H1, HS, 1);                                    // dW2 += dH2^T H1
         acc_zero(a);
-        wgemm(a, Ob, OS, 1, P.W3, FH, 1);                          // dH2 = dO W3
-        acc_each(a, [&](int m, int nn, float v) { Db[m * OS + nn] = (H2[m * HS + nn] != 0) ? v : 0.0f; });
-        __syncthreads();
-        for (int p = 0; p < 64; ++p) gb2 += Db[p * OS + lane];
-        acc_zero(a);
-        wgemm(a, Db, 1, OS, H1, HS, 1);                            // dW2 += dH2^T H1
-        acc_each(a, [&](int m, int nn, float v) { gW2[m * FH + nn] += v; });
-        acc_zero(a);
-        wgemm(a, Db, OS, 1, P.W2, FH, 1);                          // dH1 = dH2 W2
-        __syncthreads();
-        acc_each(a, [&](int m, int nn, float v) { Ob[m * OS + nn] = (H1[m * HS + nn] != 0) ? v : 0.0f; });
-        __syncthreads();
-        for (int p = 0; p < 64; ++p) {
-            const float v = Ob[p * OS + lane];
-            gb1 += v;
-            gw10 += v * Cc[2 * p];
-            gw11 += v * Cc[2 * p + 1];
-        }
+        wgemm(a, H2, HS, 1, P.W2, FH, 1);                                    // dH1 = dH2 W2
+        wave_sync();
+        acc_each(a, [&](int m, int nn, float v) {                            // masked by H1 > 0, over dO
+            const float dv = (H1[m * HS + nn] != 0) ? v : 0.0f;
+            gb1[nn >> 5] += dv;
+            gw10[nn >> 5] += dv * Cc[2 * m];
+            gw11[nn >> 5] += dv * Cc[2 * m + 1];
+            G[m * GS + nn] = dv;
+        });
+        wave_sync();
         float dc0 = 0.0f, dc1 = 0.0f;
+#pragma unroll 8
         for (int j = 0; j < FH; ++j) {
-            const float v = Ob[lane * OS + j];
+            const float v = G[lane * GS + j];
             dc0 += v * P.W1[2 * j];
             dc1 += v * P.W1[2 * j + 1];
         }
@@ -385,17 +538,30 @@ __global__ __launch_bounds__(64) void k_flow_layer_bwd(const PackedLayer *__rest
             ab[cb] += dc1;
             abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
         }
-        __syncthreads();
+        wave_sync();                                   // row reads done before the next tile
     }
-    float *out = gp + (size_t)blockIdx.x * FPL;
-    out[PW1 + 2 * lane] = gw10;
-    out[PW1 + 2 * lane + 1] = gw11;
-    out[PB1 + lane] = gb1;
-    out[PB2 + lane] = gb2;
+    // lanes l and l ^ 32 hold the two halves of the rows of the same columns
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        gb1[h] += __shfl_xor(gb1[h], 32, 64);
+        gw10[h] += __shfl_xor(gw10[h], 32, 64);
+        gw11[h] += __shfl_xor(gw11[h], 32, 64);
+        gb2[h] += __shfl_xor(gb2[h], 32, 64);
+    }
+    float *out = gp + ((size_t)blockIdx.x * FWB + wave) * FPL;
+    if (lane < 32) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = 32 * h + lane;
+            out[PW1 + 2 * c] = gw10[h];
+            out[PW1 + 2 * c + 1] = gw11[h];
+            out[PB1 + c] = gb1[h];
+            out[PB2 + c] = gb2[h];
+        }
+    }
     if (lane < FO) out[PB3 + lane] = gb3;
-    __syncthreads();
-    for (int q = lane; q < FH * FH; q += 64) out[PW2 + q] = gW2[q];
-    for (int q = lane; q < FO * FH; q += 64) out[PW3 + q] = gW3[q];
+    acc_each(gW2, [&](int m, int nn, float v) { out[PW2 + m * FH + nn] = v; });
+    acc_each(gW3, [&](int m, int nn, float v) { if (m < FO) out[PW3 + m * FH + nn] = v; });
 }
 
 // ------------------------------------------------------------------------------------ small kernels
@@ -487,14 +653,24 @@ __global__ void k_flow_coef(const float *__restrict__ w, const double *__restric
     abar[3 * i] = 0.0f; abar[3 * i + 1] = 0.0f; abar[3 * i + 2] = 0.0f;
 }
 
-// Column sums of nblk x ncol floats (fixed order) into out (FP32 gradient).
-__global__ void k_colsum_f(const float *__restrict__ parts, uint32_t nblk, uint32_t ncol, float *__restrict__ out)
+// Parameter gradient of all layers: grad[l*FPL + j] = sum over r < rows of
+// parts[(l*stride + r)*FPL + j], in a fixed order (four interleaved row groups, then their sum).
+__global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ parts, uint32_t rows, uint32_t stride,
+                                                  float *__restrict__ grad)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
+    __shared__ float red[4][64];
+    const uint32_t c = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
     float s = 0.0f;
-    for (uint32_t b = 0; b < nblk; ++b) s += parts[(size_t)b * ncol + c];
-    out[c] = s;
+    if (c < (uint32_t)(FL * FPL)) {
+        const uint32_t layer = c / FPL, j = c % FPL;
+        const float *q = parts + (size_t)layer * stride * FPL + j;
+#pragma unroll 4
+        for (uint32_t r = rg; r < rows; r += 4) s += q[(size_t)r * FPL];
+    }
+    red[rg][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < (uint32_t)(FL * FPL))
+        grad[c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
@@ -533,8 +709,8 @@ struct FlowState {
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
     float *ld = nullptr, *w = nullptr, *g = nullptr, *abar = nullptr;
-    float *gp = nullptr;        // per-block layer gradients
-    uint32_t gp_blocks = 0;
+    float *gp = nullptr;        // per-wave layer gradients, FL x gp_rows x FPL
+    uint32_t gp_rows = 0;
     double *hist = nullptr;     // training history (sum w, sum w^2 per step)
     int hist_cap = 0;
     double *acc = nullptr;      // accumulating record
@@ -573,7 +749,21 @@ void philox_host(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0
     o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
 }
 
-int flow_blocks(mi_ctx *ctx) { return (ctx->num_cu > 0 ? ctx->num_cu : 256) * 2; }
+int num_cu(mi_ctx *ctx) { return ctx->num_cu > 0 ? ctx->num_cu : 256; }
+
+// Grids: the forward / inverse kernel keeps two blocks of FWF waves per CU resident (70 KB of
+// LDS each) and walks the tiles with a grid stride; the backward kernel (124 KB of LDS) one
+// block of FWB waves per CU.
+uint32_t fwd_blocks(mi_ctx *ctx, uint64_t n)
+{
+    const uint64_t ntiles = (n + 63) / 64;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + FWF - 1) / FWF, (uint64_t)num_cu(ctx) * 2));
+}
+uint32_t bwd_blocks(mi_ctx *ctx, uint64_t n)
+{
+    const uint64_t ntiles = (n + 63) / 64;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + FWB - 1) / FWB, (uint64_t)num_cu(ctx)));
+}
 
 int flow_alloc(mi_ctx *ctx, uint64_t npts)
 {
@@ -588,12 +778,13 @@ int flow_alloc(mi_ctx *ctx, uint64_t npts)
         FCHK(hipMalloc(&F.abar, sizeof(float) * 3 * npts));
         F.cap = npts;
     }
-    const uint32_t nb = (uint32_t)flow_blocks(ctx);
-    if (nb > F.gp_blocks) {
+    const uint32_t rows = (uint32_t)num_cu(ctx) * FWB;
+    if (rows > F.gp_rows) {
         (void)hipFree(F.gp);
         F.gp = nullptr;
-        FCHK(hipMalloc(&F.gp, sizeof(float) * (size_t)FPL * nb));
-        F.gp_blocks = nb;
+        F.gp_rows = 0;
+        FCHK(hipMalloc(&F.gp, sizeof(float) * (size_t)FPL * FL * rows));
+        F.gp_rows = rows;
     }
     return MI_OK;
 }
@@ -610,7 +801,7 @@ hipError_t run_layers(mi_ctx *ctx, bool inverse, uint64_t n, float *x_first, flo
     // x_first holds the input; with keep_all the layer inputs of the normalizing direction land in
     // F.xs (x^(0) .. x^(FL)), otherwise the result overwrites x_first.
     FlowState &F = *ctx->flow;
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 63) / 64, (uint64_t)flow_blocks(ctx) * 4);
+    const uint32_t blocks = fwd_blocks(ctx, n);
     for (int s = 0; s < FL; ++s) {
         const int l = inverse ? FL - 1 - s : s;
         const float *src;
@@ -623,9 +814,11 @@ hipError_t run_layers(mi_ctx *ctx, bool inverse, uint64_t n, float *x_first, flo
             dst = x_first;
         }
         if (inverse)
-            hipLaunchKernelGGL(k_flow_layer<true>, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n, src, dst, ld);
+            hipLaunchKernelGGL(k_flow_layer<true>, dim3(blocks), dim3(64 * FWF), 0, ctx->stream, F.packed, l, n, src,
+                               dst, ld);
         else
-            hipLaunchKernelGGL(k_flow_layer<false>, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n, src, dst, ld);
+            hipLaunchKernelGGL(k_flow_layer<false>, dim3(blocks), dim3(64 * FWF), 0, ctx->stream, F.packed, l, n, src,
+                               dst, ld);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -740,6 +933,25 @@ int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, f
     return MI_OK;
 }
 
+// Backward through all layers from the layer inputs in F.xs and abar (dL/dz), then the parameter
+// gradient of every layer into F.grad.
+static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev)
+{
+    FlowState &F = *ctx->flow;
+    const uint32_t blocks = bwd_blocks(ctx, n);
+    const uint32_t rows = blocks * FWB;
+    if (rows > F.gp_rows) return ferr(ctx, MI_ERR_STATE, "gradient partials buffer too small");
+    for (int l = FL - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64 * FWB), 0, ctx->stream, F.packed, l, n,
+                           F.xs + (size_t)3 * F.cap * l, g_dev, F.abar, F.gp + (size_t)l * F.gp_rows * FPL);
+        FCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_grad_sum, dim3((FL * FPL + 63) / 64), dim3(256), 0, ctx->stream, F.gp, rows, F.gp_rows,
+                       F.grad);
+    FCHK(hipGetLastError());
+    return MI_OK;
+}
+
 // Gradient of sum_i g_i logdet_fwd(y_i) w.r.t. the parameters, for y already in F.xs[0].
 static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
 {
@@ -748,16 +960,7 @@ static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
     FCHK(hipMemsetAsync(F.ld, 0, sizeof(float) * n, ctx->stream));
     FCHK(run_layers(ctx, false, n, nullptr, F.ld, true));
     FCHK(hipMemsetAsync(F.abar, 0, sizeof(float) * 3 * n, ctx->stream));
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 63) / 64, (uint64_t)F.gp_blocks);
-    for (int l = FL - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, n,
-                           F.xs + (size_t)3 * F.cap * l, g_dev, F.abar, F.gp);
-        FCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_colsum_f, dim3((FPL + 255) / 256), dim3(256), 0, ctx->stream, F.gp, blocks, (uint32_t)FPL,
-                           F.grad + (size_t)l * FPL);
-        FCHK(hipGetLastError());
-    }
-    return MI_OK;
+    return launch_backward(ctx, n, g_dev);
 }
 
 int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g_dev, float *grad_dev)
@@ -843,15 +1046,8 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
                            batch, F.g, F.abar, F.hist, s);
         FCHK(hipGetLastError());
         // backward from the stored layer inputs (x^(0) = y ... x^(FL) = z from the sampling pass)
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((batch + 63) / 64, (uint64_t)F.gp_blocks);
-        for (int l = FL - 1; l >= 0; --l) {
-            hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64), 0, ctx->stream, F.packed, l, batch,
-                               F.xs + (size_t)3 * F.cap * l, F.g, F.abar, F.gp);
-            FCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_colsum_f, dim3((FPL + 255) / 256), dim3(256), 0, ctx->stream, F.gp, blocks,
-                               (uint32_t)FPL, F.grad + (size_t)l * FPL);
-            FCHK(hipGetLastError());
-        }
+        rc = launch_backward(ctx, batch, F.g);
+        if (rc) return rc;
         F.step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f - std::pow(b1, (float)F.step), c2 = 1.0f - std::pow(b2, (float)F.step);
