@@ -61,9 +61,14 @@ def cpu_baseline(n_sample, threads):
     return n_sample / dt, dt, s[0] / n_sample
 
 
-def time_to_solution(ctx, mi, target=1e-3, flow=False):
+def time_to_solution(ctx, mi, target=1e-3, flow=None):
     """GPU time (sum of device-side kernel times of every launch) until sigma_err/sigma <= target:
-    VEGAS adaptation (+ flow training) + the final integration with the frozen map."""
+    VEGAS adaptation (+ flow training) + a 2^20-point pilot + the final pass sized for the target.
+
+    flow=None: plain VEGAS.  flow="fixed": 200 training steps on 2^16-point batches.
+    flow="adaptive": training in chunks of 10 steps, stopped when a chunk costs more GPU time than
+    it saves on the final pass, judged from the chunk's last-batch CV and the flow's cost per
+    point measured on the pilot of the untrained (identity) flow."""
     phys = mi.physics_default(1)
     t0 = time.perf_counter()
     ctx.vegas_init(3, 64)
@@ -71,23 +76,55 @@ def time_to_solution(ctx, mi, target=1e-3, flow=False):
     gpu_ms = info.kernel_ms
     sampler = mi.MI_SAMPLER_VEGAS
     extra = {"adapt_ms": info.kernel_ms}
-    if flow:
+    pilot_n = 1 << 20
+    if flow == "fixed":
         ctx.flow_init(7)
         fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), 200, 1 << 16, 2e-3)
         gpu_ms += fi.kernel_ms
         sampler = mi.MI_SAMPLER_FLOW
-        extra.update(train_ms=fi.kernel_ms, cv_first_batch=fi.cv_vegas, cv_last_batch=fi.cv_flow)
+        extra.update(train_steps=200, train_ms=fi.kernel_ms, cv_first_batch=fi.cv_vegas, cv_last_batch=fi.cv_flow)
+    elif flow == "adaptive":
+        ctx.flow_init(7)
+        sampler = mi.MI_SAMPLER_FLOW
+        probe = ctx.integrate(mi.make_run(phys, pilot_n, key=46, sampler=sampler))   # identity flow
+        gpu_ms += probe.kernel_ms
+        ms_per_point = probe.kernel_ms / pilot_n
+        cv = probe.sigmaErr() / probe.sigmaGen() * math.sqrt(pilot_n)
+        final_ms = (cv / target) ** 2 * ms_per_point
+        steps, train_ms, chunk, batch = 0, 0.0, 10, 1 << 16
+        while steps < 200:
+            fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43, counter0=steps * batch), chunk, batch, 2e-3)
+            steps += chunk
+            train_ms += fi.kernel_ms
+            new_final = (fi.cv_flow / target) ** 2 * ms_per_point
+            saved = final_ms - new_final
+            final_ms = min(final_ms, new_final)
+            if saved < fi.kernel_ms:
+                break
+        gpu_ms += train_ms
+        extra.update(train_steps=steps, train_ms=train_ms, identity_pilot_ms=probe.kernel_ms,
+                     flow_ms_per_point=ms_per_point)
     # pilot for the variance, then the final pass sized for the target
-    pilot = ctx.integrate(mi.make_run(phys, 1 << 20, key=44, sampler=sampler))
+    pilot = ctx.integrate(mi.make_run(phys, pilot_n, key=44, sampler=sampler))
     gpu_ms += pilot.kernel_ms
     rel = pilot.sigmaErr() / pilot.sigmaGen()
-    n = int(math.ceil((1 << 20) * (rel / target) ** 2 * 1.1))
+    n = int(math.ceil(pilot_n * (rel / target) ** 2 * 1.1))
     n = max(n, 1 << 16)
     fin = ctx.integrate(mi.make_run(phys, n, key=45, sampler=sampler))
     gpu_ms += fin.kernel_ms
     wall = time.perf_counter() - t0
     return {"gpu_ms": gpu_ms, "wall_s": wall, "points_final": n, "sigma": fin.sigmaGen(),
             "rel_err": fin.sigmaErr() / fin.sigmaGen(), "pilot_rel_err": rel, "final_ms": fin.kernel_ms, **extra}
+
+
+def tts_all(ctx, mi):
+    tv = time_to_solution(ctx, mi)
+    out = {"vegas": tv}
+    for mode in ("fixed", "adaptive"):
+        tf = time_to_solution(ctx, mi, flow=mode)
+        out[f"flow_vegas_{mode}"] = tf
+        out[f"speedup_{mode}"] = tv["gpu_ms"] / tf["gpu_ms"]
+    return out
 
 
 # Flow: per layer the MLP 2 -> 64 -> 64 -> 49 is 2*64 + 64*64 + 64*49 = 7360 multiply-adds per
@@ -176,8 +213,7 @@ def main():
     phys = mi.physics_default(1)
     n = args.points
     if args.only == "tts":
-        print(json.dumps({"time_to_solution": {"vegas": time_to_solution(ctx, mi, flow=False),
-                                               "flow_vegas": time_to_solution(ctx, mi, flow=True)}}))
+        print(json.dumps({"time_to_solution": tts_all(ctx, mi)}))
         ctx.close()
         return
     if args.only == "workloads":
@@ -237,14 +273,10 @@ def main():
     if not args.no_workloads:
         out["workloads"] = workloads(ctx, mi)
     if not args.no_tts:
-        tv = time_to_solution(ctx, mi, flow=False)
-        out["time_to_solution"] = {"vegas": tv}
         try:
-            tf = time_to_solution(ctx, mi, flow=True)
-            out["time_to_solution"]["flow_vegas"] = tf
-            out["time_to_solution"]["speedup"] = tv["gpu_ms"] / tf["gpu_ms"]
+            out["time_to_solution"] = tts_all(ctx, mi)
         except mi.MIError as e:
-            out["time_to_solution"]["flow_vegas"] = {"error": str(e)}
+            out["time_to_solution"] = {"error": str(e)}
     print(json.dumps(out))
     ctx.close()
 
