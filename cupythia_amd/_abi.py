@@ -26,6 +26,8 @@ MI_MODE_PARTONIC_1D, MI_MODE_HADRONIC_3D = 0, 1
 MI_PREC_FP32, MI_PREC_FP64 = 0, 1
 MI_SAMPLER_FLAT, MI_SAMPLER_VEGAS, MI_SAMPLER_FLOW = 0, 1, 2
 MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT = 0, 1
+MI_PDF_TOY, MI_PDF_GRID = 0, 1
+MI_ABI_VERSION = 2
 
 MI_FLOW_LAYERS, MI_FLOW_BINS, MI_FLOW_HIDDEN, MI_FLOW_NOUT = 6, 16, 64, 49
 MI_FLOW_NPARAMS = 45222
@@ -37,7 +39,7 @@ class MiPhysics(C.Structure):
     _fields_ = [("mode", I), ("proc_mask", I), ("sHat", D), ("cos_max", D), ("eCM", D),
                 ("pTHatMin", D), ("alphaS_value", D), ("alphaS_order", I), ("alphaS_nfmax", I),
                 ("renorm_fac", D), ("nQuarkIn", I), ("nQuarkNew", I), ("pdf_g", D * 2),
-                ("pdf_val", D * 2), ("pdf_sea", D * 2), ("kfactor", D)]
+                ("pdf_val", D * 2), ("pdf_sea", D * 2), ("kfactor", D), ("pdf_source", I), ("fact_fac", D)]
 
 
 class MiRun(C.Structure):
@@ -98,6 +100,8 @@ PROTOTYPES = {
     "mi_write_lhe": ([VP, C.c_char_p, P(MiPhysics), P(MiEvents), U64, U64, P(MiSigmaEst)], I),
     "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, I, P(D), VP], I),
     "mi_adjoint": ([VP, P(MiRun), P(MiSigmaGrad)], I),
+    "mi_pdf_load": ([VP, C.c_char_p], I),
+    "mi_pdf_xf": ([VP, U64, VP, VP, VP], I),
     "mi_flow_init": ([VP, U64], I),
     "mi_flow_train": ([VP, P(MiRun), I, U64, D, P(MiFlowInfo)], I),
     "mi_flow_forward": ([VP, U64, VP, VP, VP], I),

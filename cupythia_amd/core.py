@@ -258,6 +258,21 @@ class Context:
                                                 len(factors), fac, _ptr(out)), "mi_reweight_scales")
         return out
 
+    # -------------------------------------------------------------- PDF grids
+    def pdf_load(self, path: str) -> None:
+        """Proton PDF from an LHAPDF6 lhagrid1 file (Pythia's LHAGrid1); runs with
+        physics_default(..., pdf_source=MI_PDF_GRID) use it."""
+        self._check(self.lib.mi_pdf_load(self.h, path.encode()), "mi_pdf_load")
+
+    def pdf_xf(self, x: torch.Tensor, Q2: torch.Tensor) -> torch.Tensor:
+        """Raw interpolated xf (n x 12 FP64: g, d, u, s, c, b, dbar, ubar, sbar, cbar, bbar, 0)."""
+        x = x.to(device=self.dev, dtype=torch.float64).contiguous()
+        Q2 = Q2.to(device=self.dev, dtype=torch.float64).contiguous()
+        out = torch.empty((x.numel(), 12), dtype=torch.float64, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_pdf_xf(self.h, x.numel(), _ptr(x), _ptr(Q2), _ptr(out)), "mi_pdf_xf")
+        return out
+
     # -------------------------------------------------------------- adjoint
     def adjoint(self, run: A.MiRun) -> A.MiSigmaGrad:
         g = A.MiSigmaGrad()

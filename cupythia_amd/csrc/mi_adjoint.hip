@@ -117,6 +117,7 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64) { ctx->err = "bad precision"; return MI_ERR_ARG; }
     if (run->n_points == 0 || run->n_points > (1ULL << 40)) { ctx->err = "n_points must be 1..2^40"; return MI_ERR_RANGE; }
     if ((p.proc_mask & 0x3F) == 0 || (p.proc_mask & ~0x3F)) { ctx->err = "bad proc_mask"; return MI_ERR_ARG; }
+    if (p.pdf_source != MI_PDF_TOY) { ctx->err = "the adjoint's PDF-shape derivatives need the toy PDFs"; return MI_ERR_ARG; }
     if (!(p.alphaS_value > 0.0) || !(p.renorm_fac > 0.0) || !(p.eCM > 2.0 * p.pTHatMin) || !(p.pTHatMin > 0.0) ||
         p.nQuarkIn < 2 || p.nQuarkIn > 5 || !(p.pdf_g[0] > -1.0) || !(p.pdf_g[1] > -1.0)) {
         ctx->err = "physics configuration out of range";

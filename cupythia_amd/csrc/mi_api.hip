@@ -132,6 +132,11 @@ int check_physics(mi_ctx *ctx, const mi_physics &p)
     if (!(p.pdf_g[0] > -1.0) || !(p.pdf_val[0] > 0.0) || !(p.pdf_sea[0] > -1.0) || !(p.pdf_g[1] > -1.0) ||
         !(p.pdf_val[1] > -1.0) || !(p.pdf_sea[1] > -1.0))
         return fail(ctx, MI_ERR_ARG, "PDF exponents out of range (need integrable densities)");
+    if (p.pdf_source != MI_PDF_TOY && p.pdf_source != MI_PDF_GRID) return fail(ctx, MI_ERR_ARG, "unknown pdf_source");
+    if (p.pdf_source == MI_PDF_GRID) {
+        if (!ctx->pdf) return fail(ctx, MI_ERR_STATE, "pdf_source = MI_PDF_GRID but no grid loaded (mi_pdf_load)");
+        if (!(p.fact_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "fact_fac must be > 0");
+    }
     return MI_OK;
 }
 
@@ -208,8 +213,9 @@ Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks)
 Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas, bool hist)
 {
     const int nb = ctx->grid.nbins;
-    const int occ = run->phys.mode == MI_MODE_HADRONIC_3D ? occ_integrate_had(run->precision, vegas, hist, nb)
-                                                          : occ_integrate_c1(vegas, hist, nb);
+    const int occ = run->phys.mode == MI_MODE_HADRONIC_3D
+                        ? occ_integrate_had(run->precision, vegas, hist, nb, run->phys.pdf_source == MI_PDF_GRID)
+                        : occ_integrate_c1(vegas, hist, nb);
     const uint32_t cap = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * (uint32_t)occ;
     return geom_for(n, 16, cap);
 }
@@ -227,6 +233,8 @@ int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, boo
         L.prec = run->precision; L.vegas = vegas; L.mask = p.proc_mask;
         L.key = run->key; L.ctr0 = ctr0; L.n = n; L.g = g;
         fill_had_consts_f32(p, L.kf); fill_had_consts_f64(p, L.kd);
+        rc = pdf_views(ctx, p, &L.gf, &L.gd);
+        if (rc) return rc;
         L.grid = &ctx->grid; L.partials = ctx->partials; L.hist = hist_parts;
         HIPCHK(ctx, launch_integrate_had(ctx->stream, L));
     } else {
@@ -280,6 +288,7 @@ int mi_ctx_destroy(mi_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     mi_flow_release(ctx);
+    pdf_release(ctx);
     (void)hipFree(ctx->partials); (void)hipFree(ctx->hist); (void)hipFree(ctx->red); (void)hipFree(ctx->counts);
     (void)hipFree(ctx->grid.edges); (void)hipFree(ctx->grid.tab32); (void)hipFree(ctx->grid.tab64);
     (void)hipEventDestroy(ctx->ev0); (void)hipEventDestroy(ctx->ev1); (void)hipEventDestroy(ctx->ev2);
@@ -314,6 +323,8 @@ int mi_physics_default(int which, mi_physics *p)
     p->pdf_val[0] = 0.5; p->pdf_val[1] = 3.0;
     p->pdf_sea[0] = -0.2; p->pdf_sea[1] = 7.0;
     p->kfactor = 1.0;
+    p->pdf_source = MI_PDF_TOY;
+    p->fact_fac = 1.0;
     p->eCM = 13600.0;
     p->pTHatMin = 20.0;
     p->sHat = 100.0 * 100.0;
@@ -371,6 +382,8 @@ int mi_points(mi_ctx *ctx, const mi_run *run, double *w_dev, double *kin_dev)
         L.prec = run->precision; L.vegas = vegas; L.mask = p.proc_mask;
         L.key = run->key; L.ctr0 = run->counter0; L.n = run->n_points;
         fill_had_consts_f32(p, L.kf); fill_had_consts_f64(p, L.kd);
+        rc = pdf_views(ctx, p, &L.gf, &L.gd);
+        if (rc) return rc;
         L.grid = &ctx->grid;
         HIPCHK(ctx, launch_points_had(ctx->stream, L, w_dev, kin_dev));
     } else {
@@ -565,6 +578,8 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
         L.h.prec = run->precision; L.h.vegas = run->sampler == MI_SAMPLER_VEGAS; L.h.mask = p.proc_mask;
         L.h.key = run->key; L.h.ctr0 = run->counter0; L.h.n = run->n_points; L.h.g = g;
         fill_had_consts_f32(p, L.h.kf); fill_had_consts_f64(p, L.h.kd);
+        rc = pdf_views(ctx, p, &L.h.gf, &L.h.gd);
+        if (rc) return rc;
         L.h.grid = &ctx->grid; L.h.partials = ctx->partials;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));

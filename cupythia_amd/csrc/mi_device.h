@@ -229,6 +229,148 @@ struct HadConst {
 };
 static_assert(sizeof(HadConst<float>) == 32 * sizeof(float), "HadConst layout");
 
+// ------------------------------------------------------------------ PDF grid (LHAGrid1)
+// A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
+// LHAGrid1::xfxevolve does (pythia8317/src/PartonDistributions.cc:648-745, PDF:extrapolate =
+// off): bisection for the neighbouring nodes in x and in q (within the q subgrid holding q),
+// 4-point Lagrange weights in ln x and ln q (linear in ln q on subgrids of 2-3 points), frozen
+// outside the q range and below xMin, 0 at x >= xMax.  Slots: 0 = g, 1..5 = d u s c b,
+// 6..10 = dbar ubar sbar cbar bbar, 11 = photon; values flavour-interleaved
+// val[(iq * nx + ix) * 12 + slot].  FP64 weights divide as Pythia does; the FP32 weights
+// multiply by reciprocals tabulated on the host (ixinv, iqinv: 12 per stencil start, the
+// (i, j != i) pairs in order; iqinv[12 m] = 1 / (lnq[m+1] - lnq[m]) for 2-3 point subgrids).
+constexpr int kGridNF = 12, kGridMaxSub = 8;
+
+template <typename T>
+struct GridPdf {
+    const T *xk = nullptr, *lnx = nullptr;       // nx: nodes (search keys) and their logs
+    const T *qk = nullptr, *lnq = nullptr;       // nq: nodes with subgrid borders averaged; logs as read
+    const T *val = nullptr;                      // nq x nx x 12
+    const T *ixinv = nullptr, *iqinv = nullptr;  // FP32 only
+    T qDiv[kGridMaxSub] = {};                    // last q of each subgrid, as read
+    T xMin = 0, xMax = 0, qMin = 0, qMax = 0;
+    T factFac2 = 1;                              // Q2_F = factFac2 * pT2
+    int nqSum[kGridMaxSub] = {};                 // cumulative q count after each subgrid
+    int nx = 0, nq = 0, nqSub = 0, active = 0;
+};
+
+// w[i] = prod_{j != i} (t - n[j]) / (n[i] - n[j]) over the 4 nodes n (Lagrange weights).
+template <typename T>
+__host__ __device__ inline void lagrange4(T t, const T *n, const T *inv, T *w)
+{
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        w[i] = (T)1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j != i) {
+                if (inv) w[i] *= (t - n[j]) * inv[k];
+                else     w[i] *= (t - n[j]) / (n[i] - n[j]);
+                ++k;
+            }
+    }
+}
+
+// q stencil shared by both partons: start row m3q, n3q rows (1, 2 or 4) with weights wq.
+template <typename T>
+__host__ __device__ inline void grid_qw(const GridPdf<T> &g, T q, T lnq, int &m3q, int &n3q, T *wq)
+{
+    wq[0] = wq[1] = wq[2] = wq[3] = (T)1;
+    if (q <= g.qMin) { n3q = 1; m3q = 0; return; }
+    if (q >= g.qMax) { n3q = 1; m3q = g.nq - 1; return; }
+    int iqDiv = 0;
+    for (int s = 1; s < g.nqSub; ++s)
+        if (q > g.qDiv[s - 1]) iqDiv = s;
+    const int minS = iqDiv == 0 ? 0 : g.nqSum[iqDiv - 1], maxS = g.nqSum[iqDiv] - 1;
+    int minq = minS, maxq = maxS;
+    while (maxq - minq > 1) {
+        const int mid = (minq + maxq) >> 1;
+        if (q < g.qk[mid]) maxq = mid;
+        else               minq = mid;
+    }
+    if (maxS - minS < 3) {
+        n3q = 2;
+        m3q = minq;
+        wq[1] = g.iqinv ? (lnq - g.lnq[minq]) * g.iqinv[kGridNF * minq] : (lnq - g.lnq[minq]) / (g.lnq[maxq] - g.lnq[minq]);
+        wq[0] = (T)1 - wq[1];
+    } else {
+        n3q = 4;
+        m3q = (minq == minS) ? minS : ((maxq == maxS) ? maxS - 3 : minq - 1);
+        lagrange4<T>(lnq, g.lnq + m3q, g.iqinv ? g.iqinv + kGridNF * m3q : nullptr, wq);
+    }
+}
+
+// Raw interpolated xf of slots 0..10 at (x, ln x) for the q stencil of grid_qw.
+template <typename T>
+__host__ __device__ inline void grid_xf(const GridPdf<T> &g, T x, T lnx, int m3q, int n3q, const T *wq, T *out)
+{
+#pragma unroll
+    for (int f = 0; f < 11; ++f) out[f] = (T)0;
+    if (x <= g.xMin) {
+        for (int i3q = 0; i3q < n3q; ++i3q) {
+            const T *p = g.val + (size_t)(m3q + i3q) * g.nx * kGridNF;
+#pragma unroll
+            for (int f = 0; f < 11; ++f) out[f] += wq[i3q] * p[f];
+        }
+        return;
+    }
+    if (x >= g.xMax) return;
+    int minx = 0, maxx = g.nx - 1;
+    while (maxx - minx > 1) {
+        const int mid = (minx + maxx) >> 1;
+        if (x < g.xk[mid]) maxx = mid;
+        else               minx = mid;
+    }
+    const int m3x = (minx == 0) ? 0 : ((maxx == g.nx - 1) ? g.nx - 4 : minx - 1);
+    T wx[4];
+    lagrange4<T>(lnx, g.lnx + m3x, g.ixinv ? g.ixinv + kGridNF * m3x : nullptr, wx);
+    for (int i3q = 0; i3q < n3q; ++i3q) {
+        const T *p = g.val + ((size_t)(m3q + i3q) * g.nx + m3x) * kGridNF;
+#pragma unroll
+        for (int f = 0; f < 11; ++f)
+            out[f] += wq[i3q] * (wx[0] * p[f] + wx[1] * p[kGridNF + f] + wx[2] * p[2 * kGridNF + f]
+                                 + wx[3] * p[3 * kGridNF + f]);
+    }
+}
+
+// Parton luminosities of the five flux types from the grid at the factorization scale
+// (PDF values clipped at 0 as Pythia's PDF::xf does): gg, qg, all quark pairs QQ, same-flavour
+// pairs (q q and qbar qbar), and q qbar pairs of one flavour; nQ flavours enter.
+template <typename T>
+struct Lumi { T gg, qg, QQ, same, opp, g1, g2, Q1, Q2; };
+
+template <typename T>
+__host__ __device__ inline Lumi<T> grid_lumi(const GridPdf<T> &g, int nQ, T x1, T lx1, T x2, T lx2, T q, T lnq)
+{
+    int m3q, n3q;
+    T wq[4], f1[11], f2[11];
+    grid_qw<T>(g, q, lnq, m3q, n3q, wq);
+    grid_xf<T>(g, x1, lx1, m3q, n3q, wq, f1);
+    grid_xf<T>(g, x2, lx2, m3q, n3q, wq, f2);
+#pragma unroll
+    for (int f = 0; f < 11; ++f) {
+        f1[f] = f1[f] > (T)0 ? f1[f] : (T)0;
+        f2[f] = f2[f] > (T)0 ? f2[f] : (T)0;
+    }
+    Lumi<T> L;
+    T Q1 = 0, Q2 = 0, same = 0, opp = 0;
+    for (int k = 1; k <= nQ; ++k) {
+        const T q1 = f1[k], a1 = f1[5 + k], q2 = f2[k], a2 = f2[5 + k];
+        Q1 = Q1 + (q1 + a1);
+        Q2 = Q2 + (q2 + a2);
+        same = same + (q1 * q2 + a1 * a2);
+        opp = opp + (q1 * a2 + a1 * q2);
+    }
+    L.g1 = f1[0]; L.g2 = f2[0]; L.Q1 = Q1; L.Q2 = Q2;
+    L.gg = f1[0] * f2[0];
+    L.qg = Q1 * f2[0] + f1[0] * Q2;
+    L.QQ = Q1 * Q2;
+    L.same = same;
+    L.opp = opp;
+    return L;
+}
+
 // Intermediate values the adjoint pass needs from a point (see mi_adjoint.hip).
 template <typename T>
 struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
@@ -240,7 +382,8 @@ struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
                                                float u0, float om0, float u1, float om1,
                                                float u2, float om2, float* ws, float* sHo, float* tHo,
-                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr)
+                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr,
+                                               const GridPdf<float>* gp = nullptr)
 {
     const float LT = k.LT;
     const float targ = -(u0 * LT);
@@ -282,22 +425,32 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     }
     const float ia = b * inv_ab, ib = a * inv_ab;
 
-    const float l1a = own_logf(one_minus_expf(lx1, x1)), l1b = own_logf(one_minus_expf(lx2, x2));
-    const float g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
-    const float g2 = own_expf(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
-    const float S1 = own_expf(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
-    const float S2 = own_expf(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
-    const float V1 = own_expf(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
-    const float V2 = own_expf(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
-    const float Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
-    const float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
-    const float Q1 = __builtin_fmaf(k.nSea, S1, V1 + Vd1), Q2s = __builtin_fmaf(k.nSea, S2, V2 + Vd2);
-    const float SS = S1 * S2;
-    const float Lgg = g1 * g2;
-    const float Lqg = __builtin_fmaf(Q1, g2, g1 * Q2s);
-    const float LQQ = Q1 * Q2s;
-    const float Lsame = __builtin_fmaf(U1, U2, __builtin_fmaf(D1, D2, (k.nSea - 2.0f) * SS));
-    const float Lopp = __builtin_fmaf(U1 + D1, S2, __builtin_fmaf(S1, U2 + D2, (k.nSea - 4.0f) * SS));
+    float Lgg, Lqg, LQQ, Lsame, Lopp, l1a = 0.0f, l1b = 0.0f, g1, g2, Q1, Q2s;
+    if (gp) {
+        // PDF grid at Q2_F = factFac^2 pT2 (q = sqrt, ln q = ln Q2_F / 2)
+        const float Q2F = gp->factFac2 * pT2;
+        const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lx1, x2, lx2, sqrtf(Q2F),
+                                               0.5f * own_logf(Q2F));
+        Lgg = L.gg; Lqg = L.qg; LQQ = L.QQ; Lsame = L.same; Lopp = L.opp;
+        g1 = L.g1; g2 = L.g2; Q1 = L.Q1; Q2s = L.Q2;
+    } else {
+        l1a = own_logf(one_minus_expf(lx1, x1)); l1b = own_logf(one_minus_expf(lx2, x2));
+        g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
+        g2 = own_expf(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
+        const float S1 = own_expf(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
+        const float S2 = own_expf(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
+        const float V1 = own_expf(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
+        const float V2 = own_expf(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
+        const float Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
+        const float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+        Q1 = __builtin_fmaf(k.nSea, S1, V1 + Vd1); Q2s = __builtin_fmaf(k.nSea, S2, V2 + Vd2);
+        const float SS = S1 * S2;
+        Lgg = g1 * g2;
+        Lqg = __builtin_fmaf(Q1, g2, g1 * Q2s);
+        LQQ = Q1 * Q2s;
+        Lsame = __builtin_fmaf(U1, U2, __builtin_fmaf(D1, D2, (k.nSea - 2.0f) * SS));
+        Lopp = __builtin_fmaf(U1 + D1, S2, __builtin_fmaf(S1, U2 + D2, (k.nSea - 4.0f) * SS));
+    }
 
     const float q2 = __builtin_fmaf(a, a, b * b);
     const float common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;
@@ -338,7 +491,8 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
                                                 double u0, double om0, double u1, double om1,
                                                 double u2, double om2, double* ws, double* sHo,
                                                 double* tHo, double* x1o, double* x2o,
-                                                PointAux<double>* aux = nullptr)
+                                                PointAux<double>* aux = nullptr,
+                                                const GridPdf<double>* gp = nullptr)
 {
     const double LT = k.LT;
     const double targ = -(u0 * LT);
@@ -375,22 +529,31 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
         cfo = cf;
     }
 
-    const double l1a = log(-expm1(lx1)), l1b = log(-expm1(lx2));
-    const double g1 = exp(fma(k.bg, l1a, fma(k.ag, lx1, k.lnAg)));
-    const double g2 = exp(fma(k.bg, l1b, fma(k.ag, lx2, k.lnAg)));
-    const double S1 = exp(fma(k.bs, l1a, fma(k.as, lx1, k.lnAs)));
-    const double S2 = exp(fma(k.bs, l1b, fma(k.as, lx2, k.lnAs)));
-    const double V1 = exp(fma(k.bv, l1a, fma(k.av, lx1, k.lnAuv)));
-    const double V2 = exp(fma(k.bv, l1b, fma(k.av, lx2, k.lnAuv)));
-    const double Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
-    const double U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
-    const double Q1 = fma(k.nSea, S1, V1 + Vd1), Q2s = fma(k.nSea, S2, V2 + Vd2);
-    const double SS = S1 * S2;
-    const double Lgg = g1 * g2;
-    const double Lqg = fma(Q1, g2, g1 * Q2s);
-    const double LQQ = Q1 * Q2s;
-    const double Lsame = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
-    const double Lopp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+    double Lgg, Lqg, LQQ, Lsame, Lopp, l1a = 0.0, l1b = 0.0, g1, g2, Q1, Q2s;
+    if (gp) {
+        const double Q2F = gp->factFac2 * pT2;
+        const double q = sqrt(Q2F);
+        const Lumi<double> L = grid_lumi<double>(*gp, (int)(0.5 * k.nSea), x1, lx1, x2, lx2, q, log(q));
+        Lgg = L.gg; Lqg = L.qg; LQQ = L.QQ; Lsame = L.same; Lopp = L.opp;
+        g1 = L.g1; g2 = L.g2; Q1 = L.Q1; Q2s = L.Q2;
+    } else {
+        l1a = log(-expm1(lx1)); l1b = log(-expm1(lx2));
+        g1 = exp(fma(k.bg, l1a, fma(k.ag, lx1, k.lnAg)));
+        g2 = exp(fma(k.bg, l1b, fma(k.ag, lx2, k.lnAg)));
+        const double S1 = exp(fma(k.bs, l1a, fma(k.as, lx1, k.lnAs)));
+        const double S2 = exp(fma(k.bs, l1b, fma(k.as, lx2, k.lnAs)));
+        const double V1 = exp(fma(k.bv, l1a, fma(k.av, lx1, k.lnAuv)));
+        const double V2 = exp(fma(k.bv, l1b, fma(k.av, lx2, k.lnAuv)));
+        const double Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
+        const double U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+        Q1 = fma(k.nSea, S1, V1 + Vd1); Q2s = fma(k.nSea, S2, V2 + Vd2);
+        const double SS = S1 * S2;
+        Lgg = g1 * g2;
+        Lqg = fma(Q1, g2, g1 * Q2s);
+        LQQ = Q1 * Q2s;
+        Lsame = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
+        Lopp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+    }
 
     const double q2 = fma(a, a, b * b);
     const double common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;

@@ -595,7 +595,7 @@ __global__ void k_flow_sample_z(uint64_t key, uint64_t ctr0, uint32_t sid, uint6
 __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int mask, const float *__restrict__ gtab,
                                                         int nbins, uint64_t n, const float *__restrict__ y,
                                                         const float *__restrict__ ld, float *__restrict__ wout,
-                                                        double *__restrict__ partials)
+                                                        double *__restrict__ partials, GridPdf<float> gpdf)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     float *stab = reinterpret_cast<float *>(smem);
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
         }
         float ws[kMaxProc];
         const float f = had_point_f32(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr, nullptr,
-                                      nullptr);
+                                      nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
         const float wt = f * jac * expf(ld[i]);
         if (wout) wout[i] = wt;
         const double dw = (double)wt;
@@ -999,12 +999,16 @@ static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, ui
     }
     HadConst<float> k;
     fill_had_consts_f32(run->phys, k);
+    GridPdf<float> gpdf;
+    int rc = pdf_views(ctx, run->phys, &gpdf, nullptr);
+    if (rc) return rc;
     const int occ = 2;
     const uint32_t nblk = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock,
                                                       (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ);
     if ((size_t)nblk * kRec > ctx->partials_cap) return ferr(ctx, MI_ERR_STATE, "partials buffer too small");
     hipLaunchKernelGGL(k_flow_weight, dim3(nblk), dim3(kBlock), sizeof(float) * 9 * ctx->grid.nbins, ctx->stream, k,
-                       run->phys.proc_mask, ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials);
+                       run->phys.proc_mask, ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials,
+                       gpdf);
     FCHK(hipGetLastError());
     FCHK(launch_reduce_records(ctx->stream, ctx->partials, nblk, out_rec));
     return MI_OK;

@@ -30,6 +30,7 @@ struct Grid {
 };
 
 struct FlowState;              // mi_flow.hip
+struct PdfState;               // mi_pdf.hip
 
 }  // namespace mi
 
@@ -46,6 +47,7 @@ struct mi_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     mi::Grid grid;
     mi::FlowState *flow = nullptr;
+    mi::PdfState *pdf = nullptr;
 };
 
 namespace mi {
@@ -54,12 +56,20 @@ namespace mi {
 void fill_had_consts_f32(const mi_physics &p, HadConst<float> &k);
 void fill_had_consts_f64(const mi_physics &p, HadConst<double> &k);
 
+// ------------------------------------------------------------------ PDF grid (mi_pdf.hip)
+// Views of the loaded grid for p (active only when p.pdf_source == MI_PDF_GRID); MI_ERR_STATE
+// when a grid is asked for and none is loaded.
+int pdf_views(mi_ctx *ctx, const mi_physics &p, GridPdf<float> *g32, GridPdf<double> *g64);
+// Host (FP64) view for host-side evaluation; false when p does not use a grid.
+bool pdf_host_view(const mi_ctx *ctx, const mi_physics &p, GridPdf<double> *g);
+void pdf_release(mi_ctx *ctx);
+
 // ------------------------------------------------------------------ launch geometry
 struct Geom { uint32_t blocks; uint32_t ppt; };
 // Points per thread >= min_ppt, blocks <= max_blocks.
 Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks);
 // Resident blocks per CU of the integration kernels (hipOccupancyMaxActiveBlocksPerMultiprocessor).
-int occ_integrate_had(int prec, bool vegas, bool hist, int nbins);
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid);
 int occ_integrate_c1(bool vegas, bool hist, int nbins);
 
 // ------------------------------------------------------------------ launchers (mi_kernels.hip)
@@ -82,6 +92,8 @@ struct HadLaunch {
     const Grid *grid;
     double *partials;
     double *hist;
+    GridPdf<float> gf;     // PDF grid (active = 0: toy PDFs)
+    GridPdf<double> gd;
 };
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L);
 

@@ -88,18 +88,21 @@ template <> struct HadPointAux<double> {
     { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
 };
 
+// gp: the PDF grid, or nullptr for the toy PDFs.
 template <typename T> struct HadPoint;
 template <> struct HadPoint<float> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
                                                  float u1, float o1, float u2, float o2, float *ws,
-                                                 float *sH, float *tH, float *x1, float *x2)
-    { return had_point_f32(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2); }
+                                                 float *sH, float *tH, float *x1, float *x2,
+                                                 const GridPdf<float> *gp)
+    { return had_point_f32(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
 };
 template <> struct HadPoint<double> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
-                                                  double *sH, double *tH, double *x1, double *x2)
-    { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2); }
+                                                  double *sH, double *tH, double *x1, double *x2,
+                                                  const GridPdf<double> *gp)
+    { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
 };
 
 // One candidate of the hadronic integrand: Philox stream 0 of event e -> three 24-bit
@@ -107,7 +110,7 @@ template <> struct HadPoint<double> {
 template <typename T, bool VEGAS>
 __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint64_t key, uint64_t e,
                                            const T *stab, int nbins, T nbf, T *ws, int *bins,
-                                           T *sH, T *tH, T *x1, T *x2)
+                                           T *sH, T *tH, T *x1, T *x2, const GridPdf<T> *gp = nullptr)
 {
     const U4 r = philox_event(key, e, 0u);
     MapOut<T> m0, m1, m2;
@@ -118,7 +121,7 @@ __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint6
     } else {
         m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
     }
-    T tot = HadPoint<T>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2);
+    T tot = HadPoint<T>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
     if (VEGAS) {
         T jac = (T)1;
         jac = jac * m0.jac;

@@ -106,12 +106,13 @@ template <typename T> __device__ __forceinline__ double max_value(double m) { re
 
 // ------------------------------------------------------------------------------------ hadronic integration
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
-template <typename T, bool VEGAS, bool HIST>
+// GRID: parton densities from the PDF grid gpdf instead of the toy PDFs.
+template <typename T, bool VEGAS, bool HIST, bool GRID>
 __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
                                                           uint64_t ctr0, uint64_t n, uint32_t ppt,
                                                           const T *__restrict__ gtab, int nbins,
                                                           double *__restrict__ partials,
-                                                          double *__restrict__ hist)
+                                                          double *__restrict__ hist, GridPdf<T> gpdf)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *stab = reinterpret_cast<T *>(smem);                                   // 3 dims x nbins x {lo, wd, 1-lo}
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
         T ws[kMaxProc];
         int bins[3];
         const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
-                                              (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr);
+                                              (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
+                                              GRID ? &gpdf : nullptr);
         const double dt = (double)tot;
         v[0] += dt;
         v[7] = fma(dt, dt, v[7]);
@@ -186,29 +188,37 @@ static size_t had_smem(const HadLaunch &L)
     return tb + (L.hist ? sizeof(double) * 4 * 3 * L.grid->nbins : 0);
 }
 
-template <typename T>
-static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab)
+template <typename T, bool GRID>
+static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
+                                const GridPdf<T> &gp)
 {
     const size_t sm = had_smem(L);
     const int nb = L.vegas ? L.grid->nbins : 0;
     dim3 grid(L.g.blocks), blk(kBlock);
     if (!L.vegas)
-        hipLaunchKernelGGL((k_integrate_had<T, false, false>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist);
+        hipLaunchKernelGGL((k_integrate_had<T, false, false, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else if (!L.hist)
-        hipLaunchKernelGGL((k_integrate_had<T, true, false>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist);
+        hipLaunchKernelGGL((k_integrate_had<T, true, false, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else
-        hipLaunchKernelGGL((k_integrate_had<T, true, true>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist);
+        hipLaunchKernelGGL((k_integrate_had<T, true, true, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
+                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
     return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
+                               const GridPdf<T> &gp)
+{
+    return gp.active ? launch_had_tg<T, true>(s, L, k, tab, gp) : launch_had_tg<T, false>(s, L, k, tab, gp);
 }
 
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L)
 {
     if (L.prec == MI_PREC_FP32)
-        return launch_had_t<float>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr);
-    return launch_had_t<double>(s, L, L.kd, L.vegas ? L.grid->tab64 : nullptr);
+        return launch_had_t<float>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr, L.gf);
+    return launch_had_t<double>(s, L, L.kd, L.vegas ? L.grid->tab64 : nullptr, L.gd);
 }
 
 // ------------------------------------------------------------------------------------ partonic integration
@@ -278,7 +288,8 @@ hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L)
 template <typename T, bool VEGAS>
 __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, uint64_t key, uint64_t ctr0,
                                                        uint64_t n, const T *__restrict__ gtab, int nbins,
-                                                       double *__restrict__ w, double *__restrict__ kin)
+                                                       double *__restrict__ w, double *__restrict__ kin,
+                                                       GridPdf<T> gpdf)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, 
     int bins[3];
     T sH, tH, x1, x2;
     const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, &sH, &tH,
-                                          &x1, &x2);
+                                          &x1, &x2, gpdf.active ? &gpdf : nullptr);
     w[i * (kMaxProc + 1)] = (double)tot;
 #pragma unroll
     for (int p = 0; p < kMaxProc; ++p) w[i * (kMaxProc + 1) + 1 + p] = (double)ws[p];
@@ -303,12 +314,12 @@ hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, doubl
     const int nb = L.vegas ? L.grid->nbins : 0;
     if (L.prec == MI_PREC_FP32) {
         const float *tab = L.vegas ? L.grid->tab32 : nullptr;
-        if (L.vegas) hipLaunchKernelGGL((k_points_had<float, true>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
-        else         hipLaunchKernelGGL((k_points_had<float, false>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+        if (L.vegas) hipLaunchKernelGGL((k_points_had<float, true>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gf);
+        else         hipLaunchKernelGGL((k_points_had<float, false>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gf);
     } else {
         const double *tab = L.vegas ? L.grid->tab64 : nullptr;
-        if (L.vegas) hipLaunchKernelGGL((k_points_had<double, true>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
-        else         hipLaunchKernelGGL((k_points_had<double, false>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin);
+        if (L.vegas) hipLaunchKernelGGL((k_points_had<double, true>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gd);
+        else         hipLaunchKernelGGL((k_points_had<double, false>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gd);
     }
     return hipGetLastError();
 }
@@ -481,25 +492,26 @@ hipError_t launch_vegas_tables(hipStream_t s, Grid &g)
 
 // ------------------------------------------------------------------------------------ occupancy
 // Resident blocks per CU of each integration kernel instance (for one-wave, equal-work grids).
-int occ_integrate_had(int prec, bool vegas, bool hist, int nbins)
+template <typename T, bool GRID>
+static int occ_had_t(bool vegas, bool hist, size_t sm)
+{
+    int nb = 0;
+    hipError_t e;
+    if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, false, false, GRID>, kBlock, sm);
+    else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, false, GRID>, kBlock, sm);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, true, GRID>, kBlock, sm);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid)
 {
     HadLaunch L{};
     L.prec = prec; L.vegas = vegas;
     Grid g; g.nbins = nbins; g.ndim = 3;
     L.grid = &g; L.hist = hist ? (double *)1 : nullptr;
     const size_t sm = had_smem(L);
-    int nb = 0;
-    hipError_t e;
-    if (prec == MI_PREC_FP32) {
-        if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, false, false>, kBlock, sm);
-        else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, true, false>, kBlock, sm);
-        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<float, true, true>, kBlock, sm);
-    } else {
-        if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, false, false>, kBlock, sm);
-        else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, true, false>, kBlock, sm);
-        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<double, true, true>, kBlock, sm);
-    }
-    return (e == hipSuccess && nb > 0) ? nb : 1;
+    if (prec == MI_PREC_FP32) return grid ? occ_had_t<float, true>(vegas, hist, sm) : occ_had_t<float, false>(vegas, hist, sm);
+    return grid ? occ_had_t<double, true>(vegas, hist, sm) : occ_had_t<double, false>(vegas, hist, sm);
 }
 
 int occ_integrate_c1(bool vegas, bool hist, int nbins)

@@ -94,6 +94,10 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
     }
     const bool partonic = phys->mode == MI_MODE_PARTONIC_1D;
     if (!partonic && (phys->nQuarkIn < 2 || phys->nQuarkIn > 5)) { ctx->err = "nQuarkIn must be 2..5"; return MI_ERR_ARG; }
+    if (!partonic && phys->pdf_source == MI_PDF_GRID && !ctx->pdf) {
+        ctx->err = "pdf_source = MI_PDF_GRID but no grid loaded (mi_pdf_load)";
+        return MI_ERR_STATE;
+    }
     if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return MI_ERR_HIP; }
     std::vector<double> sH(n), tH(n), x1(n), x2(n), w(n);
     std::vector<int32_t> pid(n);
@@ -117,6 +121,8 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
     const double eBeam = partonic ? 0.5 * std::sqrt(phys->sHat) : 0.5 * phys->eCM;
     const int beamId = partonic ? 21 : 2212;
     const Toy toy = make_toy(*phys);
+    GridPdf<double> hg;
+    const bool grid = !partonic && pdf_host_view(ctx, *phys, &hg);
     const int nQ = partonic ? 0 : phys->nQuarkIn;
     const double mb2pb = 1e9;
     std::fprintf(f, "<LesHouchesEvents version=\"3.0\">\n<header>\n"
@@ -146,9 +152,24 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
             std::vector<std::pair<int, int>> pairs;
             std::vector<double> wt;
             const double X1 = x1[i], X2 = x2[i];
+            // parton densities: the toy PDFs, or the grid at mu_F^2 = fact_fac^2 pT^2 (clipped at 0)
+            double g1[11] = {}, g2[11] = {};
+            if (grid) {
+                const double Q2F = hg.factFac2 * (t * u / s), q = std::sqrt(Q2F);
+                int m3q, n3q;
+                double wq[4];
+                grid_qw<double>(hg, q, std::log(q), m3q, n3q, wq);
+                grid_xf<double>(hg, X1, std::log(X1), m3q, n3q, wq, g1);
+                grid_xf<double>(hg, X2, std::log(X2), m3q, n3q, wq, g2);
+            }
+            auto xfa = [&](const double *gv, int id, double X) {
+                if (!grid) return toy.xf(id, X);
+                const double v = gv[id == 21 ? 0 : (id > 0 ? id : 5 - id)];
+                return v > 0.0 ? v : 0.0;
+            };
             auto add = [&](int a, int b, double br) {
                 pairs.push_back({a, b});
-                wt.push_back(toy.xf(a, X1) * toy.xf(b, X2) * br);
+                wt.push_back(xfa(g1, a, X1) * xfa(g2, b, X2) * br);
             };
             for (int q = -nQ; q <= nQ; ++q) {
                 if (q == 0) continue;

@@ -50,12 +50,12 @@ __device__ __forceinline__ void decide(uint64_t key, uint64_t e, T tot, const T 
 
 template <typename T, bool VEGAS>
 __device__ __forceinline__ void had_cand(const HadConst<T> &k, int mask, uint64_t key, uint64_t e, const T *stab,
-                                         int nbins, double wmax, Cand &c)
+                                         int nbins, double wmax, const GridPdf<T> *gp, Cand &c)
 {
     T ws[kMaxProc];
     int bins[3];
     T sH, tH, x1, x2;
-    const T tot = had_candidate<T, VEGAS>(k, mask, key, e, stab, nbins, (T)nbins, ws, bins, &sH, &tH, &x1, &x2);
+    const T tot = had_candidate<T, VEGAS>(k, mask, key, e, stab, nbins, (T)nbins, ws, bins, &sH, &tH, &x1, &x2, gp);
     c.w = (double)tot;
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) c.ws[i] = (double)ws[i];
@@ -100,13 +100,15 @@ struct Params {
     double wmax;
     double pref, cMax, sHat;
     int compat;
+    GridPdf<T> gpdf;       // PDF grid (active = 0: toy PDFs)
 };
 
 template <int KIND, typename T>
 __device__ __forceinline__ void candidate(const Params<KIND, T> &P, const T *stab, uint64_t e, Cand &c)
 {
-    if (KIND == 0) had_cand<T, false>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, c);
-    else if (KIND == 1) had_cand<T, true>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, c);
+    const GridPdf<T> *gp = P.gpdf.active ? &P.gpdf : nullptr;
+    if (KIND == 0) had_cand<T, false>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
+    else if (KIND == 1) had_cand<T, true>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
     else c1_cand(P.pref, P.cMax, P.sHat, P.key, e, P.compat != 0, P.wmax, c);
 }
 
@@ -212,11 +214,15 @@ __global__ __launch_bounds__(kBlock) void k_unw_write(Params<KIND, T> P, const u
     }
 }
 
+inline const GridPdf<float> &grid_of(const HadLaunch &h, float *) { return h.gf; }
+inline const GridPdf<double> &grid_of(const HadLaunch &h, double *) { return h.gd; }
+
 template <int KIND, typename T>
 Params<KIND, T> make_params(const UnwLaunch &L, const HadConst<T> &k, const T *tab)
 {
     Params<KIND, T> P{};
     P.k = k;
+    if (!L.partonic) P.gpdf = grid_of(L.h, (T *)nullptr);
     if (L.partonic) {
         P.key = L.c.key; P.ctr0 = L.c.ctr0; P.n = L.c.n; P.ppt = L.c.g.ppt;
     } else {

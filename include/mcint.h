@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 1
+#define MI_ABI_VERSION 2
 
 typedef struct mi_ctx mi_ctx;   /* opaque */
 
@@ -59,6 +59,7 @@ typedef struct mi_ctx mi_ctx;   /* opaque */
 enum mi_mode      { MI_MODE_PARTONIC_1D = 0, MI_MODE_HADRONIC_3D = 1 };
 enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1 };
 enum mi_sampler   { MI_SAMPLER_FLAT = 0, MI_SAMPLER_VEGAS = 1, MI_SAMPLER_FLOW = 2 };
+enum mi_pdf_source { MI_PDF_TOY = 0, MI_PDF_GRID = 1 };
 enum mi_rng       { MI_RNG_PHILOX = 0, MI_RNG_SPLITMIX_COMPAT = 1 };
 
 /* Physics configuration.  Defaults (mi_physics_default) mirror Pythia's settings where the
@@ -84,6 +85,8 @@ typedef struct mi_physics {
     double pdf_val[2];      /* valence (u_v, d_v) */
     double pdf_sea[2];      /* sea (every quark and antiquark) */
     double kfactor;
+    int    pdf_source;      /* enum mi_pdf_source: toy PDFs above, or the grid of mi_pdf_load */
+    double fact_fac;        /* mu_F = fact_fac * pT (PDF scale with a grid) */
 } mi_physics;
 
 typedef struct mi_run {
@@ -242,8 +245,22 @@ int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const do
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
  * same points, FP64 accumulation.  Hadronic mode with alphaS_order 1 (or 0 for the fixed-
- * coupling alphaS^2 check). */
+ * coupling alphaS^2 check) and the toy PDFs (the gluon shape parameters are theirs). */
 int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out);
+
+/* ------------------------------------------------------------- PDF grids ---- */
+/* Loads a proton PDF from an LHAPDF6 "lhagrid1" data file into the context, read as Pythia's
+ * LHAGrid1 does (pythia8317/src/PartonDistributions.cc:478-598, e.g. the default
+ * NNPDF23_lo_as_0130_qed_0000.dat); runs with phys.pdf_source = MI_PDF_GRID then take their
+ * parton densities from it at mu_F^2 = fact_fac^2 pT^2, interpolated as LHAGrid1::xfxevolve
+ * (:648-745) with PDF:extrapolate = off.  Replaces cuPythia's grid fill and interpolator
+ * (cuPythia/pipeline/genpdf.cc:18-70, pdf.cuh:21-49).  A later call replaces the grid. */
+int mi_pdf_load(mi_ctx *ctx, const char *path);
+/* Batched PDF evaluation (FP64, device arrays): xf_dev[i * 12 + slot] for slots g, d, u, s, c,
+ * b, dbar, ubar, sbar, cbar, bbar (11 = photon, written 0), raw interpolated values (Pythia's
+ * PDF::xf clips them at 0, PartonDistributions.cc:122-133).  The counterpart of Pythia's
+ * PDF::xf(id, x, Q2) for all flavours at once. */
+int mi_pdf_xf(mi_ctx *ctx, uint64_t n, const double *x_dev, const double *Q2_dev, double *xf_dev);
 
 /* ------------------------------------------------------------------- flow ---- */
 /* Identity initialisation (last layer of every conditioner zero) with hidden layers drawn

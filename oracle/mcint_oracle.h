@@ -89,6 +89,33 @@ void   orc_toypdf_init(orc_toypdf *p, const double g_ab[2], const double v_ab[2]
                        const double s_ab[2], int nQuarkIn);
 double orc_toypdf_xf(const orc_toypdf *p, int id, double x);
 
+/* -------------------------------------------- PDF grids (LHAGrid1) ---- */
+/* LHAPDF6 "lhagrid1" data file read as Pythia's LHAGrid1::init does
+   (pythia8317/src/PartonDistributions.cc:478-598): one x grid shared by all Q subgrids, the
+   subgrids' Q values concatenated with each shared border averaged in q (the log grid keeps
+   the values as read), flavour columns mapped to slots 0 = g, 1..5 = d u s c b,
+   6..10 = dbar ubar sbar cbar bbar, 11 = photon (others dropped).  Values flavour-interleaved:
+   val[(iq * nx + ix) * 12 + slot].
+   Pinned: node exactness and cubic exactness of the interpolation on synthetic files, the
+   subgrid borders, and the momentum / valence sum rules of the NNPDF2.3 LO file
+   (tests/test_oracle_pdfgrid.py). */
+#define ORC_GRID_NF 12
+#define ORC_GRID_MAXSUB 8
+typedef struct {
+  int nx, nq, nqSub;
+  double *x, *lnx, *q, *lnq;
+  double *val;
+  int nqSum[ORC_GRID_MAXSUB];        /* cumulative Q count after subgrid s */
+  double qDiv[ORC_GRID_MAXSUB];      /* last Q of subgrid s, as read */
+  double xMin, xMax, qMin, qMax;
+} orc_lhagrid;
+int  orc_lhagrid_load(const char *path, orc_lhagrid **out);   /* 0 on success */
+void orc_lhagrid_free(orc_lhagrid *g);
+/* LHAGrid1::xfxevolve (PartonDistributions.cc:648-745) with PDF:extrapolate = off: 4-point
+   Lagrange interpolation in ln x and ln q (linear in ln q on 2-3 point subgrids), frozen at the
+   q borders and at x <= xMin, 0 at x >= xMax; raw values (Pythia's xf clips them at 0). */
+void orc_lhagrid_xf(const orc_lhagrid *g, double x, double Q2, double out[12]);
+
 /* ------------------------------------------------ hadronic integrand ---- */
 typedef struct {
   double eCM, pTHatMin;
@@ -98,6 +125,9 @@ typedef struct {
   int nQuarkIn, nQuarkNew;
   int procMask;              /* bit i <-> code 111+i */
   double kfactor;
+  int pdfSource;             /* 0: toy PDFs (pdf); 1: the grid below */
+  double factFac;            /* mu_F = factFac * pT  (Q2Fac = factFac^2 * pT2) */
+  const orc_lhagrid *grid;
 } orc_hadcfg;
 void orc_hadcfg_default(orc_hadcfg *c);   /* the 5-process library at sqrt(S) = 13.6 TeV */
 
@@ -154,6 +184,21 @@ float orc32_had_point(const float k[ORC32_NCONST], int procMask, float u0, float
 /* Same with the complements 1 - u_i given explicitly (as the VEGAS map provides them). */
 float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, float om0, float u1,
                          float om1, float u2, float om2, float w[6], float kin[4]);
+/* FP32 tables of a PDF grid exactly as the library builds them for its FP32 kernels: nodes,
+   logs and values rounded from FP64, reciprocals of the Lagrange denominators computed in FP64
+   and rounded (12 per stencil start; [12 m] = 1 / (lnq[m+1] - lnq[m]) on 2-3 point Q subgrids). */
+typedef struct {
+  int nx, nq, nqSub;
+  float *xk, *lnx, *qk, *lnq, *val, *ixinv, *iqinv;
+  int nqSum[ORC_GRID_MAXSUB];
+  float qDiv[ORC_GRID_MAXSUB];
+  float xMin, xMax, qMin, qMax, factFac2;
+} orc32_grid;
+void orc32_grid_build(const orc_lhagrid *g, double factFac, orc32_grid *out);
+void orc32_grid_free(orc32_grid *g);
+/* orc32_had_point_om with the parton densities from the FP32 grid g (NULL: the toy PDFs). */
+float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int procMask, float u0, float om0,
+                        float u1, float om1, float u2, float om2, float w[6], float kin[4]);
 
 /* ------------------------------------------------ batched points, VEGAS, unweighting ---- */
 /* Events ctr0 .. ctr0+n-1 (Philox stream 0, three 24-bit uniforms) through the flat map

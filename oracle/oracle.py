@@ -36,10 +36,48 @@ class OrcToyPdf(C.Structure):
                 ("nQuarkIn", C.c_int)]
 
 
+class OrcLhaGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nq", C.c_int), ("nqSub", C.c_int),
+                ("x", C.POINTER(C.c_double)), ("lnx", C.POINTER(C.c_double)),
+                ("q", C.POINTER(C.c_double)), ("lnq", C.POINTER(C.c_double)),
+                ("val", C.POINTER(C.c_double)), ("nqSum", C.c_int * 8), ("qDiv", C.c_double * 8),
+                ("xMin", C.c_double), ("xMax", C.c_double), ("qMin", C.c_double), ("qMax", C.c_double)]
+
+
 class OrcHadCfg(C.Structure):
     _fields_ = [("eCM", C.c_double), ("pTHatMin", C.c_double), ("as_", OrcAlphas),
                 ("renormFac", C.c_double), ("pdf", OrcToyPdf), ("nQuarkIn", C.c_int),
-                ("nQuarkNew", C.c_int), ("procMask", C.c_int), ("kfactor", C.c_double)]
+                ("nQuarkNew", C.c_int), ("procMask", C.c_int), ("kfactor", C.c_double),
+                ("pdfSource", C.c_int), ("factFac", C.c_double), ("grid", C.POINTER(OrcLhaGrid))]
+
+
+class LhaGrid:
+    """An LHAPDF6 lhagrid1 file read by the oracle (Pythia's LHAGrid1 semantics)."""
+
+    def __init__(self, path: str):
+        ptr = C.POINTER(OrcLhaGrid)()
+        rc = lib().orc_lhagrid_load(path.encode(), C.byref(ptr))
+        if rc != 0:
+            raise ValueError(f"cannot read lhagrid1 file {path} (code {rc})")
+        self.ptr = ptr
+        self.g = ptr.contents
+
+    def xf(self, x: float, Q2: float):
+        """Raw interpolated xf for the 12 slots (g, d u s c b, dbar ubar sbar cbar bbar, photon)."""
+        out = (C.c_double * 12)()
+        lib().orc_lhagrid_xf(self.ptr, x, Q2, out)
+        return list(out)
+
+    def close(self):
+        if self.ptr:
+            lib().orc_lhagrid_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class OrcKin(C.Structure):
@@ -102,6 +140,12 @@ def _bind_extras(L):
     d, i, u64 = C.c_double, C.c_int, C.c_uint64
     P = C.POINTER
     f = C.c_float
+    L.orc_lhagrid_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(OrcLhaGrid))]
+    L.orc_lhagrid_load.restype = C.c_int
+    L.orc_lhagrid_free.argtypes = [C.POINTER(OrcLhaGrid)]
+    L.orc_lhagrid_free.restype = None
+    L.orc_lhagrid_xf.argtypes = [C.POINTER(OrcLhaGrid), C.c_double, C.c_double, C.POINTER(C.c_double)]
+    L.orc_lhagrid_xf.restype = None
     specs = {
         "orc_had_point_ld": ([P(OrcHadCfg), P(d), P(d), P(OrcKin), i], d),
         "orc32_had_point_om": ([P(f), i, f, f, f, f, f, f, P(f), P(f)], f),
@@ -148,8 +192,13 @@ def hadcfg_default(**kw) -> OrcHadCfg:
     alpha_order = kw.pop("alphaS_order", None)
     alpha_nfmax = kw.pop("alphaS_nfmax", None)
     pdf_g = kw.pop("pdf_g", None)
+    grid = kw.pop("grid", None)
     for k, v in kw.items():
         setattr(cfg, k, v)
+    if grid is not None:
+        cfg.grid = grid.ptr
+        cfg.pdfSource = 1
+        cfg._grid_ref = grid            # keeps the grid alive with the config
     if alpha_value is not None or alpha_order is not None or alpha_nfmax is not None:
         lib().orc_alphas_init(C.byref(cfg.as_),
                               cfg.as_.valueRef if alpha_value is None else alpha_value,

@@ -23,6 +23,7 @@
  */
 #include "mcint_oracle.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #ifndef M_PI
@@ -133,8 +134,157 @@ float orc32_had_point(const float k[ORC32_NCONST], int procMask, float u0, float
     return orc32_had_point_om(k, procMask, u0, 1.0f - u0, u1, 1.0f - u1, u2, 1.0f - u2, w, kin);
 }
 
+/* ------------------------------------------------------------------ FP32 PDF grid */
+static float *f32copy(const double *a, size_t n)
+{
+    float *o = (float *)malloc(sizeof(float) * (n ? n : 1));
+    for (size_t i = 0; i < n; ++i) o[i] = (float)a[i];
+    return o;
+}
+
+static float *lagrange_inv32(const double *t, int n)
+{
+    float *inv = (float *)calloc((size_t)ORC_GRID_NF * n, sizeof(float));
+    for (int m = 0; m + 3 < n; ++m) {
+        int k = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                if (j != i) inv[(size_t)ORC_GRID_NF * m + k++] = (float)(1.0 / (t[m + i] - t[m + j]));
+    }
+    return inv;
+}
+
+void orc32_grid_build(const orc_lhagrid *g, double factFac, orc32_grid *o)
+{
+    memset(o, 0, sizeof *o);
+    o->nx = g->nx; o->nq = g->nq; o->nqSub = g->nqSub;
+    o->xk = f32copy(g->x, (size_t)g->nx);
+    o->lnx = f32copy(g->lnx, (size_t)g->nx);
+    o->qk = f32copy(g->q, (size_t)g->nq);
+    o->lnq = f32copy(g->lnq, (size_t)g->nq);
+    o->val = f32copy(g->val, (size_t)g->nq * g->nx * ORC_GRID_NF);
+    o->ixinv = lagrange_inv32(g->lnx, g->nx);
+    o->iqinv = lagrange_inv32(g->lnq, g->nq);
+    for (int s = 0; s < g->nqSub; ++s) {
+        int minS = s == 0 ? 0 : g->nqSum[s - 1], maxS = g->nqSum[s] - 1;
+        if (maxS - minS < 3)
+            for (int m = minS; m < maxS; ++m) o->iqinv[(size_t)ORC_GRID_NF * m] = (float)(1.0 / (g->lnq[m + 1] - g->lnq[m]));
+    }
+    for (int s = 0; s < ORC_GRID_MAXSUB; ++s) { o->nqSum[s] = g->nqSum[s]; o->qDiv[s] = (float)g->qDiv[s]; }
+    o->xMin = (float)g->x[0]; o->xMax = (float)g->x[g->nx - 1];
+    o->qMin = (float)g->q[0]; o->qMax = (float)g->q[g->nq - 1];
+    o->factFac2 = (float)(factFac * factFac);
+}
+
+void orc32_grid_free(orc32_grid *g)
+{
+    free(g->xk); free(g->lnx); free(g->qk); free(g->lnq); free(g->val); free(g->ixinv); free(g->iqinv);
+    memset(g, 0, sizeof *g);
+}
+
+/* The device's FP32 grid evaluation, operation for operation (GridPdf, grid_qw, grid_xf,
+ * grid_lumi in the device header): Lagrange weights by reciprocals, slots 0..10. */
+static void lagrange4_32(float t, const float *n, const float *inv, float *w)
+{
+    int k = 0;
+    for (int i = 0; i < 4; ++i) {
+        w[i] = 1.0f;
+        for (int j = 0; j < 4; ++j)
+            if (j != i) { w[i] *= (t - n[j]) * inv[k]; ++k; }
+    }
+}
+
+static void grid_qw32(const orc32_grid *g, float q, float lnq, int *m3q, int *n3q, float *wq)
+{
+    wq[0] = wq[1] = wq[2] = wq[3] = 1.0f;
+    if (q <= g->qMin) { *n3q = 1; *m3q = 0; return; }
+    if (q >= g->qMax) { *n3q = 1; *m3q = g->nq - 1; return; }
+    int iqDiv = 0;
+    for (int s = 1; s < g->nqSub; ++s)
+        if (q > g->qDiv[s - 1]) iqDiv = s;
+    int minS = iqDiv == 0 ? 0 : g->nqSum[iqDiv - 1], maxS = g->nqSum[iqDiv] - 1;
+    int minq = minS, maxq = maxS;
+    while (maxq - minq > 1) {
+        int mid = (minq + maxq) >> 1;
+        if (q < g->qk[mid]) maxq = mid;
+        else                minq = mid;
+    }
+    if (maxS - minS < 3) {
+        *n3q = 2;
+        *m3q = minq;
+        wq[1] = (lnq - g->lnq[minq]) * g->iqinv[ORC_GRID_NF * minq];
+        wq[0] = 1.0f - wq[1];
+    } else {
+        *n3q = 4;
+        *m3q = (minq == minS) ? minS : ((maxq == maxS) ? maxS - 3 : minq - 1);
+        lagrange4_32(lnq, g->lnq + *m3q, g->iqinv + ORC_GRID_NF * *m3q, wq);
+    }
+}
+
+static void grid_xf32(const orc32_grid *g, float x, float lnx, int m3q, int n3q, const float *wq, float *out)
+{
+    for (int f = 0; f < 11; ++f) out[f] = 0.0f;
+    if (x <= g->xMin) {
+        for (int i3q = 0; i3q < n3q; ++i3q) {
+            const float *p = g->val + (size_t)(m3q + i3q) * g->nx * ORC_GRID_NF;
+            for (int f = 0; f < 11; ++f) out[f] += wq[i3q] * p[f];
+        }
+        return;
+    }
+    if (x >= g->xMax) return;
+    int minx = 0, maxx = g->nx - 1;
+    while (maxx - minx > 1) {
+        int mid = (minx + maxx) >> 1;
+        if (x < g->xk[mid]) maxx = mid;
+        else                minx = mid;
+    }
+    int m3x = (minx == 0) ? 0 : ((maxx == g->nx - 1) ? g->nx - 4 : minx - 1);
+    float wx[4];
+    lagrange4_32(lnx, g->lnx + m3x, g->ixinv + ORC_GRID_NF * m3x, wx);
+    for (int i3q = 0; i3q < n3q; ++i3q) {
+        const float *p = g->val + ((size_t)(m3q + i3q) * g->nx + m3x) * ORC_GRID_NF;
+        for (int f = 0; f < 11; ++f)
+            out[f] += wq[i3q] * (wx[0] * p[f] + wx[1] * p[ORC_GRID_NF + f] + wx[2] * p[2 * ORC_GRID_NF + f]
+                                 + wx[3] * p[3 * ORC_GRID_NF + f]);
+    }
+}
+
+/* luminosities gg, qg, QQ, same, opp from the grid at (x1, x2), q (grid_lumi) */
+static void grid_lumi32(const orc32_grid *g, int nQ, float x1, float lx1, float x2, float lx2, float q, float lnq,
+                        float L[5])
+{
+    int m3q, n3q;
+    float wq[4], f1[11], f2[11];
+    grid_qw32(g, q, lnq, &m3q, &n3q, wq);
+    grid_xf32(g, x1, lx1, m3q, n3q, wq, f1);
+    grid_xf32(g, x2, lx2, m3q, n3q, wq, f2);
+    for (int f = 0; f < 11; ++f) {
+        f1[f] = f1[f] > 0.0f ? f1[f] : 0.0f;
+        f2[f] = f2[f] > 0.0f ? f2[f] : 0.0f;
+    }
+    float Q1 = 0.0f, Q2 = 0.0f, same = 0.0f, opp = 0.0f;
+    for (int k = 1; k <= nQ; ++k) {
+        float q1 = f1[k], a1 = f1[5 + k], q2 = f2[k], a2 = f2[5 + k];
+        Q1 = Q1 + (q1 + a1);
+        Q2 = Q2 + (q2 + a2);
+        same = same + (q1 * q2 + a1 * a2);
+        opp = opp + (q1 * a2 + a1 * q2);
+    }
+    L[0] = f1[0] * f2[0];
+    L[1] = Q1 * f2[0] + f1[0] * Q2;
+    L[2] = Q1 * Q2;
+    L[3] = same;
+    L[4] = opp;
+}
+
 float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, float om0, float u1,
                          float om1, float u2, float om2, float w[6], float kin[4])
+{
+    return orc32_had_point_g(k, NULL, procMask, u0, om0, u1, om1, u2, om2, w, kin);
+}
+
+float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int procMask, float u0, float om0,
+                        float u1, float om1, float u2, float om2, float w[6], float kin[4])
 {
     const float LT = k[0];
     float r = orc_expf(-(u0 * LT));                    /* tauMin / tau */
@@ -178,22 +328,30 @@ float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, fl
     float ia = b * inv_ab, ib = a * inv_ab;
 
     /* parton densities at x1 and x2 */
-    float l1a = orc_logf(one_minus_exp(lx1, x1)), l1b = orc_logf(one_minus_exp(lx2, x2));
-    float g1 = orc_expf(fmaf(k[21], l1a, fmaf(k[20], lx1, k[19])));
-    float g2 = orc_expf(fmaf(k[21], l1b, fmaf(k[20], lx2, k[19])));
-    float S1 = orc_expf(fmaf(k[24], l1a, fmaf(k[23], lx1, k[22])));
-    float S2 = orc_expf(fmaf(k[24], l1b, fmaf(k[23], lx2, k[22])));
-    float V1 = orc_expf(fmaf(k[27], l1a, fmaf(k[26], lx1, k[25])));
-    float V2 = orc_expf(fmaf(k[27], l1b, fmaf(k[26], lx2, k[25])));
-    float Vd1 = k[28] * V1, Vd2 = k[28] * V2;
-    float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
-    float Q1 = fmaf(k[29], S1, V1 + Vd1), Q2s = fmaf(k[29], S2, V2 + Vd2);
-    float SS = S1 * S2;
-    float Lgg = g1 * g2;
-    float Lqg = fmaf(Q1, g2, g1 * Q2s);
-    float LQQ = Q1 * Q2s;
-    float Lsame = fmaf(U1, U2, fmaf(D1, D2, (k[29] - 2.0f) * SS));
-    float Lopp = fmaf(U1 + D1, S2, fmaf(S1, U2 + D2, (k[29] - 4.0f) * SS));
+    float Lgg, Lqg, LQQ, Lsame, Lopp;
+    if (g) {
+        float Q2F = g->factFac2 * pT2;
+        float L[5];
+        grid_lumi32(g, (int)(0.5f * k[29]), x1, lx1, x2, lx2, sqrtf(Q2F), 0.5f * orc_logf(Q2F), L);
+        Lgg = L[0]; Lqg = L[1]; LQQ = L[2]; Lsame = L[3]; Lopp = L[4];
+    } else {
+        float l1a = orc_logf(one_minus_exp(lx1, x1)), l1b = orc_logf(one_minus_exp(lx2, x2));
+        float g1 = orc_expf(fmaf(k[21], l1a, fmaf(k[20], lx1, k[19])));
+        float g2 = orc_expf(fmaf(k[21], l1b, fmaf(k[20], lx2, k[19])));
+        float S1 = orc_expf(fmaf(k[24], l1a, fmaf(k[23], lx1, k[22])));
+        float S2 = orc_expf(fmaf(k[24], l1b, fmaf(k[23], lx2, k[22])));
+        float V1 = orc_expf(fmaf(k[27], l1a, fmaf(k[26], lx1, k[25])));
+        float V2 = orc_expf(fmaf(k[27], l1b, fmaf(k[26], lx2, k[25])));
+        float Vd1 = k[28] * V1, Vd2 = k[28] * V2;
+        float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+        float Q1 = fmaf(k[29], S1, V1 + Vd1), Q2s = fmaf(k[29], S2, V2 + Vd2);
+        float SS = S1 * S2;
+        Lgg = g1 * g2;
+        Lqg = fmaf(Q1, g2, g1 * Q2s);
+        LQQ = Q1 * Q2s;
+        Lsame = fmaf(U1, U2, fmaf(D1, D2, (k[29] - 2.0f) * SS));
+        Lopp = fmaf(U1 + D1, S2, fmaf(S1, U2 + D2, (k[29] - 4.0f) * SS));
+    }
 
     /* brackets: d sigma / dt = pi alpS^2 / sH^2 * bracket */
     float q2 = fmaf(a, a, b * b);

@@ -87,9 +87,96 @@ static REAL FN(xf)(const orc_toypdf *p, int id, REAL x)
     return sea;
 }
 
-static void FN(fill_xf)(const orc_hadcfg *c, REAL x, REAL *xf)
+/* LHAGrid1::xfxevolve (PartonDistributions.cc:648-745), PDF:extrapolate = off: the grid
+ * neighbours by bisection in x and in q (within the q subgrid that holds q), 4-point Lagrange
+ * weights in ln x and ln q (linear in ln q on subgrids of fewer than 4 points), frozen outside
+ * the q range and below xMin.  x >= xMax (x = 1) gives 0 (Pythia keeps the previous values). */
+static void FN(grid_xf)(const orc_lhagrid *g, REAL x, REAL Q2, REAL out[ORC_GRID_NF])
+{
+    REAL q = M_SQRT(Q2);
+    int nx = g->nx, nq = g->nq;
+    int inx = (x <= (REAL)g->xMin) ? -1 : ((x >= (REAL)g->xMax) ? 1 : 0);
+    int inq = (q <= (REAL)g->qMin) ? -1 : ((q >= (REAL)g->qMax) ? 1 : 0);
+    int minx = 0, maxx = nx - 1, m3x = 0;
+    REAL wx[4] = { 1, 1, 1, 1 };
+    if (inx == 0) {
+        while (maxx - minx > 1) {
+            int midx = (minx + maxx) / 2;
+            if (x < (REAL)g->x[midx]) maxx = midx;
+            else                      minx = midx;
+        }
+        REAL lnx = M_LOG(x);
+        if      (minx == 0)      m3x = 0;
+        else if (maxx == nx - 1) m3x = nx - 4;
+        else                     m3x = minx - 1;
+        for (int i3 = 0; i3 < 4; ++i3)
+            for (int j = 0; j < 4; ++j)
+                if (j != i3) wx[i3] *= (lnx - (REAL)g->lnx[m3x + j]) / ((REAL)g->lnx[m3x + i3] - (REAL)g->lnx[m3x + j]);
+    }
+    int iqDiv = 0;
+    for (int s = 1; s < g->nqSub; ++s)
+        if (q > (REAL)g->qDiv[s - 1]) iqDiv = s;
+    int minS = (iqDiv == 0) ? 0 : g->nqSum[iqDiv - 1];
+    int maxS = g->nqSum[iqDiv] - 1;
+    int minq = minS, maxq = maxS, n3q = 4, m3q = 0;
+    REAL wq[4] = { 1, 1, 1, 1 };
+    if (inq == 0) {
+        while (maxq - minq > 1) {
+            int midq = (minq + maxq) / 2;
+            if (q < (REAL)g->q[midq]) maxq = midq;
+            else                      minq = midq;
+        }
+        REAL lnq = M_LOG(q);
+        if (maxS - minS < 3) {
+            n3q = 2;
+            m3q = minq;
+            wq[1] = (lnq - (REAL)g->lnq[minq]) / ((REAL)g->lnq[maxq] - (REAL)g->lnq[minq]);
+            wq[0] = 1 - wq[1];
+        } else {
+            if      (minq == minS) m3q = minS;
+            else if (maxq == maxS) m3q = maxS - 3;
+            else                   m3q = minq - 1;
+            for (int i3 = 0; i3 < 4; ++i3)
+                for (int j = 0; j < 4; ++j)
+                    if (j != i3) wq[i3] *= (lnq - (REAL)g->lnq[m3q + j]) / ((REAL)g->lnq[m3q + i3] - (REAL)g->lnq[m3q + j]);
+        }
+    } else {
+        n3q = 1;
+        m3q = (inq == 1) ? nq - 1 : 0;
+    }
+    for (int f = 0; f < ORC_GRID_NF; ++f) {
+        REAL sum0 = 0;
+        if (inx == 0) {
+            for (int i3q = 0; i3q < n3q; ++i3q) {
+                const double *p = g->val + ((size_t)(m3q + i3q) * nx + m3x) * ORC_GRID_NF + f;
+                sum0 += wq[i3q] * (wx[0] * (REAL)p[0] + wx[1] * (REAL)p[ORC_GRID_NF] + wx[2] * (REAL)p[2 * ORC_GRID_NF]
+                                   + wx[3] * (REAL)p[3 * ORC_GRID_NF]);
+            }
+        } else if (inx == -1) {
+            for (int i3q = 0; i3q < n3q; ++i3q)
+                sum0 += wq[i3q] * (REAL)g->val[((size_t)(m3q + i3q) * nx) * ORC_GRID_NF + f];
+        }
+        out[f] = sum0;
+    }
+}
+
+/* Parton densities of one beam at x (and Q2 = the factorization scale for grids), indexed
+ * id + nQuarkIn for quarks / antiquarks, 2 nQuarkIn + 1 for the gluon.  Grid values are clipped
+ * at 0 as PDF::xf does (PartonDistributions.cc:122-133, 178-180). */
+static void FN(fill_xf)(const orc_hadcfg *c, REAL x, REAL Q2, REAL *xf)
 {
     int nQ = c->nQuarkIn;
+    if (c->pdfSource == 1) {
+        REAL v[ORC_GRID_NF];
+        FN(grid_xf)(c->grid, x, Q2, v);
+        for (int id = -nQ; id <= nQ; ++id) {
+            if (id == 0) { xf[nQ] = 0; continue; }
+            REAL t = v[id > 0 ? id : 5 - id];
+            xf[id + nQ] = t > 0 ? t : 0;
+        }
+        xf[2 * nQ + 1] = v[0] > 0 ? v[0] : 0;
+        return;
+    }
     for (int id = -nQ; id <= nQ; ++id) xf[id + nQ] = (id == 0) ? 0 : FN(xf)(&c->pdf, id, x);
     xf[2 * nQ + 1] = FN(xf)(&c->pdf, 21, x);
 }
@@ -179,9 +266,10 @@ static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[6], R
     REAL Q2Ren = (REAL)c->renormFac * (REAL)c->renormFac * pT2;
     REAL alpS = FN(alphas)(&c->as, Q2Ren);
 
+    REAL Q2Fac = (REAL)c->factFac * (REAL)c->factFac * pT2;
     REAL xfA[16], xfB[16];
-    FN(fill_xf)(c, x1, xfA);
-    FN(fill_xf)(c, x2, xfB);
+    FN(fill_xf)(c, x1, Q2Fac, xfA);
+    FN(fill_xf)(c, x2, Q2Fac, xfB);
 
     REAL jac = wtTau * wtY * wtZ;
     REAL total = 0;

@@ -158,7 +158,12 @@ void orc_hadcfg_default(orc_hadcfg *c)
     orc_toypdf_init(&c->pdf, g, v, s, c->nQuarkIn);
     c->procMask = 0x1F;                       /* 111..115 */
     c->kfactor = 1.0;
+    c->pdfSource = 0;
+    c->factFac = 1.0;
+    c->grid = NULL;
 }
+
+void orc_lhagrid_xf(const orc_lhagrid *g, double x, double Q2, double out[12]) { grid_xf_d(g, x, Q2, out); }
 
 static double had_point_out(double total, const double *w, const double *kin, double wout[6], orc_kin *k)
 {
@@ -247,8 +252,10 @@ double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, do
                         double y = 0.5 * (ya + yb) + 0.5 * (yb - ya) * gx[gj];
                         double wy = 0.5 * (yb - ya) * gw[gj];
                         double x1 = sqrt(tau) * exp(y), x2 = sqrt(tau) * exp(-y);
-                        fill_xf_d(c, x1, xfA);
-                        fill_xf_d(c, x2, xfB);
+                        if (c->pdfSource != 1) {                  /* toy PDFs: no scale dependence */
+                            fill_xf_d(c, x1, 0.0, xfA);
+                            fill_xf_d(c, x2, 0.0, xfB);
+                        }
                         for (int ir = 0; ir < n_rho; ++ir) {
                             double ra = -rhoM + 2.0 * rhoM * ir / n_rho, rb = -rhoM + 2.0 * rhoM * (ir + 1) / n_rho;
                             for (int gk = 0; gk < GL_ORDER; ++gk) {
@@ -261,6 +268,11 @@ double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, do
                                 double tH = -sH * a, uH = -sH * b;
                                 double pT2 = sH * a * b;
                                 double alpS = alphas_d(&c->as, c->renormFac * c->renormFac * pT2);
+                                if (c->pdfSource == 1) {
+                                    double Q2Fac = c->factFac * c->factFac * pT2;
+                                    fill_xf_d(c, x1, Q2Fac, xfA);
+                                    fill_xf_d(c, x2, Q2Fac, xfB);
+                                }
                                 double jac = wv * wy * wr * sH * a * b;
                                 for (int p = 0; p < 6; ++p) {
                                     if (!(c->procMask & (1 << p))) continue;

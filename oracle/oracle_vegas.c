@@ -75,8 +75,8 @@ static void tables32(int nbins, const double *edges, float *t)
         }
 }
 
-static float point32(const float *k, int mask, const uint32_t o[4], int nbins, const float *tab, float w[6],
-                     float kin[4], int bins[3])
+static float point32(const float *k, const orc32_grid *g, int mask, const uint32_t o[4], int nbins, const float *tab,
+                     float w[6], float kin[4], int bins[3])
 {
     float u[3], om[3], jd[3];
     const float nbf = (float)nbins;
@@ -98,7 +98,7 @@ static float point32(const float *k, int mask, const uint32_t o[4], int nbins, c
             bins[d] = b;
         }
     }
-    float tot = orc32_had_point_om(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], w, kin);
+    float tot = orc32_had_point_g(k, g, mask, u[0], om[0], u[1], om[1], u[2], om[2], w, kin);
     if (tab) {
         float jac = 1.0f;
         jac = jac * jd[0];
@@ -115,6 +115,9 @@ void orc32_had_points(const orc_hadcfg *c, int nbins, const double *edges, uint6
 {
     float k[ORC32_NCONST];
     orc32_consts_fill(c, k);
+    orc32_grid g32;
+    const orc32_grid *gp = NULL;
+    if (c->pdfSource == 1 && c->grid) { orc32_grid_build(c->grid, c->factFac, &g32); gp = &g32; }
     float *tab = NULL;
     if (edges) { tab = (float *)malloc(sizeof(float) * 9 * nbins); tables32(nbins, edges, tab); }
     uint32_t kk[2] = { (uint32_t)key, (uint32_t)(key >> 32) };
@@ -125,12 +128,13 @@ void orc32_had_points(const orc_hadcfg *c, int nbins, const double *edges, uint6
         orc_philox4x32_10(c4, kk, o);
         float w[6], kin[4];
         int bins[3];
-        float tot = point32(k, c->procMask, o, nbins, tab, w, kin, bins);
+        float tot = point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
         w_out[7 * i] = tot;
         for (int p = 0; p < 6; ++p) w_out[7 * i + 1 + p] = w[p];
         if (kin_out) for (int q = 0; q < 4; ++q) kin_out[4 * i + q] = kin[q];
     }
     free(tab);
+    if (gp) orc32_grid_free(&g32);
 }
 
 uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uint64_t key, uint64_t ctr0,
@@ -138,6 +142,9 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
 {
     float k[ORC32_NCONST];
     orc32_consts_fill(c, k);
+    orc32_grid g32;
+    const orc32_grid *gp = NULL;
+    if (c->pdfSource == 1 && c->grid) { orc32_grid_build(c->grid, c->factFac, &g32); gp = &g32; }
     float *tab = NULL;
     if (edges) { tab = (float *)malloc(sizeof(float) * 9 * nbins); tables32(nbins, edges, tab); }
     uint32_t kk[2] = { (uint32_t)key, (uint32_t)(key >> 32) };
@@ -149,7 +156,7 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
         orc_philox4x32_10(c4, kk, o);
         float w[6], kin[4];
         int bins[3];
-        float tot = point32(k, c->procMask, o, nbins, tab, w, kin, bins);
+        float tot = point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
         uint32_t c5[4] = { (uint32_t)e, (uint32_t)(e >> 32), 1u, 0u };
         orc_philox4x32_10(c5, kk, a);
         double u = orc_u53(a[0], a[1]);
@@ -170,6 +177,7 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
         acc += (uint64_t)f;
     }
     free(tab);
+    if (gp) orc32_grid_free(&g32);
     return acc;
 }
 
@@ -225,6 +233,9 @@ void orc32_vegas_adapt(const orc_hadcfg *c, int nbins, double alpha, int n_iter,
 {
     float k[ORC32_NCONST];
     orc32_consts_fill(c, k);
+    orc32_grid g32;
+    const orc32_grid *gp = NULL;
+    if (c->pdfSource == 1 && c->grid) { orc32_grid_build(c->grid, c->factFac, &g32); gp = &g32; }
     float *tab = (float *)malloc(sizeof(float) * 9 * nbins);
     double *hist = (double *)malloc(sizeof(double) * 3 * nbins);
     uint32_t kk[2] = { (uint32_t)key, (uint32_t)(key >> 32) };
@@ -238,7 +249,7 @@ void orc32_vegas_adapt(const orc_hadcfg *c, int nbins, double alpha, int n_iter,
             orc_philox4x32_10(c4, kk, o);
             float w[6], kin[4];
             int bins[3];
-            double t = (double)point32(k, c->procMask, o, nbins, tab, w, kin, bins);
+            double t = (double)point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
             sw += t;
             sw2 += t * t;
             for (int d = 0; d < 3; ++d) hist[d * nbins + bins[d]] += t * t;
@@ -249,6 +260,7 @@ void orc32_vegas_adapt(const orc_hadcfg *c, int nbins, double alpha, int n_iter,
     }
     free(tab);
     free(hist);
+    if (gp) orc32_grid_free(&g32);
 }
 
 /* 1-D VEGAS on the partonic gg -> gg integrand (FP64, Philox stream 0, 53-bit uniform). */

@@ -53,8 +53,9 @@ def test_host_only_entry_points():
     import cupythia_amd as mi
     from cupythia_amd import _abi
     lib = _abi.load()
-    assert lib.mi_version() == 1
+    assert lib.mi_version() == _abi.MI_ABI_VERSION == 2
     p = mi.physics_default(1)
+    assert p.pdf_source == mi.MI_PDF_TOY and p.fact_fac == 1.0
     assert p.mode == mi.MI_MODE_HADRONIC_3D and p.proc_mask == 0x1F and p.eCM == 13600.0
     assert p.pTHatMin == 20.0 and p.alphaS_value == 0.118 and p.alphaS_order == 1
     assert tuple(p.pdf_g) == (-0.1, 5.0) and tuple(p.pdf_val) == (0.5, 3.0) and tuple(p.pdf_sea) == (-0.2, 7.0)

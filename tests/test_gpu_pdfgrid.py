@@ -1,0 +1,125 @@
+"""PDF grids on the device (mi_pdf_load / mi_pdf_xf and the integrand with pdf_source =
+MI_PDF_GRID) against the oracle's LHAGrid1 restatement, on grid files written by the test
+(the shapes of the toy PDFs with a Q dependence on an NNPDF-like node set, and a grid with two Q
+subgrids, the second of 3 nodes)."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import cupythia_amd as mi
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import pdfgrid_util as PG  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def grid_files(tmp_path_factory, O):
+    d = tmp_path_factory.mktemp("grids")
+    main = str(d / "toyq.dat")
+    PG.write_lhagrid1(main, PG.nnpdf_like_x_grid(50, 50, 1e-9), [PG.q_grid(1.0, 1.0e4, 50)], PG.toy_fn(O, Qdep=0.05))
+    sub = str(d / "sub.dat")
+    PG.write_lhagrid1(sub, PG.nnpdf_like_x_grid(30, 20, 1e-7), [PG.q_grid(1.0, 100.0, 12), np.array([100.0, 1e3, 1e4])],
+                      PG.toy_fn(O, Qdep=0.08))
+    return main, sub
+
+
+def _xq_points(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.exp(rng.uniform(math.log(1e-10), 0.0, n))
+    q = np.exp(rng.uniform(math.log(0.5), math.log(2e4), n))
+    x[:8] = [1e-9, 1e-12, 0.1, 0.5, 0.999999, 1.0, 3e-9, 1e-3]      # nodes, below xMin, x = 1
+    q[8:12] = [1.0, 1e4, 0.3, 5e4]                                   # q borders, outside
+    return x, q * q
+
+
+def test_pdf_xf_vs_oracle(ctx, O, grid_files):
+    for path in grid_files:
+        ctx.pdf_load(path)
+        g = O.LhaGrid(path)
+        x, Q2 = _xq_points(4096, 3)
+        dev = ctx.pdf_xf(torch.tensor(x), torch.tensor(Q2)).cpu().numpy()
+        ref = np.array([g.xf(float(a), float(b)) for a, b in zip(x, Q2)])
+        ref[:, 11] = 0.0
+        scale = np.maximum(np.abs(ref), 1e-300)
+        err = np.abs(dev - ref)
+        assert np.all(err <= 1e-12 * scale + 1e-300), (np.max(err / scale), path)
+
+
+def test_grid_points_fp64_vs_long_double(ctx, O, grid_files):
+    main, _ = grid_files
+    ctx.pdf_load(main)
+    g = O.LhaGrid(main)
+    for ff in (1.0, 0.6):
+        phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, fact_fac=ff, proc_mask=0x3F)
+        cfg = O.hadcfg_default(grid=g, factFac=ff, procMask=0x3F)
+        n = 1 << 16
+        w, kin = ctx.points(mi.make_run(phys, n, precision=mi.MI_PREC_FP64, key=42, counter0=99))
+        w = w.cpu().numpy()
+        wld, _ = O.had_points_ld(cfg, 42, 99, n)
+        m = wld > 1e-250
+        rel = np.abs(w[m] - wld[m]) / wld[m]
+        assert rel.max() < 1e-11, rel.max()
+        assert np.all(w[~m] < 1e-240)
+
+
+def test_grid_points_fp32_bit_exact(ctx, O, grid_files):
+    for path in grid_files:
+        ctx.pdf_load(path)
+        g = O.LhaGrid(path)
+        phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, fact_fac=1.3)
+        cfg = O.hadcfg_default(grid=g, factFac=1.3)
+        n = 1 << 17
+        w, kin = ctx.points(mi.make_run(phys, n, precision=mi.MI_PREC_FP32, key=42, counter0=5))
+        w32, k32 = O.had_points32(cfg, 42, 5, n)
+        assert np.array_equal(w.cpu().numpy().astype(np.float32), w32), path
+        assert np.array_equal(kin.cpu().numpy().astype(np.float32), k32)
+
+
+def test_grid_integral_vs_quadrature(ctx, O, grid_files):
+    main, _ = grid_files
+    ctx.pdf_load(main)
+    g = O.LhaGrid(main)
+    q, qp = O.had_quadrature(O.hadcfg_default(grid=g), 12, 12, 24)
+    phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
+    e = ctx.integrate(mi.make_run(phys, 1 << 24, key=42))
+    assert abs(e.sigmaGen() - q) < 3.5 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    for i, code in enumerate((111, 112, 113, 114, 115)):
+        assert abs(e.sigmaGen(code) - qp[i]) < 3.5 * e.sigmaErr(code) + 1e-12, code
+    # the toy run differs (the grid carries a Q dependence)
+    et = ctx.integrate(mi.make_run(mi.physics_default(1), 1 << 24, key=42))
+    assert abs(et.sigmaGen() - e.sigmaGen()) > 5 * e.sigmaErr()
+
+
+def test_grid_unweight_bit_exact(ctx, O, grid_files):
+    main, _ = grid_files
+    ctx.pdf_load(main)
+    g = O.LhaGrid(main)
+    phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
+    cfg = O.hadcfg_default(grid=g)
+    n = 1 << 19
+    ev = ctx.unweight_compact(mi.make_run(phys, n, key=42, counter0=1000))
+    cnt, flags, pid, w = O.unweight32(cfg, 42, 1000, n, ev.wmax, None)
+    assert ev.wmax == float(w.max()) and ev.n == cnt and cnt > 0
+    idx = np.nonzero(flags)[0]
+    assert np.array_equal(ev.index.cpu().numpy(), idx + 1000)
+    assert np.array_equal(ev.pid.cpu().numpy(), pid[idx])
+
+
+def test_grid_errors(ctx, O, grid_files):
+    with mi.Context(0) as fresh:
+        with pytest.raises(mi.MIError) as e:
+            fresh.integrate(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID), 1024))
+        assert e.value.status == mi._abi.MI_ERR_STATE
+        with pytest.raises(mi.MIError):
+            fresh.pdf_load("/nonexistent/file.dat")
+    ctx.pdf_load(grid_files[0])
+    with pytest.raises(mi.MIError):
+        ctx.adjoint(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID), 1 << 16))
+    with pytest.raises(mi.MIError):
+        ctx.integrate(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, fact_fac=0.0), 1024))
