@@ -134,6 +134,33 @@ FLOW_FLOPS_PER_POINT = 2 * 6 * 7360
 PEAK_BF16_TFLOPS = 2500.0         # MI355X dense BF16 matrix peak
 
 
+def write_bench_grid(path):
+    """An lhagrid1 file with smooth proton-like shapes: xf = A x^a (1-x)^b (1 + c ln(Q / 10 GeV)),
+    valence u, d on top of a flavour-symmetric sea, on NNPDF2.3's node layout."""
+    import numpy as np
+    xs = np.concatenate([np.exp(np.linspace(math.log(1e-9), math.log(0.1), 50, endpoint=False)),
+                         np.linspace(0.1, 1.0, 50)])
+    qs = np.exp(np.linspace(0.0, math.log(1e4), 50))
+    ids = [-5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 21]
+    X, Q = np.meshgrid(xs, qs, indexing="ij")
+    L = np.log(Q / 10.0)
+    om = np.clip(1.0 - X, 0.0, None)
+    g = 1.7 * X ** -0.1 * om ** 5 * (1 + 0.08 * L)
+    sea = 0.09 * X ** -0.2 * om ** 7 * (1 + 0.10 * L)
+    uv = 2.2 * X ** 0.5 * om ** 3 * (1 - 0.02 * L)
+    dv = 0.5 * uv
+    cols = {21: g, 1: sea + dv, 2: sea + uv}
+    with open(path, "w") as f:
+        f.write("PdfType: central\nFormat: lhagrid1\n---\n")
+        f.write(" ".join(f"{x:.10e}" for x in xs) + "\n")
+        f.write(" ".join(f"{q:.10e}" for q in qs) + "\n")
+        f.write(" ".join(str(i) for i in ids) + "\n")
+        for ix in range(len(xs)):
+            for iq in range(len(qs)):
+                f.write(" ".join(f"{cols.get(i, sea)[ix, iq]:.10e}" for i in ids) + "\n")
+        f.write("---\n")
+
+
 def workloads(ctx, mi):
     """The other configurations of the path, timed on the device (sum of kernel times):
     VEGAS adaptation 12 x 2^24 + 2^26-candidate unweighting, the adjoint pass at 2^26 points,
@@ -159,6 +186,18 @@ def workloads(ctx, mi):
     out["adjoint"] = {"points": 1 << 26, "kernel_ms": g.kernel_ms, "points_per_s": (1 << 26) / (g.kernel_ms / 1e3),
                       "sigma_mb": g.sigma, "d_alphaS_MZ": g.d_alphaS_MZ, "d_renorm_fac": g.d_renorm_fac,
                       "d_pdf_g_a": g.d_pdf_g_a, "d_pdf_g_b": g.d_pdf_g_b}
+    # parton densities from a PDF grid (LHAPDF6 lhagrid1 file written here: 100 x nodes laid out as
+    # NNPDF2.3's, 50 Q nodes 1..1e4 GeV, smooth shapes with a ln Q slope)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "grid.dat")
+        write_bench_grid(path)
+        ctx.pdf_load(path)
+    gphys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
+    ge = ctx.integrate(mi.make_run(gphys, 1 << 26, key=48))
+    out["pdf_grid"] = {"points": 1 << 26, "kernel_ms": ge.kernel_ms, "points_per_s": (1 << 26) / (ge.kernel_ms / 1e3),
+                       "sigma_mb": ge.sigmaGen(), "sigma_err_mb": ge.sigmaErr(),
+                       "grid": "100 x-nodes (NNPDF2.3 layout) x 50 Q-nodes, 11 flavours, FP32 Lagrange interpolation"}
     # flow on top of the adapted grid
     ctx.flow_init(7)
     steps, batch = 40, 1 << 20
