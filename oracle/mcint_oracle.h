@@ -55,7 +55,10 @@ double   orc_u01(uint64_t x);
 /* --------------------------------------------------------- sigma-hat ---- */
 /* Pythia process codes 111..116 (SigmaQCD.h). */
 enum { ORC_GG2GG = 111, ORC_GG2QQBAR = 112, ORC_QG2QG = 113, ORC_QQ2QQ = 114,
-       ORC_QQBAR2GG = 115, ORC_QQBAR2QQBARNEW = 116 };
+       ORC_QQBAR2GG = 115, ORC_QQBAR2QQBARNEW = 116,
+       ORC_GG2CCBAR = 121, ORC_QQBAR2CCBAR = 122, ORC_GG2BBBAR = 123, ORC_QQBAR2BBBAR = 124 };
+/* Process slots: i = 0..5 <-> codes 111..116 (massless), i = 6..9 <-> 121..124 (c cbar, b bbar). */
+#define ORC_NPROC 10
 /* d(sigmaHat)/d(tHat) in GeV^-2, Pythia's literal expression (sigmaKin + sigmaHat), massless.
    id1/id2 matter only for 114.  nQuarkNew multiplies 112/116.
    Pinned: textbook identities (< 1e-12 per point), t<->u symmetry, cuPythia 08 sigma_cut KATs. */
@@ -63,6 +66,13 @@ double orc_sigma_hat_pythia(int code, double sH, double tH, double uH, double al
                             int id1, int id2, int nQuarkNew);
 /* Independent textbook (ESW / Combridge) brackets; d sigma/dt = pi alpS^2/sH^2 * bracket. */
 double orc_bracket_textbook(int code, double s, double t, double u, int id1, int id2, int nQuarkNew);
+/* Pythia's massive sigmaKin (GeV^-2) of codes 121..124 for m3 = m4 = m (m2 = m^2), and the
+   textbook total partonic cross sections over the full t range (Combridge, Nucl. Phys. B151
+   (1979) 429; Gluck, Owens, Reya, Phys. Rev. D17 (1978) 2324), beta = sqrt(1 - 4 m2 / s):
+   qqbar: 8 pi alpS^2 / (27 s) beta (1 + 2 m2 / s);
+   gg:    pi alpS^2 / (3 s) [(1 + rho + rho^2/16) ln((1+beta)/(1-beta)) - beta (7/4 + 31 rho / 16)], rho = 4 m2 / s. */
+double orc_sigma_hat_massive(int code, double sH, double tH, double uH, double m2, double alpS);
+double orc_sigma_tot_massive_textbook(int code, double sH, double m2, double alpS);
 
 /* ------------------------------------------------------------ alphaS ---- */
 typedef struct {
@@ -123,8 +133,9 @@ typedef struct {
   double renormFac;          /* mu_R = renormFac * pT  (Q2Ren = renormFac^2 * pT2) */
   orc_toypdf pdf;
   int nQuarkIn, nQuarkNew;
-  int procMask;              /* bit i <-> code 111+i */
+  int procMask;              /* bit i <-> code 111+i (i < 6), 121+(i-6) (i = 6..9) */
   double kfactor;
+  double mQ[2];              /* c and b masses of the heavy-quark pair codes 121..124 */
   int pdfSource;             /* 0: toy PDFs (pdf); 1: the grid below */
   double factFac;            /* mu_F = factFac * pT  (Q2Fac = factFac^2 * pT2) */
   const orc_lhagrid *grid;
@@ -138,14 +149,14 @@ typedef struct { double sH, tH, uH, x1, x2, pT2, alpS, jac; } orc_kin;
    w[0..5] = codes 111..116 (0 where masked out); returns the sum.
    If literalU is non-zero, uHat is recomputed as -(sH + tH) like store2Kin does
    (SigmaProcess.cc:877) instead of keeping PhaseSpace's stable value. */
-double orc_had_point(const orc_hadcfg *c, const double u[3], double w[6], orc_kin *k, int literalU);
+double orc_had_point(const orc_hadcfg *c, const double u[3], double w[ORC_NPROC], orc_kin *k, int literalU);
 /* The same steps in 80-bit long double (rounding ~1e-19): the per-point reference for the FP64
    kernels, free of the ~1e-10 cancellation that FP64 Pythia arithmetic shows near the pT cut. */
-double orc_had_point_ld(const orc_hadcfg *c, const double u[3], double w[6], orc_kin *k, int literalU);
+double orc_had_point_ld(const orc_hadcfg *c, const double u[3], double w[ORC_NPROC], orc_kin *k, int literalU);
 
 /* Deterministic quadrature of sigma_p over the same cut (Gauss-Legendre, variables
    (ln tau, y, rho = ln(-t/-u))).  out[0..5] per code, returns the sum. */
-double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, double out[6]);
+double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, double out[ORC_NPROC]);
 
 /* -------------------------------------------------- partonic gg->gg ---- */
 /* w(c) = CONV * dsigma/dt(s, t(c), u(c), alpS) * (2 cMax) * (s/2); stable t = -s(1-c)/2, u = -s(1+c)/2 */
@@ -176,14 +187,15 @@ float orc_logf(float x);
 /* the 128-entry table of orc_logf (and of the device log): G_j, L_j */
 void orc32_logtab(float G[128], float L[128]);
 /* constant block shared by host and device (see orc32_consts_fill for the layout). */
-#define ORC32_NCONST 32
+#define ORC32_NCONST 42
 void  orc32_consts_fill(const orc_hadcfg *c, float k[ORC32_NCONST]);
-/* One point from three 24-bit uniforms; w[0..5] per code; returns total (FP32). */
+/* One point from three 24-bit uniforms; w[0..9] per code (111..116, 121..124); returns total
+   (FP32); kin = {sHat, tHat, x1, x2} of the first selected family. */
 float orc32_had_point(const float k[ORC32_NCONST], int procMask, float u0, float u1, float u2,
-                      float w[6], float kin[4]);
+                      float w[ORC_NPROC], float kin[4]);
 /* Same with the complements 1 - u_i given explicitly (as the VEGAS map provides them). */
 float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, float om0, float u1,
-                         float om1, float u2, float om2, float w[6], float kin[4]);
+                         float om1, float u2, float om2, float w[ORC_NPROC], float kin[4]);
 /* FP32 tables of a PDF grid exactly as the library builds them for its FP32 kernels: nodes,
    logs and values rounded from FP64, reciprocals of the Lagrange denominators computed in FP64
    and rounded (12 per stencil start; [12 m] = 1 / (lnq[m+1] - lnq[m]) on 2-3 point Q subgrids). */
@@ -198,12 +210,12 @@ void orc32_grid_build(const orc_lhagrid *g, double factFac, orc32_grid *out);
 void orc32_grid_free(orc32_grid *g);
 /* orc32_had_point_om with the parton densities from the FP32 grid g (NULL: the toy PDFs). */
 float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int procMask, float u0, float om0,
-                        float u1, float om1, float u2, float om2, float w[6], float kin[4]);
+                        float u1, float om1, float u2, float om2, float w[ORC_NPROC], float kin[4]);
 
 /* ------------------------------------------------ batched points, VEGAS, unweighting ---- */
 /* Events ctr0 .. ctr0+n-1 (Philox stream 0, three 24-bit uniforms) through the flat map
-   (edges == NULL) or a 3 x (nbins + 1) grid; w_out is n x 7 (total, codes 111..116), Jacobian
-   applied; kin_out (may be NULL) n x 4 (sHat, tHat, x1, x2).
+   (edges == NULL) or a 3 x (nbins + 1) grid; w_out is n x 11 (total, codes 111..116, 121..124),
+   Jacobian applied; kin_out (may be NULL) n x 4 (sHat, tHat, x1, x2).
    _ld: the Pythia-form point in long double with the FP64 map of the device.
    32 : the FP32 device arithmetic, bit for bit. */
 void orc_had_points_ld(const orc_hadcfg *c, int nbins, const double *edges, uint64_t key, uint64_t ctr0,

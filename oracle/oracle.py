@@ -18,7 +18,10 @@ _lib = None
 
 CONVERT2MB = 0.389380
 CONV_CUPYTHIA = 0.3893793721
-NCONST32 = 32
+NCONST32 = 42
+NPROC = 10
+# process code of each weight slot (bit i of procMask)
+CODES = (111, 112, 113, 114, 115, 116, 121, 122, 123, 124)
 
 
 class OrcAlphas(C.Structure):
@@ -48,7 +51,7 @@ class OrcHadCfg(C.Structure):
     _fields_ = [("eCM", C.c_double), ("pTHatMin", C.c_double), ("as_", OrcAlphas),
                 ("renormFac", C.c_double), ("pdf", OrcToyPdf), ("nQuarkIn", C.c_int),
                 ("nQuarkNew", C.c_int), ("procMask", C.c_int), ("kfactor", C.c_double),
-                ("pdfSource", C.c_int), ("factFac", C.c_double), ("grid", C.POINTER(OrcLhaGrid))]
+                ("mQ", C.c_double * 2), ("pdfSource", C.c_int), ("factFac", C.c_double), ("grid", C.POINTER(OrcLhaGrid))]
 
 
 class LhaGrid:
@@ -158,6 +161,8 @@ def _bind_extras(L):
         "orc_vegas1d_compat": ([i, i, i, u64, i, d, d, d, d, P(d)], None),
         "orc_adjoint": ([P(OrcHadCfg), u64, u64, u64, P(d)], None),
         "orc_had_mc": ([P(OrcHadCfg), u64, u64, u64, i, P(d)], None),
+        "orc_sigma_hat_massive": ([i, d, d, d, d, d], d),
+        "orc_sigma_tot_massive_textbook": ([i, d, d, d], d),
     }
     for name, (argt, rest) in specs.items():
         if hasattr(L, name):
@@ -214,7 +219,7 @@ def hadcfg_default(**kw) -> OrcHadCfg:
 
 def had_point(cfg: OrcHadCfg, u, literal_u: bool = False):
     uu = (C.c_double * 3)(*u)
-    w = (C.c_double * 6)()
+    w = (C.c_double * NPROC)()
     k = OrcKin()
     tot = lib().orc_had_point(C.byref(cfg), uu, w, C.byref(k), int(literal_u))
     return tot, np.array(list(w)), k
@@ -227,14 +232,14 @@ def consts32(cfg: OrcHadCfg) -> np.ndarray:
 
 
 def had_point32(k: np.ndarray, mask: int, u0: float, u1: float, u2: float):
-    w = (C.c_float * 6)()
+    w = (C.c_float * NPROC)()
     kin = (C.c_float * 4)()
     tot = lib().orc32_had_point(k.ctypes.data_as(C.POINTER(C.c_float)), mask, u0, u1, u2, w, kin)
     return np.float32(tot), np.array(list(w), dtype=np.float32), np.array(list(kin), dtype=np.float32)
 
 
 def had_quadrature(cfg: OrcHadCfg, n_tau=16, n_y=16, n_rho=32):
-    out = (C.c_double * 6)()
+    out = (C.c_double * NPROC)()
     tot = lib().orc_had_quadrature(C.byref(cfg), n_tau, n_y, n_rho, out)
     return tot, np.array(list(out))
 
@@ -251,17 +256,18 @@ def _edges_ptr(edges):
 
 
 def had_points_ld(cfg, key, ctr0, n, edges=None):
-    """Per-point weights (n x 7) and kinematics (n x 4) from the long-double Pythia-form oracle."""
+    """Per-point weights (n x 11: total, then the codes of CODES) and kinematics (n x 4) from the
+    long-double Pythia-form oracle."""
     nb, keep, ep = _edges_ptr(edges)
-    w = np.zeros((n, 7)); kin = np.zeros((n, 4))
+    w = np.zeros((n, NPROC + 1)); kin = np.zeros((n, 4))
     lib().orc_had_points_ld(C.byref(cfg), nb, ep, key, ctr0, n, _dp(w), _dp(kin))
     return w, kin
 
 
 def had_points32(cfg, key, ctr0, n, edges=None):
-    """Per-point FP32 weights (n x 7) and kinematics (n x 4), the device arithmetic bit for bit."""
+    """Per-point FP32 weights (n x 11) and kinematics (n x 4), the device arithmetic bit for bit."""
     nb, keep, ep = _edges_ptr(edges)
-    w = np.zeros((n, 7), dtype=np.float32); kin = np.zeros((n, 4), dtype=np.float32)
+    w = np.zeros((n, NPROC + 1), dtype=np.float32); kin = np.zeros((n, 4), dtype=np.float32)
     lib().orc32_had_points(C.byref(cfg), nb, ep, key, ctr0, n, _fp(w), _fp(kin))
     return w, kin
 

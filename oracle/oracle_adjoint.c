@@ -66,7 +66,7 @@ void orc_adjoint(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, d
             orc_philox4x32_10(c4, kk, o);
             double u[3] = { (double)(o[0] >> 8) * (1.0 / 16777216.0), (double)(o[1] >> 8) * (1.0 / 16777216.0),
                             (double)(o[2] >> 8) * (1.0 / 16777216.0) };
-            double w[6];
+            double w[ORC_NPROC];
             double w0 = orc_had_point_ld(c, u, w, NULL, 0);
             loc[0] += w0;
             for (int q = 0; q < 4; ++q) {

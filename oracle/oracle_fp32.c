@@ -97,6 +97,9 @@ float orc_logf(float x)
  * 11 lnL6s 12 lnL5s 13 lnL4s 14 lnL3s                     15 c6 16 c5 17 c4 18 c3 (12 pi / b0)
  * 19 lnAg 20 ag 21 bg        22 lnAs 23 as 24 bs          25 lnAuv 26 av 27 bv 28 rdu = Adv/Auv
  * 29 nSea = 2 nQuarkIn       30 nQuarkNew                 31 unused
+ * 32..36 the c cbar family: LT, 4 mT^2 (mT^2 = m_c^2 + pTmin^2), pref with 4 mT^2, m_c^2,
+ *        m_c^2 / (4 mT^2)
+ * 37..41 the b bbar family: the same with m_b
  */
 void orc32_consts_fill(const orc_hadcfg *c, float k[ORC32_NCONST])
 {
@@ -125,11 +128,21 @@ void orc32_consts_fill(const orc_hadcfg *c, float k[ORC32_NCONST])
     k[28] = (float)(c->pdf.Adv / c->pdf.Auv);
     k[29] = (float)(2 * c->nQuarkIn);
     k[30] = (float)c->nQuarkNew;
+    for (int f = 0; f < 2; ++f) {
+        double m2 = c->mQ[f] * c->mQ[f];
+        double fourMT2 = 4.0 * (m2 + c->pTHatMin * c->pTHatMin);
+        double LTf = log(s / fourMT2);
+        k[32 + 5 * f] = (float)LTf;
+        k[33 + 5 * f] = (float)fourMT2;
+        k[34 + 5 * f] = (float)(ORC_CONVERT2MB * M_PI * LTf * c->kfactor / fourMT2);
+        k[35 + 5 * f] = (float)m2;
+        k[36 + 5 * f] = (float)(m2 / fourMT2);
+    }
 }
 
 /* The device kernel's per-point arithmetic, restated operation by operation. */
 float orc32_had_point(const float k[ORC32_NCONST], int procMask, float u0, float u1, float u2,
-                      float w[6], float kin[4])
+                      float w[ORC_NPROC], float kin[4])
 {
     return orc32_had_point_om(k, procMask, u0, 1.0f - u0, u1, 1.0f - u1, u2, 1.0f - u2, w, kin);
 }
@@ -278,15 +291,20 @@ static void grid_lumi32(const orc32_grid *g, int nQ, float x1, float lx1, float 
 }
 
 float orc32_had_point_om(const float k[ORC32_NCONST], int procMask, float u0, float om0, float u1,
-                         float om1, float u2, float om2, float w[6], float kin[4])
+                         float om1, float u2, float om2, float w[ORC_NPROC], float kin[4])
 {
     return orc32_had_point_g(k, NULL, procMask, u0, om0, u1, om1, u2, om2, w, kin);
 }
 
-float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int procMask, float u0, float om0,
-                        float u1, float om1, float u2, float om2, float w[6], float kin[4])
+/* One family: 0 = the massless codes 111..116, 1 = c cbar (121, 122), 2 = b bbar (123, 124).
+ * The massive map is the massless one with pTmin^2 replaced by mT^2 = m^2 + pTmin^2 (a, b and
+ * the Jacobian are unchanged), Q^2 = sH a b for both scales, tH = m^2 - sH a, mu = m^2 / sH. */
+static float family32(const float *k, const orc32_grid *g, int fam, int procMask, float u0, float om0,
+                      float u1, float om1, float u2, float om2, float w[ORC_NPROC], float kin[4])
 {
-    const float LT = k[0];
+    const float *kf = fam == 0 ? k : k + 27 + 5 * fam;
+    const float LT = kf[0];
+    const float m2 = fam == 0 ? 0.0f : kf[3];
     float r = orc_expf(-(u0 * LT));                    /* tauMin / tau */
     float yMax = 0.5f * (LT * om0);
     /* ln x1 = y - yMax = -2 yMax (1 - u1), ln x2 = -y - yMax = -2 yMax u1: no cancellation */
@@ -298,13 +316,13 @@ float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int pr
     float opz = 1.0f + zMax;
     float R1 = 1.0f / (r * opz);
     float omz = (r * r) * R1;                           /* r/(1 + zMax) = 1 - zMax, cancellation-free */
-    float sH = (k[1] * opz) * R1;                       /* 4 pTmin^2 / r */
+    float sH = (kf[1] * opz) * R1;                      /* 4 mT^2 / r */
     float twoz = zMax + zMax;
     float A = fmaf(twoz, om2, omz);                     /* 1 - z */
     float B = fmaf(twoz, u2, omz);                      /* 1 + z */
-    float a = 0.5f * A, b = 0.5f * B;                   /* -t/s, -u/s */
+    float a = 0.5f * A, b = 0.5f * B;                   /* -(tH - m^2)/s, -(uH - m^2)/s */
     float ab = a * b;
-    float pT2 = sH * ab;
+    float pT2 = sH * ab;                                /* pT^2 + m^2 */
 
     float alpS, inv_ab;
     if (k[4] == 0.0f) {
@@ -355,28 +373,62 @@ float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int pr
 
     /* brackets: d sigma / dt = pi alpS^2 / sH^2 * bracket */
     float q2 = fmaf(a, a, b * b);
-    float common = k[2] * (alpS * alpS) * (LT * om0) * zMax * r;
-    float ws[6];
-    ws[0] = Lgg * (2.25f * fmaf(b * ia, ia, fmaf(a * ib, ib, 3.0f - ab)));
-    ws[1] = Lgg * (k[30] * q2 * fmaf(inv_ab, 0.16666666666666667f, -0.375f));
-    ws[2] = Lqg * (fmaf(b, b, 1.0f) * fmaf(ia, ia, 0.44444444444444444f * ib));
-    {
-        float sigT = 0.44444444444444444f * fmaf(b, b, 1.0f) * (ia * ia);
-        float sigU = 0.44444444444444444f * fmaf(a, a, 1.0f) * (ib * ib);
-        float sigTU = -0.29629629629629630f * inv_ab;
-        float sigST = 0.29629629629629630f * (b * b) * ia;
-        float csame = 0.5f * (sigU + sigTU - sigT);
-        ws[3] = fmaf(sigT, LQQ, fmaf(csame, Lsame, sigST * Lopp));
+    float common = kf[2] * (alpS * alpS) * (LT * om0) * zMax * r;
+    float ws[ORC_NPROC];
+    int lo, hi;
+    if (fam == 0) {
+        lo = 0; hi = 6;
+        ws[0] = Lgg * (2.25f * fmaf(b * ia, ia, fmaf(a * ib, ib, 3.0f - ab)));
+        ws[1] = Lgg * (k[30] * q2 * fmaf(inv_ab, 0.16666666666666667f, -0.375f));
+        ws[2] = Lqg * (fmaf(b, b, 1.0f) * fmaf(ia, ia, 0.44444444444444444f * ib));
+        {
+            float sigT = 0.44444444444444444f * fmaf(b, b, 1.0f) * (ia * ia);
+            float sigU = 0.44444444444444444f * fmaf(a, a, 1.0f) * (ib * ib);
+            float sigTU = -0.29629629629629630f * inv_ab;
+            float sigST = 0.29629629629629630f * (b * b) * ia;
+            float csame = 0.5f * (sigU + sigTU - sigT);
+            ws[3] = fmaf(sigT, LQQ, fmaf(csame, Lsame, sigST * Lopp));
+        }
+        ws[4] = Lopp * (q2 * fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
+        ws[5] = Lopp * (k[30] * 0.44444444444444444f * q2);
+    } else {
+        /* Pythia's massive forms (SigmaQCD.cc Sigma2gg2QQbar::sigmaKin, Sigma2qqbar2QQbar::
+         * sigmaKin) with tHQ = -sH a, uHQ = -sH b and mu = m^2 / sH = r m^2 / (4 mT^2):
+         * gg:   [b/a - 9/4 b^2 + mu/a^2 (a (9/2 b - 1/2 + mu) - 4 mu)  +  (a <-> b)] / 6
+         * qqb:  4/9 (a^2 + b^2 + 2 mu) */
+        lo = 4 + 2 * fam; hi = lo + 2;
+        float mu = kf[4] * r;
+        float cT = fmaf(a, fmaf(4.5f, b, -0.5f) + mu, -4.0f * mu);
+        float cU = fmaf(b, fmaf(4.5f, a, -0.5f) + mu, -4.0f * mu);
+        float sTS = fmaf(b, ia, fmaf(-2.25f * b, b, (mu * (ia * ia)) * cT));
+        float sUS = fmaf(a, ib, fmaf(-2.25f * a, a, (mu * (ib * ib)) * cU));
+        ws[lo] = Lgg * (0.16666666666666667f * (sTS + sUS));
+        ws[lo + 1] = Lopp * (0.44444444444444444f * fmaf(2.0f, mu, q2));
     }
-    ws[4] = Lopp * (q2 * fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
-    ws[5] = Lopp * (k[30] * 0.44444444444444444f * q2);
     float tot = 0.0f;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = lo; i < hi; ++i) {
         float wi = (procMask & (1 << i)) ? common * ws[i] : 0.0f;
         if (!(wi > 0.0f)) wi = 0.0f;
         w[i] = wi;
         tot += wi;
     }
-    if (kin) { kin[0] = sH; kin[1] = -(sH * a); kin[2] = x1; kin[3] = x2; }
+    if (kin) { kin[0] = sH; kin[1] = fmaf(-sH, a, m2); kin[2] = x1; kin[3] = x2; }
+    return tot;
+}
+
+float orc32_had_point_g(const float k[ORC32_NCONST], const orc32_grid *g, int procMask, float u0, float om0,
+                        float u1, float om1, float u2, float om2, float w[ORC_NPROC], float kin[4])
+{
+    static const int famMask[3] = { 0x3F, 0xC0, 0x300 };
+    for (int i = 0; i < ORC_NPROC; ++i) w[i] = 0.0f;
+    float tot = 0.0f;
+    int have_kin = 0;
+    for (int fam = 0; fam < 3; ++fam) {
+        if (!(procMask & famMask[fam])) continue;
+        float kf[4];
+        tot += family32(k, g, fam, procMask, u0, om0, u1, om1, u2, om2, w, kf);
+        if (kin && !have_kin) { for (int q = 0; q < 4; ++q) kin[q] = kf[q]; have_kin = 1; }
+    }
+    if (kin && !have_kin) for (int q = 0; q < 4; ++q) kin[q] = 0.0f;
     return tot;
 }

@@ -216,17 +216,56 @@ static REAL FN(sigma_pdf)(const orc_hadcfg *c, int code, REAL sH, REAL tH, REAL 
     return sum;
 }
 
-/* One point: selectTau (PhaseSpace.cc:1256-1337, iTau = 0), y flat in [-yMax, yMax]
- * (limitY :1183-1200, wtY = 2 yMax), limitZ/selectZ (:1205-1250, :1453-1568, iZ = 0 with its
- * neg/pos split), store2Kin (SigmaProcess.cc:846-925: massless pT2 = tH uH / sH,
- * Q2Ren = renormMultFac pT2), weight = sigmaPDF * wtTau * wtY * wtZ (PhaseSpace.cc:1074). */
-static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[6], REAL kin[8], int literalU)
+/* Massive heavy-quark pair production, m3 = m4 = m (s3 = s4 = m2), Pythia's sigmaKin:
+ *   121/123 Sigma2gg2QQbar::sigmaKin       SigmaQCD.cc:736-766 (openFracPair = 1, no top model)
+ *   122/124 Sigma2qqbar2QQbar::sigmaKin    SigmaQCD.cc:869-891
+ * with s34Avg = m2, tHQ = -(sH - tH + uH)/2, uHQ = -(sH + tH - uH)/2. */
+static REAL FN(sigma_hat_massive)(int code, REAL sH, REAL tH, REAL uH, REAL m2, REAL alpS)
 {
-    for (int i = 0; i < 6; ++i) w[i] = 0;
+    const REAL pi = (REAL)3.14159265358979323846264338327950288L;
+    REAL s3 = m2, s4 = m2;
+    REAL sH2 = sH * sH;
+    REAL s34Avg = (REAL)0.5 * (s3 + s4) - (REAL)0.25 * (s3 - s4) * (s3 - s4) / sH;
+    REAL tHQ = -(REAL)0.5 * (sH - tH + uH);
+    REAL uHQ = -(REAL)0.5 * (sH + tH - uH);
+    REAL tHQ2 = tHQ * tHQ, uHQ2 = uHQ * uHQ;
+    if (code == ORC_GG2CCBAR || code == ORC_GG2BBBAR) {
+        REAL tumHQ = tHQ * uHQ - s34Avg * sH;
+        REAL sigTS = (uHQ / tHQ - (REAL)2.25 * uHQ2 / sH2 + (REAL)4.5 * s34Avg * tumHQ / (sH * tHQ2)
+                      + (REAL)0.5 * s34Avg * (tHQ + s34Avg) / tHQ2 - s34Avg * s34Avg / (sH * tHQ)) / 6;
+        REAL sigUS = (tHQ / uHQ - (REAL)2.25 * tHQ2 / sH2 + (REAL)4.5 * s34Avg * tumHQ / (sH * uHQ2)
+                      + (REAL)0.5 * s34Avg * (uHQ + s34Avg) / uHQ2 - s34Avg * s34Avg / (sH * uHQ)) / 6;
+        return (pi / sH2) * (alpS * alpS) * (sigTS + sigUS);
+    }
+    REAL sigS = ((REAL)4 / 9) * ((tHQ2 + uHQ2) / sH2 + 2 * s34Avg / sH);
+    return (pi / sH2) * (alpS * alpS) * sigS;
+}
+
+/* One process family (a set of processes sharing the final-state masses and hence the phase
+ * space): family 0 = the massless codes 111..116, 1 = c cbar (121, 122), 2 = b bbar (123, 124).
+ * Each process of Pythia has its own PhaseSpace; the family's processes share it, evaluated at
+ * the same uniforms u.  Steps: selectTau (PhaseSpace.cc:1256-1337, iTau = 0, tauMin =
+ * (mT3Min + mT4Min)^2 / s with mT = sqrt(m^2 + pT2Min), :1166-1172), y flat in [-yMax, yMax]
+ * (limitY :1183-1200, wtY = 2 yMax), p2Abs = ((sH - s3 - s4)^2 - 4 s3 s4) / (4 sH) (:1263-1266),
+ * limitZ/selectZ (:1205-1250, :1453-1568, iZ = 0 with its neg/pos split, tHat uHat from the
+ * stable product incl. s3 s4 z^2), store2Kin (SigmaProcess.cc:846-925: pT2 = (tH uH - s3 s4)/sH,
+ * massless Q2 = pT2, massive Q2Ren = sqrt((pT2 + s3)(pT2 + s4)) [renormScale2 = 2] and
+ * Q2Fac = pT2 + min(s3, s4) [factorScale2 = 1]), weight = sigmaPDF * wtTau * wtY * wtZ
+ * (PhaseSpace.cc:1074).  kin[] of the family. */
+static REAL FN(had_family)(const orc_hadcfg *c, int fam, const double uin[3], REAL w[ORC_NPROC], REAL kin[8],
+                           int literalU)
+{
+    REAL m = fam == 0 ? 0 : (REAL)c->mQ[fam - 1];
+    REAL s3 = m * m, s4 = m * m;
     REAL u0 = (REAL)uin[0], u1 = (REAL)uin[1], u2 = (REAL)uin[2];
     REAL s = (REAL)c->eCM * (REAL)c->eCM;
     REAL pT2Min = (REAL)c->pTHatMin * (REAL)c->pTHatMin;
-    REAL tauMin = 4 * pT2Min / s;                       /* (mT3Min + mT4Min)^2 / s, massless */
+    REAL tauMin;
+    if (fam == 0) tauMin = 4 * pT2Min / s;                  /* (mT3Min + mT4Min)^2 / s, massless */
+    else {
+        REAL mT3Min = M_SQRT(s3 + pT2Min), mT4Min = M_SQRT(s4 + pT2Min);
+        tauMin = (mT3Min + mT4Min) * (mT3Min + mT4Min) / s;
+    }
     REAL tauMax = 1;
     REAL intTau0 = M_LOG(tauMax / tauMin);
     REAL tau = tauMin * M_POW(tauMax / tauMin, u0);
@@ -240,7 +279,7 @@ static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[6], R
 
     REAL sH = tau * s;
     REAL mHat = M_SQRT(sH);
-    REAL p2Abs = (REAL)0.25 * sH * sH / sH;
+    REAL p2Abs = fam == 0 ? (REAL)0.25 * sH * sH / sH : (REAL)0.25 * ((sH - s3 - s4) * (sH - s3 - s4) - 4 * s3 * s4) / sH;
     REAL pAbs = M_SQRT(p2Abs);
     REAL zArg = 1 - pT2Min / p2Abs;
     REAL zMax = zArg > 0 ? M_SQRT(zArg) : 0;
@@ -255,31 +294,69 @@ static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[6], R
     else       z = M_FMIN(zPosMax, M_FMAX(zPosMin, z));
     REAL wtZ = mHat * pAbs * area0;                    /* zCoef[0] = 1 */
 
-    REAL sH34 = -(REAL)0.5 * sH;
-    REAL tHuH = sH34 * sH34 * (1 - z) * (1 + z);
+    REAL sH34 = -(REAL)0.5 * (sH - s3 - s4);
+    REAL tHuH = sH34 * sH34 * (1 - z) * (1 + z) + s3 * s4 * z * z;
     REAL tH, uH;
     if (z < 0) { tH = sH34 + mHat * pAbs * z; uH = tHuH / tH; }
     else       { uH = sH34 - mHat * pAbs * z; tH = tHuH / uH; }
-    if (literalU) uH = -(sH + tH);
+    if (literalU) uH = s3 + s4 - (sH + tH);
 
-    REAL pT2 = tH * uH / sH;
-    REAL Q2Ren = (REAL)c->renormFac * (REAL)c->renormFac * pT2;
+    REAL pT2 = (tH * uH - s3 * s4) / sH;
+    REAL Q2Ren = fam == 0 ? pT2 : M_SQRT((pT2 + s3) * (pT2 + s4));
+    REAL Q2Fac = fam == 0 ? pT2 : pT2 + (s3 < s4 ? s3 : s4);
+    Q2Ren *= (REAL)c->renormFac * (REAL)c->renormFac;
+    Q2Fac *= (REAL)c->factFac * (REAL)c->factFac;
     REAL alpS = FN(alphas)(&c->as, Q2Ren);
 
-    REAL Q2Fac = (REAL)c->factFac * (REAL)c->factFac * pT2;
     REAL xfA[16], xfB[16];
     FN(fill_xf)(c, x1, Q2Fac, xfA);
     FN(fill_xf)(c, x2, Q2Fac, xfB);
 
     REAL jac = wtTau * wtY * wtZ;
     REAL total = 0;
-    for (int i = 0; i < 6; ++i) {
+    int nQ = c->nQuarkIn, ig = 2 * nQ + 1;
+    REAL K = (REAL)c->kfactor * (REAL)ORC_CONVERT2MB;
+    for (int i = 0; i < ORC_NPROC; ++i) {
         if (!(c->procMask & (1 << i))) continue;
-        REAL wi = FN(sigma_pdf)(c, 111 + i, sH, tH, uH, alpS, xfA, xfB) * jac;
+        int code = i < 6 ? 111 + i : 121 + (i - 6);
+        int f = i < 6 ? 0 : (i < 8 ? 1 : 2);
+        if (f != fam) continue;
+        REAL wi;
+        if (fam == 0) {
+            wi = FN(sigma_pdf)(c, code, sH, tH, uH, alpS, xfA, xfB) * jac;
+        } else if (code == ORC_GG2CCBAR || code == ORC_GG2BBBAR) {           /* "gg" */
+            wi = K * FN(sigma_hat_massive)(code, sH, tH, uH, s3, alpS) * xfA[ig] * xfB[ig] * jac;
+        } else {                                                            /* "qqbarSame" */
+            REAL sum = 0;
+            REAL sig = K * FN(sigma_hat_massive)(code, sH, tH, uH, s3, alpS);
+            for (int id = -nQ; id <= nQ; ++id) {
+                if (id == 0) continue;
+                sum += sig * xfA[id + nQ] * xfB[-id + nQ];
+            }
+            wi = sum * jac;
+        }
         if (wi < 0) wi = 0;                              /* PhaseSpace.cc:1135 */
         w[i] = wi;
         total += wi;
     }
     kin[0] = sH; kin[1] = tH; kin[2] = uH; kin[3] = x1; kin[4] = x2; kin[5] = pT2; kin[6] = alpS; kin[7] = jac;
+    return total;
+}
+
+/* The point: every family selected by procMask, at the same uniforms; kin[] of the first
+ * selected family (massless, then c cbar, then b bbar). */
+static REAL FN(had_point)(const orc_hadcfg *c, const double uin[3], REAL w[ORC_NPROC], REAL kin[8], int literalU)
+{
+    for (int i = 0; i < ORC_NPROC; ++i) w[i] = 0;
+    REAL total = 0;
+    int have_kin = 0;
+    static const int famMask[3] = { 0x3F, 0xC0, 0x300 };
+    for (int fam = 0; fam < 3; ++fam) {
+        if (!(c->procMask & famMask[fam])) continue;
+        REAL kf[8];
+        total += FN(had_family)(c, fam, uin, w, kf, literalU);
+        if (!have_kin) { for (int q = 0; q < 8; ++q) kin[q] = kf[q]; have_kin = 1; }
+    }
+    if (!have_kin) for (int q = 0; q < 8; ++q) kin[q] = 0;
     return total;
 }

@@ -158,6 +158,8 @@ void orc_hadcfg_default(orc_hadcfg *c)
     orc_toypdf_init(&c->pdf, g, v, s, c->nQuarkIn);
     c->procMask = 0x1F;                       /* 111..115 */
     c->kfactor = 1.0;
+    c->mQ[0] = 1.5;                           /* ParticleData m0 of c and b */
+    c->mQ[1] = 4.8;
     c->pdfSource = 0;
     c->factFac = 1.0;
     c->grid = NULL;
@@ -165,9 +167,9 @@ void orc_hadcfg_default(orc_hadcfg *c)
 
 void orc_lhagrid_xf(const orc_lhagrid *g, double x, double Q2, double out[12]) { grid_xf_d(g, x, Q2, out); }
 
-static double had_point_out(double total, const double *w, const double *kin, double wout[6], orc_kin *k)
+static double had_point_out(double total, const double *w, const double *kin, double wout[ORC_NPROC], orc_kin *k)
 {
-    for (int i = 0; i < 6; ++i) wout[i] = w[i];
+    for (int i = 0; i < ORC_NPROC; ++i) wout[i] = w[i];
     if (k) {
         k->sH = kin[0]; k->tH = kin[1]; k->uH = kin[2]; k->x1 = kin[3]; k->x2 = kin[4];
         k->pT2 = kin[5]; k->alpS = kin[6]; k->jac = kin[7];
@@ -175,19 +177,19 @@ static double had_point_out(double total, const double *w, const double *kin, do
     return total;
 }
 
-double orc_had_point(const orc_hadcfg *c, const double u[3], double w[6], orc_kin *k, int literalU)
+double orc_had_point(const orc_hadcfg *c, const double u[3], double w[ORC_NPROC], orc_kin *k, int literalU)
 {
-    double ww[6], kin[8];
+    double ww[ORC_NPROC], kin[8];
     double tot = had_point_d(c, u, ww, kin, literalU);
     return had_point_out(tot, ww, kin, w, k);
 }
 
-double orc_had_point_ld(const orc_hadcfg *c, const double u[3], double w[6], orc_kin *k, int literalU)
+double orc_had_point_ld(const orc_hadcfg *c, const double u[3], double w[ORC_NPROC], orc_kin *k, int literalU)
 {
-    long double ww[6], kin[8];
+    long double ww[ORC_NPROC], kin[8];
     long double tot = had_point_ld(c, u, ww, kin, literalU);
-    double wd[6], kd[8];
-    for (int i = 0; i < 6; ++i) wd[i] = (double)ww[i];
+    double wd[ORC_NPROC], kd[8];
+    for (int i = 0; i < ORC_NPROC; ++i) wd[i] = (double)ww[i];
     for (int i = 0; i < 8; ++i) kd[i] = (double)kin[i];
     return had_point_out((double)tot, wd, kd, w, k);
 }
@@ -215,21 +217,24 @@ static void gauss_legendre(int n, double *x, double *wt)
 #define GL_ORDER 8
 
 /* Deterministic quadrature of the hadronic sigma over the same cut, in the variables
- * v = ln tau (d tau / tau = dv), y, and rho = ln(a/b) with a = -t/sH = (1-z)/2, b = (1+z)/2
- * (|dt| = sH a b d rho): the integrand is smooth in all three, so composite Gauss-Legendre
- * converges fast.  Independent of the MC map and of the RNG. */
-double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, double out[6])
+ * v = ln tau (d tau / tau = dv), y, and rho = ln(a/b) with a = -tQ/sH, b = -uQ/sH, a + b = 1
+ * (tQ = t - m^2, uQ = u - m^2; |dt| = sH a b d rho): the integrand is smooth in all three, so
+ * composite Gauss-Legendre converges fast.  The pT cut is sH a b >= mT^2 = m^2 + pT2Min, i.e.
+ * |rho| <= ln((1 + zM)/(1 - zM)) with zM = sqrt(1 - 4 mT^2 / sH); per family (massless, c cbar,
+ * b bbar).  Independent of the MC map and of the RNG. */
+static void quad_family(const orc_hadcfg *c, int fam, int n_tau, int n_y, int n_rho, double acc[ORC_NPROC])
 {
     double gx[GL_ORDER], gw[GL_ORDER];
     gauss_legendre(GL_ORDER, gx, gw);
     double s = c->eCM * c->eCM;
-    double pT2Min = c->pTHatMin * c->pTHatMin;
-    double vMin = log(4.0 * pT2Min / s), vMax = 0.0;
-    double acc[6] = { 0, 0, 0, 0, 0, 0 };
-    int nQ = c->nQuarkIn;
+    double m2 = fam == 0 ? 0.0 : c->mQ[fam - 1] * c->mQ[fam - 1];
+    double mT2 = c->pTHatMin * c->pTHatMin + m2;
+    double vMin = log(4.0 * mT2 / s), vMax = 0.0;
+    int nQ = c->nQuarkIn, ig = 2 * nQ + 1;
+    double K = c->kfactor * ORC_CONVERT2MB;
 #pragma omp parallel
     {
-        double loc[6] = { 0, 0, 0, 0, 0, 0 };
+        double loc[ORC_NPROC] = { 0 };
         double xfA[16], xfB[16];
 #pragma omp for schedule(dynamic, 1)
         for (int it = 0; it < n_tau; ++it) {
@@ -242,7 +247,7 @@ double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, do
                 double wv = 0.5 * (lb - la) * gw[gi] * 2.0 * lam;
                 double tau = exp(v), sH = tau * s;
                 double yMax = -0.5 * v;
-                double r = 4.0 * pT2Min / sH;
+                double r = 4.0 * mT2 / sH;
                 if (r >= 1.0) continue;
                 double zMax = sqrt(1.0 - r);
                 double rhoM = log((1.0 + zMax) / (1.0 - zMax));
@@ -265,18 +270,34 @@ double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, do
                                 double a, b;
                                 if (rho > 0) { double e = exp(-rho); a = 1.0 / (1.0 + e); b = e / (1.0 + e); }
                                 else         { double e = exp(rho);  a = e / (1.0 + e);   b = 1.0 / (1.0 + e); }
-                                double tH = -sH * a, uH = -sH * b;
-                                double pT2 = sH * a * b;
-                                double alpS = alphas_d(&c->as, c->renormFac * c->renormFac * pT2);
+                                double tH = -sH * a + m2, uH = -sH * b + m2;
+                                double Q2 = sH * a * b;                   /* pT2 + m^2 (massless: pT2) */
+                                double alpS = alphas_d(&c->as, c->renormFac * c->renormFac * Q2);
                                 if (c->pdfSource == 1) {
-                                    double Q2Fac = c->factFac * c->factFac * pT2;
+                                    double Q2Fac = c->factFac * c->factFac * Q2;
                                     fill_xf_d(c, x1, Q2Fac, xfA);
                                     fill_xf_d(c, x2, Q2Fac, xfB);
                                 }
                                 double jac = wv * wy * wr * sH * a * b;
-                                for (int p = 0; p < 6; ++p) {
+                                for (int p = 0; p < ORC_NPROC; ++p) {
                                     if (!(c->procMask & (1 << p))) continue;
-                                    loc[p] += jac * sigma_pdf_d(c, 111 + p, sH, tH, uH, alpS, xfA, xfB);
+                                    int f = p < 6 ? 0 : (p < 8 ? 1 : 2);
+                                    if (f != fam) continue;
+                                    int code = p < 6 ? 111 + p : 121 + (p - 6);
+                                    double val;
+                                    if (fam == 0) {
+                                        val = sigma_pdf_d(c, code, sH, tH, uH, alpS, xfA, xfB);
+                                    } else {
+                                        double sig = K * orc_sigma_hat_massive(code, sH, tH, uH, m2, alpS);
+                                        if (code == ORC_GG2CCBAR || code == ORC_GG2BBBAR) {
+                                            val = sig * xfA[ig] * xfB[ig];
+                                        } else {
+                                            val = 0.0;
+                                            for (int id = -nQ; id <= nQ; ++id)
+                                                if (id) val += sig * xfA[id + nQ] * xfB[-id + nQ];
+                                        }
+                                    }
+                                    loc[p] += jac * val;
                                 }
                             }
                         }
@@ -285,12 +306,33 @@ double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, do
             }
         }
 #pragma omp critical
-        for (int p = 0; p < 6; ++p) acc[p] += loc[p];
+        for (int p = 0; p < ORC_NPROC; ++p) acc[p] += loc[p];
     }
-    (void)nQ;
+}
+
+double orc_had_quadrature(const orc_hadcfg *c, int n_tau, int n_y, int n_rho, double out[ORC_NPROC])
+{
+    double acc[ORC_NPROC] = { 0 };
+    if (c->procMask & 0x3F) quad_family(c, 0, n_tau, n_y, n_rho, acc);
+    if (c->procMask & 0xC0) quad_family(c, 1, n_tau, n_y, n_rho, acc);
+    if (c->procMask & 0x300) quad_family(c, 2, n_tau, n_y, n_rho, acc);
     double tot = 0.0;
-    for (int p = 0; p < 6; ++p) { out[p] = acc[p]; tot += acc[p]; }
+    for (int p = 0; p < ORC_NPROC; ++p) { out[p] = acc[p]; tot += acc[p]; }
     return tot;
+}
+
+double orc_sigma_hat_massive(int code, double sH, double tH, double uH, double m2, double alpS)
+{
+    return sigma_hat_massive_d(code, sH, tH, uH, m2, alpS);
+}
+
+double orc_sigma_tot_massive_textbook(int code, double sH, double m2, double alpS)
+{
+    double rho = 4.0 * m2 / sH, beta = sqrt(1.0 - rho);
+    if (code == ORC_QQBAR2CCBAR || code == ORC_QQBAR2BBBAR)
+        return 8.0 * M_PI * alpS * alpS / (27.0 * sH) * beta * (1.0 + 0.5 * rho);
+    return M_PI * alpS * alpS / (3.0 * sH) *
+           ((1.0 + rho + rho * rho / 16.0) * log((1.0 + beta) / (1.0 - beta)) - beta * (7.0 / 4.0 + 31.0 * rho / 16.0));
 }
 
 /* ------------------------------------------------------------------------------------

@@ -49,11 +49,11 @@ void orc_had_points_ld(const orc_hadcfg *c, int nbins, const double *edges, uint
             map64(o[d] >> 8, nbins, edges ? edges + (size_t)d * (nbins + 1) : NULL, &u[d], &om[d], &jd);
             jac = jac * jd;
         }
-        double w[6];
+        double w[ORC_NPROC];
         orc_kin k;
         double tot = orc_had_point_ld(c, u, w, &k, 0);
-        w_out[7 * i] = tot * jac;
-        for (int p = 0; p < 6; ++p) w_out[7 * i + 1 + p] = w[p] * jac;
+        w_out[(ORC_NPROC + 1) * i] = tot * jac;
+        for (int p = 0; p < ORC_NPROC; ++p) w_out[(ORC_NPROC + 1) * i + 1 + p] = w[p] * jac;
         if (kin_out) {
             kin_out[4 * i + 0] = k.sH; kin_out[4 * i + 1] = k.tH;
             kin_out[4 * i + 2] = k.x1; kin_out[4 * i + 3] = k.x2;
@@ -76,7 +76,7 @@ static void tables32(int nbins, const double *edges, float *t)
 }
 
 static float point32(const float *k, const orc32_grid *g, int mask, const uint32_t o[4], int nbins, const float *tab,
-                     float w[6], float kin[4], int bins[3])
+                     float w[ORC_NPROC], float kin[4], int bins[3])
 {
     float u[3], om[3], jd[3];
     const float nbf = (float)nbins;
@@ -104,7 +104,7 @@ static float point32(const float *k, const orc32_grid *g, int mask, const uint32
         jac = jac * jd[0];
         jac = jac * jd[1];
         jac = jac * jd[2];
-        for (int i = 0; i < 6; ++i) w[i] = w[i] * jac;
+        for (int i = 0; i < ORC_NPROC; ++i) w[i] = w[i] * jac;
         tot = tot * jac;
     }
     return tot;
@@ -126,11 +126,11 @@ void orc32_had_points(const orc_hadcfg *c, int nbins, const double *edges, uint6
         uint64_t e = ctr0 + (uint64_t)i;
         uint32_t c4[4] = { (uint32_t)e, (uint32_t)(e >> 32), 0u, 0u }, o[4];
         orc_philox4x32_10(c4, kk, o);
-        float w[6], kin[4];
+        float w[ORC_NPROC], kin[4];
         int bins[3];
         float tot = point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
-        w_out[7 * i] = tot;
-        for (int p = 0; p < 6; ++p) w_out[7 * i + 1 + p] = w[p];
+        w_out[(ORC_NPROC + 1) * i] = tot;
+        for (int p = 0; p < ORC_NPROC; ++p) w_out[(ORC_NPROC + 1) * i + 1 + p] = w[p];
         if (kin_out) for (int q = 0; q < 4; ++q) kin_out[4 * i + q] = kin[q];
     }
     free(tab);
@@ -154,7 +154,7 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
         uint64_t e = ctr0 + (uint64_t)i;
         uint32_t c4[4] = { (uint32_t)e, (uint32_t)(e >> 32), 0u, 0u }, o[4], a[4];
         orc_philox4x32_10(c4, kk, o);
-        float w[6], kin[4];
+        float w[ORC_NPROC], kin[4];
         int bins[3];
         float tot = point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
         uint32_t c5[4] = { (uint32_t)e, (uint32_t)(e >> 32), 1u, 0u };
@@ -165,9 +165,9 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
         float target = (float)(a[2] >> 8) * (1.0f / 16777216.0f) * tot;
         float cum = 0.0f;
         int p = 0;
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < ORC_NPROC; ++q) {
             if (!((c->procMask >> q) & 1) || !(w[q] > 0.0f)) continue;
-            p = 111 + q;
+            p = q < 6 ? 111 + q : 121 + (q - 6);
             cum = cum + w[q];
             if (cum > target) break;
         }
@@ -247,7 +247,7 @@ void orc32_vegas_adapt(const orc_hadcfg *c, int nbins, double alpha, int n_iter,
             uint64_t e = ctr0 + (uint64_t)it * n + i;
             uint32_t c4[4] = { (uint32_t)e, (uint32_t)(e >> 32), 0u, 0u }, o[4];
             orc_philox4x32_10(c4, kk, o);
-            float w[6], kin[4];
+            float w[ORC_NPROC], kin[4];
             int bins[3];
             double t = (double)point32(k, gp, c->procMask, o, nbins, tab, w, kin, bins);
             sw += t;
@@ -389,7 +389,7 @@ void orc_had_mc(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, in
             orc_philox4x32_10(c4, kk, o);
             double u[3] = { (double)(o[0] >> 8) * (1.0 / 16777216.0), (double)(o[1] >> 8) * (1.0 / 16777216.0),
                             (double)(o[2] >> 8) * (1.0 / 16777216.0) };
-            double w[6];
+            double w[ORC_NPROC];
             double tot = orc_had_point(c, u, w, NULL, 1);
             a += tot;
             b += tot * tot;

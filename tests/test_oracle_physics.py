@@ -174,7 +174,7 @@ def test_long_double_and_literal_forms(O):
         u = rng.integers(0, 2 ** 24, 3) / 2.0 ** 24
         t64, _, _ = O.had_point(cfg, u)
         import ctypes as C
-        uu = (C.c_double * 3)(*u); w = (C.c_double * 6)(); k = O.OrcKin()
+        uu = (C.c_double * 3)(*u); w = (C.c_double * O.NPROC)(); k = O.OrcKin()
         tld = O.lib().orc_had_point_ld(C.byref(cfg), uu, w, C.byref(k), 0)
         if tld > 0:
             r = abs(t64 - tld) / tld
