@@ -20,14 +20,17 @@ MI_ERR_NUMERIC = -5
 MI_ERR_CAPACITY = -6
 
 MI_GG2GG, MI_GG2QQBAR, MI_QG2QG, MI_QQ2QQ, MI_QQBAR2GG, MI_QQBAR2QQBARNEW = 111, 112, 113, 114, 115, 116
-MI_MAX_PROC = 6
+MI_GG2CCBAR, MI_QQBAR2CCBAR, MI_GG2BBBAR, MI_QQBAR2BBBAR = 121, 122, 123, 124
+MI_MAX_PROC = 10
+# Pythia code of process slot i (bit i of proc_mask, column 1 + i of per-process arrays)
+PROC_CODES = (111, 112, 113, 114, 115, 116, 121, 122, 123, 124)
 
 MI_MODE_PARTONIC_1D, MI_MODE_HADRONIC_3D = 0, 1
 MI_PREC_FP32, MI_PREC_FP64 = 0, 1
 MI_SAMPLER_FLAT, MI_SAMPLER_VEGAS, MI_SAMPLER_FLOW = 0, 1, 2
 MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT = 0, 1
 MI_PDF_TOY, MI_PDF_GRID = 0, 1
-MI_ABI_VERSION = 2
+MI_ABI_VERSION = 3
 
 MI_FLOW_LAYERS, MI_FLOW_BINS, MI_FLOW_HIDDEN, MI_FLOW_NOUT = 6, 16, 64, 49
 MI_FLOW_NPARAMS = 45222
@@ -39,7 +42,8 @@ class MiPhysics(C.Structure):
     _fields_ = [("mode", I), ("proc_mask", I), ("sHat", D), ("cos_max", D), ("eCM", D),
                 ("pTHatMin", D), ("alphaS_value", D), ("alphaS_order", I), ("alphaS_nfmax", I),
                 ("renorm_fac", D), ("nQuarkIn", I), ("nQuarkNew", I), ("pdf_g", D * 2),
-                ("pdf_val", D * 2), ("pdf_sea", D * 2), ("kfactor", D), ("pdf_source", I), ("fact_fac", D)]
+                ("pdf_val", D * 2), ("pdf_sea", D * 2), ("kfactor", D), ("pdf_source", I), ("fact_fac", D),
+                ("mass_Q", D * 2)]
 
 
 class MiRun(C.Structure):
@@ -98,7 +102,7 @@ PROTOTYPES = {
     "mi_vegas_set_grid": ([VP, P(D), I, I], I),
     "mi_unweight_compact": ([VP, P(MiRun), D, P(MiEvents), P(U64), P(D), P(MiSigmaEst)], I),
     "mi_write_lhe": ([VP, C.c_char_p, P(MiPhysics), P(MiEvents), U64, U64, P(MiSigmaEst)], I),
-    "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, I, P(D), VP], I),
+    "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, VP, I, P(D), VP], I),
     "mi_adjoint": ([VP, P(MiRun), P(MiSigmaGrad)], I),
     "mi_pdf_load": ([VP, C.c_char_p], I),
     "mi_pdf_xf": ([VP, U64, VP, VP, VP], I),

@@ -26,7 +26,7 @@ def physics_default(which: int = 1, **overrides) -> A.MiPhysics:
     if rc:
         raise MIError(rc, f"mi_physics_default({which})")
     for k, v in overrides.items():
-        if k in ("pdf_g", "pdf_val", "pdf_sea"):
+        if k in ("pdf_g", "pdf_val", "pdf_sea", "mass_Q"):
             getattr(p, k)[0], getattr(p, k)[1] = v
         else:
             setattr(p, k, v)
@@ -53,7 +53,7 @@ class SigmaEstimate:
     raw: A.MiSigmaEst
 
     def _idx(self, code: int) -> int:
-        return 0 if code == 0 else code - 110
+        return 0 if code == 0 else 1 + A.PROC_CODES.index(code)
 
     def sigmaGen(self, code: int = 0) -> float:
         return self.raw.sigmaGen[self._idx(code)]
@@ -248,13 +248,15 @@ class Context:
 
     # -------------------------------------------------------------- reweighting
     def reweight_scales(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, w: torch.Tensor,
-                        factors) -> torch.Tensor:
-        """Weights at renormalisation-scale factors (n x nvar), one pass over the events."""
+                        factors, pid: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Weights at renormalisation-scale factors (n x nvar), one pass over the events; pid (the
+        events' process codes, int32) gives the heavy pairs their scale pT^2 + m^2."""
         fac = (C.c_double * len(factors))(*factors)
         n = sHat.numel()
         out = torch.empty((n, len(factors)), dtype=torch.float64, device=self.dev)
         self.sync_stream()
-        self._check(self.lib.mi_reweight_scales(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat), _ptr(w),
+        self._check(self.lib.mi_reweight_scales(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat),
+                                                _ptr(pid) if pid is not None else None, _ptr(w),
                                                 len(factors), fac, _ptr(out)), "mi_reweight_scales")
         return out
 

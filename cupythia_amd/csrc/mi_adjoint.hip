@@ -26,7 +26,8 @@ struct AdjConst {
     T xi;
 };
 
-// Record layout: [0] sum w, [7] sum w^2, [1..4] sum d_k, [8..11] sum d_k^2, [21] nonzero.
+// Record layout: [kRS] sum w, [kRQ] sum w^2, [kRS + 1 + k] sum d_k, [kRQ + 1 + k] sum d_k^2,
+// [kRNz] nonzero.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> ac, int mask, uint64_t key,
                                                     uint64_t ctr0, uint64_t n, uint32_t ppt,
@@ -66,14 +67,14 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> a
             d_b += ws[2] * fb;
         }
         const double dt = (double)tot;
-        v[0] += dt;
-        v[7] = fma(dt, dt, v[7]);
-        v[21] += (tot > (T)0) ? 1.0 : 0.0;
+        v[kRS] += dt;
+        v[kRQ] = fma(dt, dt, v[kRQ]);
+        v[kRNz] += (tot > (T)0) ? 1.0 : 0.0;
         const double d[4] = {(double)d_as, (double)d_xi, (double)d_a, (double)d_b};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            v[1 + q] += d[q];
-            v[8 + q] = fma(d[q], d[q], v[8 + q]);
+            v[kRS + 1 + q] += d[q];
+            v[kRQ + 1 + q] = fma(d[q], d[q], v[kRQ + 1 + q]);
         }
     }
     block_record(v, partials + (size_t)blockIdx.x * kRec);
@@ -116,7 +117,10 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     if (run->sampler != MI_SAMPLER_FLAT) { ctx->err = "the adjoint samples the flat map"; return MI_ERR_ARG; }
     if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64) { ctx->err = "bad precision"; return MI_ERR_ARG; }
     if (run->n_points == 0 || run->n_points > (1ULL << 40)) { ctx->err = "n_points must be 1..2^40"; return MI_ERR_RANGE; }
-    if ((p.proc_mask & 0x3F) == 0 || (p.proc_mask & ~0x3F)) { ctx->err = "bad proc_mask"; return MI_ERR_ARG; }
+    if ((p.proc_mask & 0x3F) == 0 || (p.proc_mask & ~0x3F)) {
+        ctx->err = "the adjoint covers the massless codes 111..116 (proc_mask bits 0..5)";
+        return MI_ERR_ARG;
+    }
     if (p.pdf_source != MI_PDF_TOY) { ctx->err = "the adjoint's PDF-shape derivatives need the toy PDFs"; return MI_ERR_ARG; }
     if (!(p.alphaS_value > 0.0) || !(p.renorm_fac > 0.0) || !(p.eCM > 2.0 * p.pTHatMin) || !(p.pTHatMin > 0.0) ||
         p.nQuarkIn < 2 || p.nQuarkIn > 5 || !(p.pdf_g[0] > -1.0) || !(p.pdf_g[1] > -1.0)) {
@@ -173,11 +177,11 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
         *err = v > 0.0 ? std::sqrt(v) : 0.0;
     };
     std::memset(out, 0, sizeof *out);
-    mean_err(r[0], r[7], &out->sigma, &out->sigma_err);
-    mean_err(r[1], r[8], &out->d_alphaS_MZ, &out->d_alphaS_MZ_err);
-    mean_err(r[2], r[9], &out->d_renorm_fac, &out->d_renorm_fac_err);
-    mean_err(r[3], r[10], &out->d_pdf_g_a, &out->d_pdf_g_a_err);
-    mean_err(r[4], r[11], &out->d_pdf_g_b, &out->d_pdf_g_b_err);
+    mean_err(r[kRS], r[kRQ], &out->sigma, &out->sigma_err);
+    mean_err(r[kRS + 1], r[kRQ + 1], &out->d_alphaS_MZ, &out->d_alphaS_MZ_err);
+    mean_err(r[kRS + 2], r[kRQ + 2], &out->d_renorm_fac, &out->d_renorm_fac_err);
+    mean_err(r[kRS + 3], r[kRQ + 3], &out->d_pdf_g_a, &out->d_pdf_g_a_err);
+    mean_err(r[kRS + 4], r[kRQ + 4], &out->d_pdf_g_b, &out->d_pdf_g_b_err);
     out->kernel_ms = ms;
     if (!std::isfinite(out->sigma) || !std::isfinite(out->d_alphaS_MZ) || !std::isfinite(out->d_pdf_g_a)) {
         ctx->err = "non-finite adjoint result";

@@ -112,6 +112,16 @@ void fill_consts(const mi_physics &p, HadConst<T> &k)
     k.rdu = (T)(n.Adv / n.Auv);
     k.nSea = (T)(2 * p.nQuarkIn);
     k.nQuarkNew = (T)p.nQuarkNew;
+    for (int f = 0; f < 2; ++f) {
+        const double m2 = p.mass_Q[f] * p.mass_Q[f];
+        const double fourMT2 = 4.0 * (m2 + p.pTHatMin * p.pTHatMin);
+        const double LTf = std::log(s / fourMT2);
+        k.fam[f].LT = (T)LTf;
+        k.fam[f].fourMT2 = (T)fourMT2;
+        k.fam[f].pref = (T)(kConvert2mb * kPi * LTf * p.kfactor / fourMT2);
+        k.fam[f].m2 = (T)m2;
+        k.fam[f].mu = (T)(m2 / fourMT2);
+    }
 }
 
 int check_physics(mi_ctx *ctx, const mi_physics &p)
@@ -126,7 +136,14 @@ int check_physics(mi_ctx *ctx, const mi_physics &p)
         return fail(ctx, MI_ERR_ARG, "need eCM > 2 pTHatMin > 0");
     if (p.nQuarkIn < 2 || p.nQuarkIn > 5) return fail(ctx, MI_ERR_ARG, "nQuarkIn must be 2..5");
     if (p.nQuarkNew < 0 || p.nQuarkNew > 5) return fail(ctx, MI_ERR_ARG, "nQuarkNew must be 0..5");
-    if ((p.proc_mask & 0x3F) == 0 || (p.proc_mask & ~0x3F)) return fail(ctx, MI_ERR_ARG, "proc_mask must select codes 111..116");
+    if (p.proc_mask == 0 || (p.proc_mask & ~0x3FF))
+        return fail(ctx, MI_ERR_ARG, "proc_mask must select codes among 111..116, 121..124 (bits 0..9)");
+    for (int f = 0; f < 2; ++f) {
+        if (!(p.proc_mask & kFamMask[1 + f])) continue;
+        const double m = p.mass_Q[f];
+        if (!(m > 0.0) || !(p.eCM * p.eCM > 4.0 * (m * m + p.pTHatMin * p.pTHatMin)))
+            return fail(ctx, MI_ERR_ARG, "heavy-quark codes need mass_Q[%d] > 0 and eCM > 2 mT", f);
+    }
     if (!(p.alphaS_value > 0.0)) return fail(ctx, MI_ERR_ARG, "alphaS_value must be > 0");
     if (!(p.renorm_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "renorm_fac must be > 0");
     if (!(p.pdf_g[0] > -1.0) || !(p.pdf_val[0] > 0.0) || !(p.pdf_sea[0] > -1.0) || !(p.pdf_g[1] > -1.0) ||
@@ -171,15 +188,15 @@ void fill_estimate(const double *r, int64_t n, int partonic, int mask, double ms
 {
     std::memset(o, 0, sizeof *o);
     o->code[0] = 0;
-    for (int i = 0; i < MI_MAX_PROC; ++i) o->code[1 + i] = 111 + i;
+    for (int i = 0; i < MI_MAX_PROC; ++i) o->code[1 + i] = proc_code(i);
     for (int i = 0; i <= MI_MAX_PROC; ++i) {
         const bool on = (i == 0) || (partonic ? i == 1 : ((mask >> (i - 1)) & 1));
         if (!on) continue;
-        const double sw = r[i], sw2 = r[7 + i];
+        const double sw = r[kRS + i], sw2 = r[kRQ + i];
         const double avg = sw / (double)n;
         o->sumW[i] = sw;
         o->sumW2[i] = sw2;
-        o->wMax[i] = r[14 + i];
+        o->wMax[i] = r[kRM + i];
         o->sigmaGen[i] = avg;
         // ProcessContainer::sigmaDelta with nSel = nAcc = nTry (weighted integration)
         double d2 = 0.0;
@@ -187,7 +204,7 @@ void fill_estimate(const double *r, int64_t n, int partonic, int mask, double ms
         o->sigmaErr[i] = (d2 > 0.0 ? std::sqrt(d2) : 0.0) * avg;
     }
     o->nTry = o->nSel = o->nAcc = n;
-    o->nNonZero = (int64_t)r[21];
+    o->nNonZero = (int64_t)r[kRNz];
     o->kernel_ms = ms;
 }
 
@@ -214,7 +231,8 @@ Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas
 {
     const int nb = ctx->grid.nbins;
     const int occ = run->phys.mode == MI_MODE_HADRONIC_3D
-                        ? occ_integrate_had(run->precision, vegas, hist, nb, run->phys.pdf_source == MI_PDF_GRID)
+                        ? occ_integrate_had(run->precision, vegas, hist, nb, run->phys.pdf_source == MI_PDF_GRID,
+                                            has_heavy(run->phys.proc_mask))
                         : occ_integrate_c1(vegas, hist, nb);
     const uint32_t cap = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * (uint32_t)occ;
     return geom_for(n, 16, cap);
@@ -325,6 +343,8 @@ int mi_physics_default(int which, mi_physics *p)
     p->kfactor = 1.0;
     p->pdf_source = MI_PDF_TOY;
     p->fact_fac = 1.0;
+    p->mass_Q[0] = 1.5;     // ParticleData m0 of c and b (ParticleData.xml)
+    p->mass_Q[1] = 4.8;
     p->eCM = 13600.0;
     p->pTHatMin = 20.0;
     p->sHat = 100.0 * 100.0;
@@ -358,8 +378,11 @@ int mi_sigma_hat(mi_ctx *ctx, int code, uint64_t n, const double *sH_dev, const 
                  int nQuarkNew, double kfactor, double *out_dev)
 {
     if (!ctx) return MI_ERR_ARG;
-    if (code < MI_GG2GG || code > MI_QQBAR2QQBARNEW) return fail(ctx, MI_ERR_ARG, "unknown process code %d", code);
+    const bool heavy = code >= MI_GG2CCBAR && code <= MI_QQBAR2BBBAR;
+    if ((code < MI_GG2GG || code > MI_QQBAR2QQBARNEW) && !heavy)
+        return fail(ctx, MI_ERR_ARG, "unknown process code %d", code);
     if (n && (!sH_dev || !tH_dev || !alpS_dev || !out_dev)) return fail(ctx, MI_ERR_ARG, "NULL input/output array");
+    if (n && heavy && !uH_dev) return fail(ctx, MI_ERR_ARG, "codes 121..124 need uH (the mass is (s + t + u) / 2)");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, launch_sigma_hat(ctx->stream, code, n, sH_dev, tH_dev, uH_dev, alpS_dev, id1_dev, id2_dev,
                                  nQuarkNew, kfactor, out_dev));
@@ -521,16 +544,16 @@ int mi_vegas_adapt(mi_ctx *ctx, const mi_run *run, int n_iter, double alpha, mi_
         std::vector<double> I(n_iter), V(n_iter);
         for (int it = 0; it < n_iter; ++it) {
             const double *q = &rec[(size_t)it * kRec];
-            const double avg = q[0] / n;
-            const double var = (q[7] / n - avg * avg) / n;
+            const double avg = q[kRS] / n;
+            const double var = (q[kRQ] / n - avg * avg) / n;
             I[it] = avg;
             V[it] = var;
             if (var > 0.0) { sw += avg / var; swi += 1.0 / var; }
             if (it == n_iter - 1) {
                 out->last_sigma = avg;
                 out->last_err = var > 0.0 ? std::sqrt(var) : 0.0;
-                out->last_wmax = q[14];
-                out->last_eta = q[14] > 0.0 ? avg / q[14] : 0.0;
+                out->last_wmax = q[kRM];
+                out->last_eta = q[kRM] > 0.0 ? avg / q[kRM] : 0.0;
             }
         }
         if (swi > 0.0) {
@@ -591,7 +614,7 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
         HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, ctx->red));
         HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        wmax = r[14];
+        wmax = r[kRM];
         if (!(wmax > 0.0)) return fail(ctx, MI_ERR_NUMERIC, "all candidate weights are zero");
     }
     L.wmax = wmax;
@@ -620,8 +643,8 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
 
 // ---------------------------------------------------------------------------- reweighting
 int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev,
-                       const double *tHat_dev, const double *w_dev, int nvar, const double *factors_host,
-                       double *out_dev)
+                       const double *tHat_dev, const int32_t *pid_dev, const double *w_dev, int nvar,
+                       const double *factors_host, double *out_dev)
 {
     if (!ctx) return MI_ERR_ARG;
     if (!phys) return fail(ctx, MI_ERR_ARG, "phys is NULL");
@@ -641,7 +664,7 @@ int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const do
     int rc = ensure(ctx, &ctx->hist, &ctx->hist_cap, (size_t)nvar);
     if (rc) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->hist, factors_host, sizeof(double) * nvar, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, launch_reweight(ctx->stream, k, n, sHat_dev, tHat_dev, w_dev, nvar, ctx->hist, out_dev));
+    HIPCHK(ctx, launch_reweight(ctx->stream, k, n, sHat_dev, tHat_dev, pid_dev, w_dev, nvar, ctx->hist, out_dev));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MI_OK;
 }

@@ -18,6 +18,15 @@
 // which equals Pythia's sigmaPDF * wtTau * wtY * wtZ for tau = tauMin (1/tauMin)^u0,
 // y = yMax (2 u1 - 1), z = zMax (2 u2 - 1).
 //
+// Heavy-quark pairs (codes 121..124, Sigma2gg2QQbar / Sigma2qqbar2QQbar, SigmaQCD.cc, with
+// m3 = m4 = m): Pythia's massive phase space (tauMin = (2 mT)^2 / S with mT^2 = m^2 + pTmin^2,
+// p2Abs = sH beta^2 / 4, z in [-zMax, zMax] with zMax = sqrt(1 - pTmin^2 / p2Abs)) is the
+// massless map with pTmin^2 -> mT^2: beta z is uniform in [-sqrt(1 - 4 mT^2/sH), +...], so
+// a = (m^2 - tH)/sH, b = (m^2 - uH)/sH and the Jacobian sH zMax' come out of the same formulas;
+// Q2Ren = sqrt((pT^2 + m^2)^2) = Q2Fac = pT^2 + m^2 = sH a b.  Each family (massless, c cbar,
+// b bbar) has its own map at the same uniforms, as each of Pythia's processes has its own
+// PhaseSpace.
+//
 // FP32 arithmetic here is bit-reproducible on the host: only correctly rounded IEEE operations,
 // explicit fmaf, no contraction (the library is compiled with -ffp-contract=off), and the own
 // exp/log below (Cody-Waite reduction + fitted polynomial; table-driven log).  That is what makes
@@ -31,7 +40,9 @@ namespace mi {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kConvert2mb = 0.389380;   // Pythia's GeV^-2 -> mb
-constexpr int kMaxProc = 6;
+constexpr int kMaxProc = 10;      // slots 0..5 = codes 111..116, 6..9 = codes 121..124
+constexpr int kLightProc = 6;
+constexpr int kFamMask[3] = {0x3F, 0xC0, 0x300};   // proc_mask bits of each family
 
 // ------------------------------------------------------------------ Philox4x32-10
 struct U4 { uint32_t x, y, z, w; };
@@ -226,8 +237,16 @@ struct HadConst {
     T nSea;        // 2 nQuarkIn
     T nQuarkNew;
     T spare;
+    // heavy-quark families 0 = c cbar (121, 122), 1 = b bbar (123, 124)
+    struct Fam {
+        T LT;      // ln(S / (4 mT^2)), mT^2 = m^2 + pTmin^2
+        T fourMT2; // 4 mT^2
+        T pref;    // CONVERT2MB pi LT K / (4 mT^2)
+        T m2;      // m^2
+        T mu;      // m^2 / (4 mT^2), so m^2 / sH = mu r
+    } fam[2];
 };
-static_assert(sizeof(HadConst<float>) == 32 * sizeof(float), "HadConst layout");
+static_assert(sizeof(HadConst<float>) == 42 * sizeof(float), "HadConst layout");
 
 // ------------------------------------------------------------------ PDF grid (LHAGrid1)
 // A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
@@ -376,16 +395,18 @@ template <typename T>
 struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 
 // ------------------------------------------------------------------ FP32 hadronic point
-// Inputs: (u, 1-u) for each dimension (exact pairs); writes ws[i] = weight of code 111+i
-// (0 if masked out), returns the total.  Operation-for-operation identical to the host
-// restatement used to check compaction indices.
-__device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
-                                               float u0, float om0, float u1, float om1,
-                                               float u2, float om2, float* ws, float* sHo, float* tHo,
-                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr,
-                                               const GridPdf<float>* gp = nullptr)
+// One family (FAM 0: the massless codes 111..116 into ws[0..5]; 1, 2: c cbar, b bbar into
+// ws[6..7], ws[8..9]).  Inputs: (u, 1-u) for each dimension (exact pairs); writes the family's
+// slots (0 if masked out) and returns their sum.  Operation-for-operation identical to the
+// host restatement used to check compaction indices.
+template <int FAM>
+__device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int mask,
+                                                float u0, float om0, float u1, float om1,
+                                                float u2, float om2, float* ws, float* sHo, float* tHo,
+                                                float* x1o, float* x2o, PointAux<float>* aux,
+                                                const GridPdf<float>* gp)
 {
-    const float LT = k.LT;
+    const float LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
     const float targ = -(u0 * LT);
     const float r = own_expf(targ);                        // tauMin / tau
     const float yMax = 0.5f * (LT * om0);
@@ -397,13 +418,13 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     const float opz = 1.0f + zMax;
     const float R1 = 1.0f / (r * opz);
     const float omz = (r * r) * R1;                        // r / (1 + zMax) = 1 - zMax
-    const float sH = (k.fourPT2 * opz) * R1;               // 4 pTmin^2 / r
+    const float sH = ((FAM == 0 ? k.fourPT2 : k.fam[FAM - 1].fourMT2) * opz) * R1;   // 4 mT^2 / r
     const float twoz = zMax + zMax;
     const float A = __builtin_fmaf(twoz, om2, omz);        // 1 - z
     const float B = __builtin_fmaf(twoz, u2, omz);         // 1 + z
-    const float a = 0.5f * A, b = 0.5f * B;
+    const float a = 0.5f * A, b = 0.5f * B;                // (m^2 - tH) / sH, (m^2 - uH) / sH
     const float ab = a * b;
-    const float pT2 = sH * ab;
+    const float pT2 = sH * ab;                             // pT^2 + m^2
 
     float alpS, inv_ab, cfo = 0.0f;
     if (k.as_order == 0.0f) {
@@ -453,30 +474,53 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     }
 
     const float q2 = __builtin_fmaf(a, a, b * b);
-    const float common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;
-    float w[6];
-    w[0] = Lgg * (2.25f * __builtin_fmaf(b * ia, ia, __builtin_fmaf(a * ib, ib, 3.0f - ab)));
-    w[1] = Lgg * (k.nQuarkNew * q2 * __builtin_fmaf(inv_ab, 0.16666666666666667f, -0.375f));
-    w[2] = Lqg * (__builtin_fmaf(b, b, 1.0f) * __builtin_fmaf(ia, ia, 0.44444444444444444f * ib));
-    {
-        const float sigT = 0.44444444444444444f * __builtin_fmaf(b, b, 1.0f) * (ia * ia);
-        const float sigU = 0.44444444444444444f * __builtin_fmaf(a, a, 1.0f) * (ib * ib);
-        const float sigTU = -0.29629629629629630f * inv_ab;
-        const float sigST = 0.29629629629629630f * (b * b) * ia;
-        const float csame = 0.5f * (sigU + sigTU - sigT);
-        w[3] = __builtin_fmaf(sigT, LQQ, __builtin_fmaf(csame, Lsame, sigST * Lopp));
-    }
-    w[4] = Lopp * (q2 * __builtin_fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
-    w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
+    const float common = (FAM == 0 ? k.pref : k.fam[FAM - 1].pref) * (alpS * alpS) * (LT * om0) * zMax * r;
     float tot = 0.0f;
+    if (FAM == 0) {
+        float w[6];
+        w[0] = Lgg * (2.25f * __builtin_fmaf(b * ia, ia, __builtin_fmaf(a * ib, ib, 3.0f - ab)));
+        w[1] = Lgg * (k.nQuarkNew * q2 * __builtin_fmaf(inv_ab, 0.16666666666666667f, -0.375f));
+        w[2] = Lqg * (__builtin_fmaf(b, b, 1.0f) * __builtin_fmaf(ia, ia, 0.44444444444444444f * ib));
+        {
+            const float sigT = 0.44444444444444444f * __builtin_fmaf(b, b, 1.0f) * (ia * ia);
+            const float sigU = 0.44444444444444444f * __builtin_fmaf(a, a, 1.0f) * (ib * ib);
+            const float sigTU = -0.29629629629629630f * inv_ab;
+            const float sigST = 0.29629629629629630f * (b * b) * ia;
+            const float csame = 0.5f * (sigU + sigTU - sigT);
+            w[3] = __builtin_fmaf(sigT, LQQ, __builtin_fmaf(csame, Lsame, sigST * Lopp));
+        }
+        w[4] = Lopp * (q2 * __builtin_fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
+        w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        float wi = (mask & (1 << i)) ? common * w[i] : 0.0f;
-        if (!(wi > 0.0f)) wi = 0.0f;
-        ws[i] = wi;
-        tot += wi;
+        for (int i = 0; i < 6; ++i) {
+            float wi = (mask & (1 << i)) ? common * w[i] : 0.0f;
+            if (!(wi > 0.0f)) wi = 0.0f;
+            ws[i] = wi;
+            tot += wi;
+        }
+        if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    } else {
+        // Sigma2gg2QQbar / Sigma2qqbar2QQbar::sigmaKin with tHQ = -sH a, uHQ = -sH b, mu = m^2 / sH:
+        //   gg:  [b/a - 9/4 b^2 + mu/a^2 (a (9/2 b - 1/2 + mu) - 4 mu)  +  (a <-> b)] / 6
+        //   qqb: 4/9 (a^2 + b^2 + 2 mu)
+        const float mu = k.fam[FAM - 1].mu * r;
+        const float cT = __builtin_fmaf(a, __builtin_fmaf(4.5f, b, -0.5f) + mu, -4.0f * mu);
+        const float cU = __builtin_fmaf(b, __builtin_fmaf(4.5f, a, -0.5f) + mu, -4.0f * mu);
+        const float sTS = __builtin_fmaf(b, ia, __builtin_fmaf(-2.25f * b, b, (mu * (ia * ia)) * cT));
+        const float sUS = __builtin_fmaf(a, ib, __builtin_fmaf(-2.25f * a, a, (mu * (ib * ib)) * cU));
+        float w[2];
+        w[0] = Lgg * (0.16666666666666667f * (sTS + sUS));
+        w[1] = Lopp * (0.44444444444444444f * __builtin_fmaf(2.0f, mu, q2));
+        constexpr int lo = 4 + 2 * FAM;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            float wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0f;
+            if (!(wi > 0.0f)) wi = 0.0f;
+            ws[lo + i] = wi;
+            tot += wi;
+        }
+        if (sHo) { *sHo = sH; *tHo = __builtin_fmaf(-sH, a, k.fam[FAM - 1].m2); *x1o = x1; *x2o = x2; }
     }
-    if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
     if (aux) {
         aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
         aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
@@ -484,17 +528,47 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     return tot;
 }
 
+// The point: every family selected by mask, at the same uniforms.  HEAVY = false evaluates the
+// massless family only (codes 111..116) and writes ws[6..9] = 0; kinematics outputs are then
+// single values.  HEAVY = true: sHo .. x2o (if given) are arrays of 3, one entry per family,
+// written for the families that mask selects.
+template <bool HEAVY = false>
+__device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
+                                               float u0, float om0, float u1, float om1,
+                                               float u2, float om2, float* ws, float* sHo, float* tHo,
+                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr,
+                                               const GridPdf<float>* gp = nullptr)
+{
+    if (!HEAVY) {
+#pragma unroll
+        for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0f;
+        return had_family_f32<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0f;
+    float tot = 0.0f;
+    if (mask & kFamMask[0])
+        tot += had_family_f32<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+    if (mask & kFamMask[1])
+        tot += had_family_f32<1>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
+                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp);
+    if (mask & kFamMask[2])
+        tot += had_family_f32<2>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
+                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp);
+    return tot;
+}
+
 // ------------------------------------------------------------------ FP64 hadronic point
 // Same map and algebra in FP64 with the device libm (exp, log, expm1, sqrt); the per-point
 // parity target against the host Pythia-form oracle is 1e-12 relative.
-__device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int mask,
-                                                double u0, double om0, double u1, double om1,
-                                                double u2, double om2, double* ws, double* sHo,
-                                                double* tHo, double* x1o, double* x2o,
-                                                PointAux<double>* aux = nullptr,
-                                                const GridPdf<double>* gp = nullptr)
+template <int FAM>
+__device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int mask,
+                                                 double u0, double om0, double u1, double om1,
+                                                 double u2, double om2, double* ws, double* sHo,
+                                                 double* tHo, double* x1o, double* x2o,
+                                                 PointAux<double>* aux, const GridPdf<double>* gp)
 {
-    const double LT = k.LT;
+    const double LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
     const double targ = -(u0 * LT);
     const double r = exp(targ);
     const double yMax = 0.5 * (LT * om0);
@@ -510,7 +584,7 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
     const double ab = a * b;
     const double inv_ab = 1.0 / ab;
     const double ia = b * inv_ab, ib = a * inv_ab;
-    const double sH = k.fourPT2 / r;
+    const double sH = (FAM == 0 ? k.fourPT2 : k.fam[FAM - 1].fourMT2) / r;
     const double pT2 = sH * ab;
 
     double alpS, cfo = 0.0;
@@ -556,34 +630,81 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
     }
 
     const double q2 = fma(a, a, b * b);
-    const double common = k.pref * (alpS * alpS) * (LT * om0) * zMax * r;
-    double w[6];
-    w[0] = Lgg * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
-    w[1] = Lgg * (k.nQuarkNew * q2 * fma(inv_ab, 1.0 / 6.0, -0.375));
-    w[2] = Lqg * (fma(b, b, 1.0) * fma(ia, ia, (4.0 / 9.0) * ib));
-    {
-        const double sigT = (4.0 / 9.0) * fma(b, b, 1.0) * (ia * ia);
-        const double sigU = (4.0 / 9.0) * fma(a, a, 1.0) * (ib * ib);
-        const double sigTU = -(8.0 / 27.0) * inv_ab;
-        const double sigST = (8.0 / 27.0) * (b * b) * ia;
-        const double csame = 0.5 * (sigU + sigTU - sigT);
-        w[3] = fma(sigT, LQQ, fma(csame, Lsame, sigST * Lopp));
-    }
-    w[4] = Lopp * (q2 * fma(inv_ab, 16.0 / 27.0, -4.0 / 3.0));
-    w[5] = Lopp * (k.nQuarkNew * (4.0 / 9.0) * q2);
+    const double common = (FAM == 0 ? k.pref : k.fam[FAM - 1].pref) * (alpS * alpS) * (LT * om0) * zMax * r;
     double tot = 0.0;
+    if (FAM == 0) {
+        double w[6];
+        w[0] = Lgg * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
+        w[1] = Lgg * (k.nQuarkNew * q2 * fma(inv_ab, 1.0 / 6.0, -0.375));
+        w[2] = Lqg * (fma(b, b, 1.0) * fma(ia, ia, (4.0 / 9.0) * ib));
+        {
+            const double sigT = (4.0 / 9.0) * fma(b, b, 1.0) * (ia * ia);
+            const double sigU = (4.0 / 9.0) * fma(a, a, 1.0) * (ib * ib);
+            const double sigTU = -(8.0 / 27.0) * inv_ab;
+            const double sigST = (8.0 / 27.0) * (b * b) * ia;
+            const double csame = 0.5 * (sigU + sigTU - sigT);
+            w[3] = fma(sigT, LQQ, fma(csame, Lsame, sigST * Lopp));
+        }
+        w[4] = Lopp * (q2 * fma(inv_ab, 16.0 / 27.0, -4.0 / 3.0));
+        w[5] = Lopp * (k.nQuarkNew * (4.0 / 9.0) * q2);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double wi = (mask & (1 << i)) ? common * w[i] : 0.0;
-        if (!(wi > 0.0)) wi = 0.0;
-        ws[i] = wi;
-        tot += wi;
+        for (int i = 0; i < 6; ++i) {
+            double wi = (mask & (1 << i)) ? common * w[i] : 0.0;
+            if (!(wi > 0.0)) wi = 0.0;
+            ws[i] = wi;
+            tot += wi;
+        }
+        if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
+    } else {
+        const double mu = k.fam[FAM - 1].mu * r;
+        const double cT = fma(a, fma(4.5, b, -0.5) + mu, -4.0 * mu);
+        const double cU = fma(b, fma(4.5, a, -0.5) + mu, -4.0 * mu);
+        const double sTS = fma(b, ia, fma(-2.25 * b, b, (mu * (ia * ia)) * cT));
+        const double sUS = fma(a, ib, fma(-2.25 * a, a, (mu * (ib * ib)) * cU));
+        double w[2];
+        w[0] = Lgg * ((sTS + sUS) / 6.0);
+        w[1] = Lopp * ((4.0 / 9.0) * fma(2.0, mu, q2));
+        constexpr int lo = 4 + 2 * FAM;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            double wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0;
+            if (!(wi > 0.0)) wi = 0.0;
+            ws[lo + i] = wi;
+            tot += wi;
+        }
+        if (sHo) { *sHo = sH; *tHo = fma(-sH, a, k.fam[FAM - 1].m2); *x1o = x1; *x2o = x2; }
     }
-    if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
     if (aux) {
         aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
         aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
     }
+    return tot;
+}
+
+template <bool HEAVY = false>
+__device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int mask,
+                                                double u0, double om0, double u1, double om1,
+                                                double u2, double om2, double* ws, double* sHo,
+                                                double* tHo, double* x1o, double* x2o,
+                                                PointAux<double>* aux = nullptr,
+                                                const GridPdf<double>* gp = nullptr)
+{
+    if (!HEAVY) {
+#pragma unroll
+        for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0;
+        return had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0;
+    double tot = 0.0;
+    if (mask & kFamMask[0])
+        tot += had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+    if (mask & kFamMask[1])
+        tot += had_family_f64<1>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
+                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp);
+    if (mask & kFamMask[2])
+        tot += had_family_f64<2>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
+                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp);
     return tot;
 }
 

@@ -591,7 +591,8 @@ __global__ void k_flow_sample_z(uint64_t key, uint64_t ctr0, uint32_t sid, uint6
 }
 
 // Weight of a flow point: y -> VEGAS map -> FP32 integrand, times J_V and |dy/dz| = exp(ld).
-// Writes w (optional) and one kRec record per block.
+// Writes w (optional) and one kRec record per block.  HEAVY: mask may select codes 121..124.
+template <bool HEAVY>
 __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int mask, const float *__restrict__ gtab,
                                                         int nbins, uint64_t n, const float *__restrict__ y,
                                                         const float *__restrict__ ld, float *__restrict__ wout,
@@ -619,22 +620,22 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
             jac = jac * (nbf * q[1]);
         }
         float ws[kMaxProc];
-        const float f = had_point_f32(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr, nullptr,
-                                      nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
+        const float f = had_point_f32<HEAVY>(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr,
+                                             nullptr, nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
         const float wt = f * jac * expf(ld[i]);
         if (wout) wout[i] = wt;
         const double dw = (double)wt;
-        v[0] += dw;
-        v[7] = fma(dw, dw, v[7]);
-        v[14] = fmax(v[14], dw);
-        v[21] += (wt > 0.0f) ? 1.0 : 0.0;
+        v[kRS] += dw;
+        v[kRQ] = fma(dw, dw, v[kRQ]);
+        v[kRM] = fmax(v[kRM], dw);
+        v[kRNz] += (wt > 0.0f) ? 1.0 : 0.0;
         const float scale = jac * expf(ld[i]);
 #pragma unroll
-        for (int p = 0; p < kMaxProc; ++p) {
+        for (int p = 0; p < (HEAVY ? kMaxProc : kLightProc); ++p) {
             const double dp = (double)(ws[p] * scale);
-            v[1 + p] += dp;
-            v[8 + p] = fma(dp, dp, v[8 + p]);
-            v[15 + p] = fmax(v[15 + p], dp);
+            v[kRS + 1 + p] += dp;
+            v[kRQ + 1 + p] = fma(dp, dp, v[kRQ + 1 + p]);
+            v[kRM + 1 + p] = fmax(v[kRM + 1 + p], dp);
         }
     }
     block_record(v, partials + (size_t)blockIdx.x * kRec);
@@ -645,8 +646,8 @@ __global__ void k_flow_coef(const float *__restrict__ w, const double *__restric
                             float *__restrict__ g, float *__restrict__ abar, double *__restrict__ hist, int step)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double mean = rec[0] / (double)n;
-    if (i == 0) { hist[2 * step] = rec[0]; hist[2 * step + 1] = rec[7]; }
+    const double mean = rec[kRS] / (double)n;
+    if (i == 0) { hist[2 * step] = rec[kRS]; hist[2 * step + 1] = rec[kRQ]; }
     if (i >= n) return;
     const double r = (double)w[i] / mean;
     g[i] = (float)(-(r * r) / (double)n);
@@ -692,7 +693,7 @@ __global__ void k_add_record(double *__restrict__ acc, const double *__restrict_
     const int i = threadIdx.x;
     if (i >= kRec) return;
     if (first) { acc[i] = rec[i]; return; }
-    acc[i] = (i >= 14 && i <= 20) ? fmax(acc[i], rec[i]) : acc[i] + rec[i];
+    acc[i] = rec_is_max(i) ? fmax(acc[i], rec[i]) : acc[i] + rec[i];
 }
 
 }  // namespace mi
@@ -1006,9 +1007,13 @@ static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, ui
     const uint32_t nblk = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock,
                                                       (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ);
     if ((size_t)nblk * kRec > ctx->partials_cap) return ferr(ctx, MI_ERR_STATE, "partials buffer too small");
-    hipLaunchKernelGGL(k_flow_weight, dim3(nblk), dim3(kBlock), sizeof(float) * 9 * ctx->grid.nbins, ctx->stream, k,
-                       run->phys.proc_mask, ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials,
-                       gpdf);
+    const size_t sm = sizeof(float) * 9 * ctx->grid.nbins;
+    if (has_heavy(run->phys.proc_mask))
+        hipLaunchKernelGGL(k_flow_weight<true>, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask,
+                           ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
+    else
+        hipLaunchKernelGGL(k_flow_weight<false>, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask,
+                           ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
     FCHK(hipGetLastError());
     FCHK(launch_reduce_records(ctx->stream, ctx->partials, nblk, out_rec));
     return MI_OK;
@@ -1125,20 +1130,20 @@ int mi_flow_integrate_internal(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out
     std::memset(out, 0, sizeof *out);
     const double nn = (double)run->n_points;
     out->code[0] = 0;
-    for (int i = 0; i < MI_MAX_PROC; ++i) out->code[1 + i] = 111 + i;
+    for (int i = 0; i < MI_MAX_PROC; ++i) out->code[1 + i] = proc_code(i);
     for (int i = 0; i <= MI_MAX_PROC; ++i) {
         if (i > 0 && !((run->phys.proc_mask >> (i - 1)) & 1)) continue;
-        const double avg = r[i] / nn;
-        out->sumW[i] = r[i];
-        out->sumW2[i] = r[7 + i];
-        out->wMax[i] = r[14 + i];
+        const double avg = r[kRS + i] / nn;
+        out->sumW[i] = r[kRS + i];
+        out->sumW2[i] = r[kRQ + i];
+        out->wMax[i] = r[kRM + i];
         out->sigmaGen[i] = avg;
         double d2 = 0.0;
-        if (avg != 0.0) d2 = (r[7 + i] / nn - avg * avg) / nn / (avg * avg);
+        if (avg != 0.0) d2 = (r[kRQ + i] / nn - avg * avg) / nn / (avg * avg);
         out->sigmaErr[i] = (d2 > 0.0 ? std::sqrt(d2) : 0.0) * avg;
     }
     out->nTry = out->nSel = out->nAcc = (int64_t)run->n_points;
-    out->nNonZero = (int64_t)r[21];
+    out->nNonZero = (int64_t)r[kRNz];
     out->kernel_ms = ms;
     if (!std::isfinite(out->sigmaGen[0])) return ferr(ctx, MI_ERR_NUMERIC, "non-finite sigma");
     return MI_OK;

@@ -12,13 +12,24 @@
 
 namespace mi {
 
-// Block partial record of the integration kernels (FP64):
-//   [0..6]   sum w       (0 = total, 1 + i = code 111 + i)
-//   [7..13]  sum w^2
-//   [14..20] max w
-//   [21]     points with total weight > 0
-//   [22..23] spare (adjoint / unweighting use them)
-constexpr int kRec = 24;
+// Block partial record of the integration kernels (FP64), kRecN = 1 + kMaxProc slots per
+// quantity (0 = total, 1 + i = process slot i: codes 111..116, 121..124):
+//   [kRS ..]  sum w        [kRQ ..] sum w^2        [kRM ..] max w
+//   [kRNz]    points with total weight > 0
+//   [kRAcc]   accepted candidates (unweighting)
+constexpr int kRecN = kMaxProc + 1;
+constexpr int kRS = 0;
+constexpr int kRQ = kRecN;
+constexpr int kRM = 2 * kRecN;
+constexpr int kRNz = 3 * kRecN;
+constexpr int kRAcc = kRNz + 1;
+constexpr int kRec = 36;
+static_assert(kRAcc < kRec, "record layout");
+__host__ __device__ constexpr bool rec_is_max(int i) { return i >= kRM && i < kRM + kRecN; }
+// Does mask select a heavy-quark code (121..124)?
+__host__ __device__ constexpr bool has_heavy(int mask) { return (mask & (kFamMask[1] | kFamMask[2])) != 0; }
+// Pythia process code of slot i.
+__host__ __device__ constexpr int proc_code(int i) { return i < kLightProc ? 111 + i : 121 + (i - kLightProc); }
 constexpr int kBlock = 256;
 constexpr int kMaxBins = 256;
 
@@ -69,7 +80,7 @@ struct Geom { uint32_t blocks; uint32_t ppt; };
 // Points per thread >= min_ppt, blocks <= max_blocks.
 Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks);
 // Resident blocks per CU of the integration kernels (hipOccupancyMaxActiveBlocksPerMultiprocessor).
-int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid);
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid, bool heavy);
 int occ_integrate_c1(bool vegas, bool hist, int nbins);
 
 // ------------------------------------------------------------------ launchers (mi_kernels.hip)
@@ -137,7 +148,7 @@ hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64
 
 // Renormalisation-scale reweighting of events (mi_kernels.hip).
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
-                           const double *w, int nvar, const double *fac, double *out);
+                           const int32_t *pid, const double *w, int nvar, const double *fac, double *out);
 
 // Adjoint (mi_adjoint.hip): partials of kRec doubles per block, see that file for the layout.
 struct AdjLaunch {

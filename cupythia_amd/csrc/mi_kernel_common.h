@@ -20,27 +20,35 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
-// Combine kRec per-thread values into one block record (sum except [14..20] = max).
-// If cnt is given, it receives slot 22 (the unweighting accept count) as an integer.
+// Combine kRec per-thread values into one block record (sum except the max slots).
+// If cnt is given, it receives slot kRAcc (the unweighting accept count) as an integer.
 __device__ __forceinline__ void block_record(const double *v, double *dst, uint64_t *cnt = nullptr)
 {
     __shared__ double red[kBlock / 64][kRec];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < kRec; ++i) {
-        const bool isMax = (i >= 14 && i <= 20);
+        const bool isMax = rec_is_max(i);
         const double r = isMax ? wave_max(v[i]) : wave_sum(v[i]);
         if (lane == 0) red[wave][i] = r;
     }
     __syncthreads();
     if (threadIdx.x < kRec) {
         const int i = threadIdx.x;
-        const bool isMax = (i >= 14 && i <= 20);
+        const bool isMax = rec_is_max(i);
         double r = red[0][i];
         for (int w = 1; w < kBlock / 64; ++w) r = isMax ? fmax(r, red[w][i]) : r + red[w][i];
         dst[i] = r;
-        if (cnt && i == 22) *cnt = (uint64_t)r;
+        if (cnt && i == kRAcc) *cnt = (uint64_t)r;
     }
+}
+
+// a[f] of a family array (f < N) as selects, so a register array never needs a dynamic index.
+template <typename T, int N>
+__device__ __forceinline__ T pick_fam(const T *a, int f)
+{
+    if (N == 1) return a[0];
+    return f == 0 ? a[0] : (f == 1 ? a[1] : a[N - 1]);
 }
 
 // ------------------------------------------------------------------------------------ map
@@ -80,34 +88,36 @@ template <typename T> struct HadPointAux;
 template <> struct HadPointAux<float> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0, float u1,
                                                  float o1, float u2, float o2, float *ws, PointAux<float> *ax)
-    { return had_point_f32(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
+    { return had_point_f32<false>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
 };
 template <> struct HadPointAux<double> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0, double u1,
                                                   double o1, double u2, double o2, double *ws, PointAux<double> *ax)
-    { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
+    { return had_point_f64<false>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
 };
 
-// gp: the PDF grid, or nullptr for the toy PDFs.
-template <typename T> struct HadPoint;
-template <> struct HadPoint<float> {
+// gp: the PDF grid, or nullptr for the toy PDFs.  HEAVY: see had_point_f32.
+template <typename T, bool HEAVY> struct HadPoint;
+template <bool HEAVY> struct HadPoint<float, HEAVY> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
                                                  float u1, float o1, float u2, float o2, float *ws,
                                                  float *sH, float *tH, float *x1, float *x2,
                                                  const GridPdf<float> *gp)
-    { return had_point_f32(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
+    { return had_point_f32<HEAVY>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
 };
-template <> struct HadPoint<double> {
+template <bool HEAVY> struct HadPoint<double, HEAVY> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
                                                   double *sH, double *tH, double *x1, double *x2,
                                                   const GridPdf<double> *gp)
-    { return had_point_f64(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
+    { return had_point_f64<HEAVY>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
 };
 
 // One candidate of the hadronic integrand: Philox stream 0 of event e -> three 24-bit
-// uniforms -> (optional VEGAS map) -> weights (mapped by the grid Jacobian).
-template <typename T, bool VEGAS>
+// uniforms -> (optional VEGAS map) -> weights (mapped by the grid Jacobian).  HEAVY = false:
+// the massless codes only (ws[6..9] = 0); HEAVY = true: every family of mask, kinematics
+// outputs (if given) one per family.
+template <typename T, bool VEGAS, bool HEAVY = false>
 __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint64_t key, uint64_t e,
                                            const T *stab, int nbins, T nbf, T *ws, int *bins,
                                            T *sH, T *tH, T *x1, T *x2, const GridPdf<T> *gp = nullptr)
@@ -121,14 +131,14 @@ __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint6
     } else {
         m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
     }
-    T tot = HadPoint<T>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
+    T tot = HadPoint<T, HEAVY>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
     if (VEGAS) {
         T jac = (T)1;
         jac = jac * m0.jac;
         jac = jac * m1.jac;
         jac = jac * m2.jac;
 #pragma unroll
-        for (int i = 0; i < kMaxProc; ++i) ws[i] = ws[i] * jac;
+        for (int i = 0; i < (HEAVY ? kMaxProc : kLightProc); ++i) ws[i] = ws[i] * jac;
         tot = tot * jac;
         bins[0] = m0.bin; bins[1] = m1.bin; bins[2] = m2.bin;
     }

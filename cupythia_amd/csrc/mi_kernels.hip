@@ -35,7 +35,9 @@ hipError_t launch_philox_fill(hipStream_t s, uint64_t key, uint64_t ctr0, uint32
 
 // ------------------------------------------------------------------------------------ sigmaHat
 // Kfactor * sigmaHatWrap for one code, FP64.  Brackets in terms of the ratios t/s, u/s with
-// three reciprocals; algebraically identical to SigmaQCD.cc:115-378.
+// three reciprocals; algebraically identical to SigmaQCD.cc:115-378.  The heavy pairs 121..124
+// (Sigma2gg2QQbar, Sigma2qqbar2QQbar::sigmaKin with m3 = m4) take m^2 = (s + t + u) / 2 from
+// the three invariants and Pythia's tHQ = -(s - t + u)/2, uHQ = -(s + t - u)/2.
 __global__ __launch_bounds__(kBlock) void k_sigma_hat(int code, uint64_t n, const double *__restrict__ sH,
                                                       const double *__restrict__ tH, const double *__restrict__ uH,
                                                       const double *__restrict__ alpS,
@@ -81,6 +83,23 @@ __global__ __launch_bounds__(kBlock) void k_sigma_hat(int code, uint64_t n, cons
     case MI_QQBAR2QQBARNEW:
         br = nQN * (4.0 / 9.0) * (ts * ts + us * us);
         break;
+    case MI_GG2CCBAR:
+    case MI_GG2BBBAR:
+    case MI_QQBAR2CCBAR:
+    case MI_QQBAR2BBBAR: {
+        const double m2 = 0.5 * (s + t + u);
+        const double tq = -0.5 * (s - t + u), uq = -0.5 * (s + t - u);
+        const double ta = -tq * is, ub = -uq * is, mu = m2 * is;       // a, b, m^2 / s
+        if (code == MI_GG2CCBAR || code == MI_GG2BBBAR) {
+            const double ia = 1.0 / ta, ib = 1.0 / ub;
+            const double sTS = ub * ia - 2.25 * ub * ub + mu * ia * ia * (ta * (4.5 * ub - 0.5 + mu) - 4.0 * mu);
+            const double sUS = ta * ib - 2.25 * ta * ta + mu * ib * ib * (ub * (4.5 * ta - 0.5 + mu) - 4.0 * mu);
+            br = (sTS + sUS) / 6.0;
+        } else {
+            br = (4.0 / 9.0) * (ta * ta + ub * ub + 2.0 * mu);
+        }
+        break;
+    }
     default:
         br = 0.0;
     }
@@ -106,8 +125,10 @@ template <typename T> __device__ __forceinline__ double max_value(double m) { re
 
 // ------------------------------------------------------------------------------------ hadronic integration
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
-// GRID: parton densities from the PDF grid gpdf instead of the toy PDFs.
-template <typename T, bool VEGAS, bool HIST, bool GRID>
+// GRID: parton densities from the PDF grid gpdf instead of the toy PDFs.  HEAVY: mask may
+// select the heavy-quark codes 121..124 (their families are evaluated); otherwise only the
+// massless codes are computed and accumulated.
+template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY>
 __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
                                                           uint64_t ctr0, uint64_t n, uint32_t ppt,
                                                           const T *__restrict__ gtab, int nbins,
@@ -133,9 +154,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
     // maximum is an integer max of the bit patterns.  Converted to FP64 once at the end, which is
     // exact and order-preserving.  The nonzero count is an integer.
     using MaxT = typename std::conditional<sizeof(T) == 4, uint32_t, double>::type;
-    MaxT mx[kMaxProc + 1];
+    constexpr int NP = HEAVY ? kMaxProc : kLightProc;
+    MaxT mx[NP + 1];
 #pragma unroll
-    for (int i = 0; i <= kMaxProc; ++i) mx[i] = 0;
+    for (int i = 0; i <= NP; ++i) mx[i] = 0;
     uint32_t nnz = 0;
 
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
@@ -144,19 +166,19 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
         if (i >= n) break;
         T ws[kMaxProc];
         int bins[3];
-        const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
-                                              (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
-                                              GRID ? &gpdf : nullptr);
+        const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
+                                                     (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
+                                                     GRID ? &gpdf : nullptr);
         const double dt = (double)tot;
-        v[0] += dt;
-        v[7] = fma(dt, dt, v[7]);
+        v[kRS] += dt;
+        v[kRQ] = fma(dt, dt, v[kRQ]);
         mx[0] = run_max(mx[0], tot);
         nnz += (tot > (T)0) ? 1u : 0u;
 #pragma unroll
-        for (int p = 0; p < kMaxProc; ++p) {
+        for (int p = 0; p < NP; ++p) {
             const double d = (double)ws[p];
-            v[1 + p] += d;
-            v[8 + p] = fma(d, d, v[8 + p]);
+            v[kRS + 1 + p] += d;
+            v[kRQ + 1 + p] = fma(d, d, v[kRQ + 1 + p]);
             mx[1 + p] = run_max(mx[1 + p], ws[p]);
         }
         if (HIST) {
@@ -168,8 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
         }
     }
 #pragma unroll
-    for (int i = 0; i <= kMaxProc; ++i) v[14 + i] = max_value<T>(mx[i]);
-    v[21] = (double)nnz;
+    for (int i = 0; i <= NP; ++i) v[kRM + i] = max_value<T>(mx[i]);
+    v[kRNz] = (double)nnz;
     block_record(v, partials + (size_t)blockIdx.x * kRec);
     if (HIST) {
         __syncthreads();
@@ -188,7 +210,7 @@ static size_t had_smem(const HadLaunch &L)
     return tb + (L.hist ? sizeof(double) * 4 * 3 * L.grid->nbins : 0);
 }
 
-template <typename T, bool GRID>
+template <typename T, bool GRID, bool HEAVY>
 static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                 const GridPdf<T> &gp)
 {
@@ -196,14 +218,14 @@ static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadCons
     const int nb = L.vegas ? L.grid->nbins : 0;
     dim3 grid(L.g.blocks), blk(kBlock);
     if (!L.vegas)
-        hipLaunchKernelGGL((k_integrate_had<T, false, false, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, false, false, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
+                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else if (!L.hist)
-        hipLaunchKernelGGL((k_integrate_had<T, true, false, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, true, false, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
+                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else
-        hipLaunchKernelGGL((k_integrate_had<T, true, true, GRID>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0, L.n,
-                           L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, true, true, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
+                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     return hipGetLastError();
 }
 
@@ -211,7 +233,9 @@ template <typename T>
 static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                const GridPdf<T> &gp)
 {
-    return gp.active ? launch_had_tg<T, true>(s, L, k, tab, gp) : launch_had_tg<T, false>(s, L, k, tab, gp);
+    if (has_heavy(L.mask))
+        return gp.active ? launch_had_tg<T, true, true>(s, L, k, tab, gp) : launch_had_tg<T, false, true>(s, L, k, tab, gp);
+    return gp.active ? launch_had_tg<T, true, false>(s, L, k, tab, gp) : launch_had_tg<T, false, false>(s, L, k, tab, gp);
 }
 
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L)
@@ -247,14 +271,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_c1(double pref, double cMa
         if (i >= n) break;
         int bin;
         const double w = c1_candidate(pref, cMax, key, ctr0 + i, VEGAS, stab, nbins, &bin);
-        v[0] += w;
-        v[7] = fma(w, w, v[7]);
-        v[14] = fmax(v[14], w);
-        v[21] += (w > 0.0) ? 1.0 : 0.0;
+        v[kRS] += w;
+        v[kRQ] = fma(w, w, v[kRQ]);
+        v[kRM] = fmax(v[kRM], w);
+        v[kRNz] += (w > 0.0) ? 1.0 : 0.0;
         if (HIST) atomicAdd(&shist[wave * nbins + bin], w * w);
     }
     // gg -> gg is the only process: copy total into slot of code 111
-    v[1] = v[0]; v[8] = v[7]; v[15] = v[14];
+    v[kRS + 1] = v[kRS]; v[kRQ + 1] = v[kRQ]; v[kRM + 1] = v[kRM];
     block_record(v, partials + (size_t)blockIdx.x * kRec);
     if (HIST) {
         __syncthreads();
@@ -285,7 +309,8 @@ hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L)
 }
 
 // ------------------------------------------------------------------------------------ per-point dumps
-template <typename T, bool VEGAS>
+// Kinematics of the first family that mask selects (massless, c cbar, b bbar).
+template <typename T, bool VEGAS, bool HEAVY>
 __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, uint64_t key, uint64_t ctr0,
                                                        uint64_t n, const T *__restrict__ gtab, int nbins,
                                                        double *__restrict__ w, double *__restrict__ kin,
@@ -295,31 +320,42 @@ __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, 
     if (i >= n) return;
     T ws[kMaxProc];
     int bins[3];
-    T sH, tH, x1, x2;
-    const T tot = had_candidate<T, VEGAS>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, &sH, &tH,
-                                          &x1, &x2, gpdf.active ? &gpdf : nullptr);
-    w[i * (kMaxProc + 1)] = (double)tot;
+    constexpr int NF = HEAVY ? 3 : 1;
+    T sH[NF], tH[NF], x1[NF], x2[NF];
+    const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, sH, tH,
+                                                 x1, x2, gpdf.active ? &gpdf : nullptr);
+    w[i * kRecN] = (double)tot;
 #pragma unroll
-    for (int p = 0; p < kMaxProc; ++p) w[i * (kMaxProc + 1) + 1 + p] = (double)ws[p];
+    for (int p = 0; p < kMaxProc; ++p) w[i * kRecN + 1 + p] = (double)ws[p];
     if (kin) {
-        kin[4 * i + 0] = (double)sH; kin[4 * i + 1] = (double)tH;
-        kin[4 * i + 2] = (double)x1; kin[4 * i + 3] = (double)x2;
+        const int f = !HEAVY ? 0 : ((mask & kFamMask[0]) ? 0 : ((mask & kFamMask[1]) ? 1 : 2));
+        kin[4 * i + 0] = (double)pick_fam<T, NF>(sH, f); kin[4 * i + 1] = (double)pick_fam<T, NF>(tH, f);
+        kin[4 * i + 2] = (double)pick_fam<T, NF>(x1, f); kin[4 * i + 3] = (double)pick_fam<T, NF>(x2, f);
     }
+}
+
+template <typename T, bool HEAVY>
+static void launch_points_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
+                                const GridPdf<T> &gp, double *w, double *kin)
+{
+    dim3 grid((uint32_t)((L.n + kBlock - 1) / kBlock)), blk(kBlock);
+    const int nb = L.vegas ? L.grid->nbins : 0;
+    if (L.vegas) hipLaunchKernelGGL((k_points_had<T, true, HEAVY>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
+    else         hipLaunchKernelGGL((k_points_had<T, false, HEAVY>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
 }
 
 hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, double *kin)
 {
     if (L.n == 0) return hipSuccess;
-    dim3 grid((uint32_t)((L.n + kBlock - 1) / kBlock)), blk(kBlock);
-    const int nb = L.vegas ? L.grid->nbins : 0;
+    const bool hv = has_heavy(L.mask);
     if (L.prec == MI_PREC_FP32) {
         const float *tab = L.vegas ? L.grid->tab32 : nullptr;
-        if (L.vegas) hipLaunchKernelGGL((k_points_had<float, true>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gf);
-        else         hipLaunchKernelGGL((k_points_had<float, false>), grid, blk, 0, s, L.kf, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gf);
+        if (hv) launch_points_had_t<float, true>(s, L, L.kf, tab, L.gf, w, kin);
+        else    launch_points_had_t<float, false>(s, L, L.kf, tab, L.gf, w, kin);
     } else {
         const double *tab = L.vegas ? L.grid->tab64 : nullptr;
-        if (L.vegas) hipLaunchKernelGGL((k_points_had<double, true>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gd);
-        else         hipLaunchKernelGGL((k_points_had<double, false>), grid, blk, 0, s, L.kd, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, L.gd);
+        if (hv) launch_points_had_t<double, true>(s, L, L.kd, tab, L.gd, w, kin);
+        else    launch_points_had_t<double, false>(s, L, L.kd, tab, L.gd, w, kin);
     }
     return hipGetLastError();
 }
@@ -333,9 +369,9 @@ __global__ __launch_bounds__(kBlock) void k_points_c1(double pref, double cMax, 
     if (i >= n) return;
     int bin;
     const double wt = c1_candidate(pref, cMax, key, ctr0 + i, vegas != 0, gtab, nbins, &bin);
-    w[i * (kMaxProc + 1)] = wt;
-    w[i * (kMaxProc + 1) + 1] = wt;
-    for (int p = 1; p < kMaxProc; ++p) w[i * (kMaxProc + 1) + 1 + p] = 0.0;
+    w[i * kRecN] = wt;
+    w[i * kRecN + 1] = wt;
+    for (int p = 1; p < kMaxProc; ++p) w[i * kRecN + 1 + p] = 0.0;
     if (kin) {
         // recompute c for the kinematics record
         const U4 r = philox_event(key, ctr0 + i, 0u);
@@ -370,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_records(const double *__restr
 {
     __shared__ double sh[kBlock];
     for (int i = 0; i < kRec; ++i) {
-        const bool isMax = (i >= 14 && i <= 20);
+        const bool isMax = rec_is_max(i);
         double acc = 0.0;
         for (uint32_t b = threadIdx.x; b < nblk; b += kBlock) {
             const double x = parts[(size_t)b * kRec + i];
@@ -492,26 +528,36 @@ hipError_t launch_vegas_tables(hipStream_t s, Grid &g)
 
 // ------------------------------------------------------------------------------------ occupancy
 // Resident blocks per CU of each integration kernel instance (for one-wave, equal-work grids).
-template <typename T, bool GRID>
+template <typename T, bool GRID, bool HEAVY>
 static int occ_had_t(bool vegas, bool hist, size_t sm)
 {
     int nb = 0;
     hipError_t e;
-    if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, false, false, GRID>, kBlock, sm);
-    else if (!hist) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, false, GRID>, kBlock, sm);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, true, GRID>, kBlock, sm);
+    if (!vegas)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, false, false, GRID, HEAVY>, kBlock, sm);
+    else if (!hist)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, false, GRID, HEAVY>, kBlock, sm);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, true, GRID, HEAVY>, kBlock, sm);
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid)
+template <typename T>
+static int occ_had_tt(bool vegas, bool hist, size_t sm, bool grid, bool heavy)
+{
+    if (heavy) return grid ? occ_had_t<T, true, true>(vegas, hist, sm) : occ_had_t<T, false, true>(vegas, hist, sm);
+    return grid ? occ_had_t<T, true, false>(vegas, hist, sm) : occ_had_t<T, false, false>(vegas, hist, sm);
+}
+
+int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid, bool heavy)
 {
     HadLaunch L{};
     L.prec = prec; L.vegas = vegas;
     Grid g; g.nbins = nbins; g.ndim = 3;
     L.grid = &g; L.hist = hist ? (double *)1 : nullptr;
     const size_t sm = had_smem(L);
-    if (prec == MI_PREC_FP32) return grid ? occ_had_t<float, true>(vegas, hist, sm) : occ_had_t<float, false>(vegas, hist, sm);
-    return grid ? occ_had_t<double, true>(vegas, hist, sm) : occ_had_t<double, false>(vegas, hist, sm);
+    if (prec == MI_PREC_FP32) return occ_had_tt<float>(vegas, hist, sm, grid, heavy);
+    return occ_had_tt<double>(vegas, hist, sm, grid, heavy);
 }
 
 int occ_integrate_c1(bool vegas, bool hist, int nbins)
@@ -532,8 +578,10 @@ namespace mi {
 
 // ------------------------------------------------------------------------------------ scale reweighting
 // Same-sample renormalisation-scale variations of weighted events (cuPythia/pipeline/reweight.cu:36-51):
-// w_k = w * (alphaS(xi_k^2 pT^2) / alphaS(xi0^2 pT^2))^2 with pT^2 = tHat uHat / sHat (massless),
-// alphaS in Pythia's order-0/1 form (FP64).
+// w_k = w * (alphaS(xi_k^2 Q^2) / alphaS(xi0^2 Q^2))^2 with Pythia's Q^2 of the event's process
+// (SigmaProcess.cc:876-922): pT^2 = tHat uHat / sHat for the massless codes, and for the heavy
+// pairs (m3 = m4 = m) sqrt((pT^2 + m^2)^2) = (m^2 - tHat)(m^2 - uHat) / sHat with
+// uHat = 2 m^2 - sHat - tHat; alphaS in Pythia's order-0/1 form (FP64).
 __device__ __forceinline__ double alphas64(const HadConst<double> &k, double Q2)
 {
     if (k.as_order == 0.0) return k.as_fixed;
@@ -546,13 +594,16 @@ __device__ __forceinline__ double alphas64(const HadConst<double> &k, double Q2)
 }
 
 __global__ __launch_bounds__(kBlock) void k_reweight(HadConst<double> k, uint64_t n, const double *__restrict__ sH,
-                                                     const double *__restrict__ tH, const double *__restrict__ w,
-                                                     int nvar, const double *__restrict__ fac, double *__restrict__ out)
+                                                     const double *__restrict__ tH, const int32_t *__restrict__ pid,
+                                                     const double *__restrict__ w, int nvar,
+                                                     const double *__restrict__ fac, double *__restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double s = sH[i], t = tH[i], u = -(s + t);
-    const double pT2 = t * u / s;
+    const int code = pid ? pid[i] : 111;
+    const double m2 = (code == 121 || code == 122) ? k.fam[0].m2 : ((code == 123 || code == 124) ? k.fam[1].m2 : 0.0);
+    const double s = sH[i], t = tH[i], u = 2.0 * m2 - (s + t);
+    const double pT2 = (m2 - t) * (m2 - u) / s;
     const double a0 = alphas64(k, k.xiR2 * pT2);
     for (int v = 0; v < nvar; ++v) {
         const double r = alphas64(k, fac[v] * fac[v] * pT2) / a0;
@@ -561,11 +612,11 @@ __global__ __launch_bounds__(kBlock) void k_reweight(HadConst<double> k, uint64_
 }
 
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
-                           const double *w, int nvar, const double *fac, double *out)
+                           const int32_t *pid, const double *w, int nvar, const double *fac, double *out)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reweight, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, k, n, sH, tH, w,
-                       nvar, fac, out);
+    hipLaunchKernelGGL(k_reweight, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, k, n, sH, tH, pid,
+                       w, nvar, fac, out);
     return hipGetLastError();
 }
 

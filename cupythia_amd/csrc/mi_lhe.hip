@@ -5,11 +5,14 @@
 //   incoming flavours  -- a pair from the process's flux list with probability
 //                         xf1 xf2 dsigmaHat/dtHat (SigmaProcess::pickInState, SigmaProcess.cc:486-508);
 //   outgoing flavours and colour flow -- SigmaXxx::setIdColAcol of the process
-//                         (pythia8317/src/SigmaQCD.cc:132-389);
+//                         (pythia8317/src/SigmaQCD.cc:132-389; the heavy pairs 121..124 as
+//                         Sigma2gg2QQbar / Sigma2qqbar2QQbar, c cbar or b bbar);
 //   azimuth            -- uniform;
 // from the event's own Philox stream 3 (words 0..3) and stream 4 (word 0), so the file is a pure
 // function of (key, event index).  Momenta: the partonic centre-of-mass frame
-// (|p| = sqrt(sHat)/2, cos theta = 1 + 2 tHat / sHat) boosted along z to the lab (y = ln(x1/x2)/2).
+// (|p| = sqrt(sHat/4 - m^2), cos theta = (1 - 2a) / beta with a = (m^2 - tHat)/sHat and
+// beta = sqrt(1 - 4 m^2/sHat); massless: 1 + 2 tHat / sHat) boosted along z to the lab
+// (y = ln(x1/x2)/2).
 // Colour tags are offset by 500 (the LHEF habit).  Weights in pb (IDWTUP = 3).
 // Reference counterpart: cuPythia/pipeline/lhe_writer.cu:46-70, cuPythia/07_unweighting_lhe/
 // unweight_lhe.cu:74-102 (gg -> gg only, fixed colour flow).
@@ -134,7 +137,7 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
     for (int c = 1; c <= MI_MAX_PROC; ++c)           // XSECUP XERRUP XMAXUP LPRUP (Pythia code)
         if (est->sigmaGen[c] > 0.0)
             std::fprintf(f, "%.10E %.10E %.10E %d\n", est->sigmaGen[c] * mb2pb, est->sigmaErr[c] * mb2pb,
-                         est->wMax[c] * mb2pb, 110 + c);
+                         est->wMax[c] * mb2pb, MI_PROC_CODE(c - 1));
     std::fprintf(f, "</init>\n");
 
     const double pi = 3.14159265358979323846;
@@ -142,20 +145,24 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
         uint32_t r3[4], r4[4];
         philox_h(key, idx[i], 3u, r3);
         philox_h(key, idx[i], 4u, r4);
-        const double s = sH[i], t = tH[i], u = -(s + t);
         const int code = pid[i];
+        const bool heavy = code >= 121 && code <= 124;
+        const int idQ = (code == 121 || code == 122) ? 4 : 5;
+        const double mQ = heavy ? phys->mass_Q[idQ - 4] : 0.0, m2 = mQ * mQ;
+        const double s = sH[i], t = tH[i], u = 2.0 * m2 - (s + t);
+        const double Q2 = (m2 - t) * (m2 - u) / s;                   // pT^2 + m^2
         Ev4 E;
         std::memset(&E, 0, sizeof E);
         // ---- incoming flavours: pick from the flux list with weight xf1 xf2 dsigma/dt(ids)
         int id1 = 21, id2 = 21;
-        if (!partonic && code != 111 && code != 112) {
+        if (!partonic && code != 111 && code != 112 && code != 121 && code != 123) {
             std::vector<std::pair<int, int>> pairs;
             std::vector<double> wt;
             const double X1 = x1[i], X2 = x2[i];
-            // parton densities: the toy PDFs, or the grid at mu_F^2 = fact_fac^2 pT^2 (clipped at 0)
+            // parton densities: the toy PDFs, or the grid at mu_F^2 = fact_fac^2 (pT^2 + m^2) (clipped at 0)
             double g1[11] = {}, g2[11] = {};
             if (grid) {
-                const double Q2F = hg.factFac2 * (t * u / s), q = std::sqrt(Q2F);
+                const double Q2F = hg.factFac2 * Q2, q = std::sqrt(Q2F);
                 int m3q, n3q;
                 double wq[4];
                 grid_qw<double>(hg, q, std::log(q), m3q, n3q, wq);
@@ -244,8 +251,22 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
             if (id1 < 0) swapCA();
             break;
         }
-        default: {                                                   // 116
-            const int idNew = 1 + (int)(phys->nQuarkNew * un);
+        case 121:
+        case 123: {                                                  // Sigma2gg2QQbar
+            // sigTS, sigUS of the massive sigmaKin with tHQ = t - m^2, uHQ = u - m^2
+            const double tq = t - m2, uq = u - m2, tq2 = tq * tq, uq2 = uq * uq;
+            const double tum = tq * uq - m2 * s;
+            const double a = (uq / tq - 2.25 * uq2 / s2 + 4.5 * m2 * tum / (s * tq2) + 0.5 * m2 * (tq + m2) / tq2 -
+                              m2 * m2 / (s * tq)) / 6.;
+            const double b = (tq / uq - 2.25 * tq2 / s2 + 4.5 * m2 * tum / (s * uq2) + 0.5 * m2 * (uq + m2) / uq2 -
+                              m2 * m2 / (s * uq)) / 6.;
+            E.id[2] = idQ; E.id[3] = -idQ;
+            if ((a + b) * uc < a) setc(1, 2, 2, 3, 1, 0, 0, 3);
+            else setc(1, 2, 3, 1, 3, 0, 0, 2);
+            break;
+        }
+        default: {                                                   // 116, and 122 / 124 (Sigma2qqbar2QQbar)
+            const int idNew = heavy ? idQ : 1 + (int)(phys->nQuarkNew * un);
             const int id3 = id1 > 0 ? idNew : -idNew;
             E.id[2] = id3; E.id[3] = -id3;
             setc(1, 0, 0, 2, 1, 0, 0, 2);
@@ -259,27 +280,28 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
         const double X1 = partonic ? 1.0 : x1[i], X2 = partonic ? 1.0 : x2[i];
         const double pz1 = X1 * eBeam, pz2 = X2 * eBeam;
         const double sLab = 4.0 * pz1 * pz2;
-        const double pabs = 0.5 * std::sqrt(sLab);
-        const double cth = std::fmin(1.0, std::fmax(-1.0, 1.0 + 2.0 * t / s));
+        const double eCMp = 0.5 * std::sqrt(sLab);
+        const double pabs = std::sqrt(std::fmax(0.0, 0.25 * sLab - m2));
+        const double bQ = std::sqrt(std::fmax(0.0, 1.0 - 4.0 * m2 / s));
+        const double cth = std::fmin(1.0, std::fmax(-1.0, (1.0 - 2.0 * (m2 - t) / s) / (heavy ? bQ : 1.0)));
         const double sth = std::sqrt(std::fmax(0.0, 1.0 - cth * cth));
         const double phi = 2.0 * pi * uni(r4[0]);
         const double beta = (X1 - X2) / (X1 + X2), gam = 1.0 / std::sqrt(1.0 - beta * beta);
-        double p3[4] = {pabs * sth * std::cos(phi), pabs * sth * std::sin(phi), pabs * cth, pabs};
-        double p4[4] = {-p3[0], -p3[1], -p3[2], pabs};
+        double p3[4] = {pabs * sth * std::cos(phi), pabs * sth * std::sin(phi), pabs * cth, eCMp};
+        double p4[4] = {-p3[0], -p3[1], -p3[2], eCMp};
         for (double *p : {p3, p4}) {
             const double ez = gam * (p[3] + beta * p[2]), pz = gam * (p[2] + beta * p[3]);
             p[2] = pz; p[3] = ez;
         }
-        const double pT2 = t * u / s;
-        const double scale = phys->renorm_fac * std::sqrt(std::fmax(pT2, 0.0));
+        const double scale = phys->renorm_fac * std::sqrt(std::fmax(Q2, 0.0));
         std::fprintf(f, "<event>\n4 %d %.10E %.10E -1.0E+00 -1.0E+00\n", code, w[i] * mb2pb, scale);
         auto tag = [](int c) { return c ? c + 500 : 0; };
         std::fprintf(f, "%d -1 0 0 %d %d 0 0 %.10E %.10E 0 0 9\n", E.id[0], tag(E.col[0]), tag(E.acol[0]), pz1, pz1);
         std::fprintf(f, "%d -1 0 0 %d %d 0 0 %.10E %.10E 0 0 9\n", E.id[1], tag(E.col[1]), tag(E.acol[1]), -pz2, pz2);
-        std::fprintf(f, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E 0 0 9\n", E.id[2], tag(E.col[2]), tag(E.acol[2]), p3[0],
-                     p3[1], p3[2], p3[3]);
-        std::fprintf(f, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E 0 0 9\n</event>\n", E.id[3], tag(E.col[3]),
-                     tag(E.acol[3]), p4[0], p4[1], p4[2], p4[3]);
+        std::fprintf(f, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E %.10E 0 9\n", E.id[2], tag(E.col[2]), tag(E.acol[2]),
+                     p3[0], p3[1], p3[2], p3[3], mQ);
+        std::fprintf(f, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E %.10E 0 9\n</event>\n", E.id[3], tag(E.col[3]),
+                     tag(E.acol[3]), p4[0], p4[1], p4[2], p4[3], mQ);
     }
     std::fprintf(f, "</LesHouchesEvents>\n");
     const bool ok = std::fclose(f) == 0;

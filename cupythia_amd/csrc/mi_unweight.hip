@@ -41,26 +41,35 @@ __device__ __forceinline__ void decide(uint64_t key, uint64_t e, T tot, const T 
     int pid = 0;
     for (int i = 0; i < kMaxProc; ++i) {
         if (!((mask >> i) & 1) || !(ws[i] > (T)0)) continue;
-        pid = 111 + i;
+        pid = proc_code(i);
         cum = cum + ws[i];
         if (cum > target) break;
     }
     c.pid = pid;
 }
 
-template <typename T, bool VEGAS>
+// HEAVY: the event's kinematics are those of the chosen process's family (each family has its
+// own map at the same uniforms).
+template <typename T, bool VEGAS, bool HEAVY>
 __device__ __forceinline__ void had_cand(const HadConst<T> &k, int mask, uint64_t key, uint64_t e, const T *stab,
                                          int nbins, double wmax, const GridPdf<T> *gp, Cand &c)
 {
     T ws[kMaxProc];
     int bins[3];
-    T sH, tH, x1, x2;
-    const T tot = had_candidate<T, VEGAS>(k, mask, key, e, stab, nbins, (T)nbins, ws, bins, &sH, &tH, &x1, &x2, gp);
+    constexpr int NF = HEAVY ? 3 : 1;
+    T sH[NF], tH[NF], x1[NF], x2[NF];
+    const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, e, stab, nbins, (T)nbins, ws, bins, sH, tH, x1, x2, gp);
     c.w = (double)tot;
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) c.ws[i] = (double)ws[i];
-    c.sH = (double)sH; c.tH = (double)tH; c.x1 = (double)x1; c.x2 = (double)x2;
     decide<T>(key, e, tot, ws, mask, wmax, c);
+    int f = 0;
+    if (HEAVY) {
+        f = c.pid >= 123 ? 2 : (c.pid >= 121 ? 1 : 0);
+        if (c.pid == 0) f = (mask & kFamMask[0]) ? 0 : ((mask & kFamMask[1]) ? 1 : 2);
+    }
+    c.sH = (double)pick_fam<T, NF>(sH, f); c.tH = (double)pick_fam<T, NF>(tH, f);
+    c.x1 = (double)pick_fam<T, NF>(x1, f); c.x2 = (double)pick_fam<T, NF>(x2, f);
 }
 
 // Partonic gg -> gg candidate.  Philox: stream 0 for cos(theta), stream 1 for u.  Compat:
@@ -89,7 +98,12 @@ __device__ __forceinline__ void c1_cand(double pref, double cMax, double sHat, u
     c.pid = 111;
 }
 
-template <int KIND, typename T>   // KIND 0: hadronic flat, 1: hadronic VEGAS, 2: partonic
+// KIND 0: hadronic flat, 1: hadronic VEGAS, 2: partonic, 3 / 4: hadronic flat / VEGAS with the
+// heavy-quark codes selected.
+constexpr bool kind_vegas(int kind) { return kind == 1 || kind == 4; }
+constexpr bool kind_heavy(int kind) { return kind >= 3; }
+
+template <int KIND, typename T>
 struct Params {
     HadConst<T> k;
     int mask;
@@ -107,9 +121,8 @@ template <int KIND, typename T>
 __device__ __forceinline__ void candidate(const Params<KIND, T> &P, const T *stab, uint64_t e, Cand &c)
 {
     const GridPdf<T> *gp = P.gpdf.active ? &P.gpdf : nullptr;
-    if (KIND == 0) had_cand<T, false>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
-    else if (KIND == 1) had_cand<T, true>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
-    else c1_cand(P.pref, P.cMax, P.sHat, P.key, e, P.compat != 0, P.wmax, c);
+    if (KIND == 2) c1_cand(P.pref, P.cMax, P.sHat, P.key, e, P.compat != 0, P.wmax, c);
+    else had_cand<T, kind_vegas(KIND), kind_heavy(KIND)>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
 }
 
 template <int KIND, typename T>
@@ -118,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_unw_count(Params<KIND, T> P, double 
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *stab = reinterpret_cast<T *>(smem);
-    if (KIND == 1) {
+    if (kind_vegas(KIND)) {
         for (int i = threadIdx.x; i < 9 * P.nbins; i += blockDim.x) stab[i] = P.gtab[i];
         __syncthreads();
     }
@@ -131,16 +144,16 @@ __global__ __launch_bounds__(kBlock) void k_unw_count(Params<KIND, T> P, double 
         if (i >= P.n) break;
         Cand c;
         candidate<KIND, T>(P, stab, P.ctr0 + i, c);
-        v[0] += c.w;
-        v[7] = fma(c.w, c.w, v[7]);
-        v[14] = fmax(v[14], c.w);
-        v[21] += (c.w > 0.0) ? 1.0 : 0.0;
-        v[22] += c.acc ? 1.0 : 0.0;
+        v[kRS] += c.w;
+        v[kRQ] = fma(c.w, c.w, v[kRQ]);
+        v[kRM] = fmax(v[kRM], c.w);
+        v[kRNz] += (c.w > 0.0) ? 1.0 : 0.0;
+        v[kRAcc] += c.acc ? 1.0 : 0.0;
 #pragma unroll
-        for (int p = 0; p < kMaxProc; ++p) {
-            v[1 + p] += c.ws[p];
-            v[8 + p] = fma(c.ws[p], c.ws[p], v[8 + p]);
-            v[15 + p] = fmax(v[15 + p], c.ws[p]);
+        for (int p = 0; p < (KIND < 2 ? kLightProc : kMaxProc); ++p) {
+            v[kRS + 1 + p] += c.ws[p];
+            v[kRQ + 1 + p] = fma(c.ws[p], c.ws[p], v[kRQ + 1 + p]);
+            v[kRM + 1 + p] = fmax(v[kRM + 1 + p], c.ws[p]);
         }
     }
     block_record(v, partials + (size_t)blockIdx.x * kRec, counts + blockIdx.x);
@@ -174,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_unw_write(Params<KIND, T> P, const u
     extern __shared__ __align__(16) unsigned char smem[];
     T *stab = reinterpret_cast<T *>(smem);
     __shared__ uint32_t wcnt[kBlock / 64];
-    if (KIND == 1) {
+    if (kind_vegas(KIND)) {
         for (int i = threadIdx.x; i < 9 * P.nbins; i += blockDim.x) stab[i] = P.gtab[i];
         __syncthreads();
     }
@@ -242,7 +255,7 @@ hipError_t count_t(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, cons
     const Params<KIND, T> P = make_params<KIND, T>(L, k, tab);
     const Geom g = L.partonic ? L.c.g : L.h.g;
     double *parts = L.partonic ? L.c.partials : L.h.partials;
-    const size_t sm = (KIND == 1) ? sizeof(T) * 9 * P.nbins : 0;
+    const size_t sm = kind_vegas(KIND) ? sizeof(T) * 9 * P.nbins : 0;
     hipLaunchKernelGGL((k_unw_count<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, parts, counts);
     return hipGetLastError();
 }
@@ -253,9 +266,26 @@ hipError_t write_t(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, cons
 {
     const Params<KIND, T> P = make_params<KIND, T>(L, k, tab);
     const Geom g = L.partonic ? L.c.g : L.h.g;
-    const size_t sm = (KIND == 1) ? sizeof(T) * 9 * P.nbins : 0;
+    const size_t sm = kind_vegas(KIND) ? sizeof(T) * 9 * P.nbins : 0;
     hipLaunchKernelGGL((k_unw_write<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, offsets, out);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t count_h(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, uint64_t *counts)
+{
+    const bool hv = has_heavy(L.h.mask);
+    if (L.h.vegas) return hv ? count_t<4, T>(s, L, k, tab, counts) : count_t<1, T>(s, L, k, tab, counts);
+    return hv ? count_t<3, T>(s, L, k, nullptr, counts) : count_t<0, T>(s, L, k, nullptr, counts);
+}
+
+template <typename T>
+hipError_t write_h(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, const uint64_t *offsets,
+                   const mi_events &out)
+{
+    const bool hv = has_heavy(L.h.mask);
+    if (L.h.vegas) return hv ? write_t<4, T>(s, L, k, tab, offsets, out) : write_t<1, T>(s, L, k, tab, offsets, out);
+    return hv ? write_t<3, T>(s, L, k, nullptr, offsets, out) : write_t<0, T>(s, L, k, nullptr, offsets, out);
 }
 
 }  // namespace
@@ -266,11 +296,8 @@ hipError_t launch_unweight_count(hipStream_t s, const UnwLaunch &L, uint64_t *co
         HadConst<double> kd{};
         return count_t<2, double>(s, L, kd, nullptr, counts);
     }
-    if (L.h.prec == MI_PREC_FP32)
-        return L.h.vegas ? count_t<1, float>(s, L, L.h.kf, L.h.grid->tab32, counts)
-                         : count_t<0, float>(s, L, L.h.kf, nullptr, counts);
-    return L.h.vegas ? count_t<1, double>(s, L, L.h.kd, L.h.grid->tab64, counts)
-                     : count_t<0, double>(s, L, L.h.kd, nullptr, counts);
+    if (L.h.prec == MI_PREC_FP32) return count_h<float>(s, L, L.h.kf, L.h.vegas ? L.h.grid->tab32 : nullptr, counts);
+    return count_h<double>(s, L, L.h.kd, L.h.vegas ? L.h.grid->tab64 : nullptr, counts);
 }
 
 hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk)
@@ -286,10 +313,8 @@ hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64
         return write_t<2, double>(s, L, kd, nullptr, offsets, out);
     }
     if (L.h.prec == MI_PREC_FP32)
-        return L.h.vegas ? write_t<1, float>(s, L, L.h.kf, L.h.grid->tab32, offsets, out)
-                         : write_t<0, float>(s, L, L.h.kf, nullptr, offsets, out);
-    return L.h.vegas ? write_t<1, double>(s, L, L.h.kd, L.h.grid->tab64, offsets, out)
-                     : write_t<0, double>(s, L, L.h.kd, nullptr, offsets, out);
+        return write_h<float>(s, L, L.h.kf, L.h.vegas ? L.h.grid->tab32 : nullptr, offsets, out);
+    return write_h<double>(s, L, L.h.kd, L.h.vegas ? L.h.grid->tab64 : nullptr, offsets, out);
 }
 
 }  // namespace mi

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 2
+#define MI_ABI_VERSION 3
 
 typedef struct mi_ctx mi_ctx;   /* opaque */
 
@@ -47,14 +47,21 @@ typedef struct mi_ctx mi_ctx;   /* opaque */
 #define MI_ERR_NUMERIC  -5   /* non-finite result */
 #define MI_ERR_CAPACITY -6   /* output buffer too small (count still reported) */
 
-/* Pythia process codes (SigmaQCD.h): index i of per-process arrays <-> code 111 + i */
+/* Pythia process codes (SigmaQCD.h).  Slot i of per-process arrays and bit i of proc_mask:
+ * i = 0..5 <-> codes 111..116 (massless), i = 6..9 <-> codes 121..124 (heavy-quark pairs with
+ * m3 = m4 = mass_Q: HardQCD:gg2ccbar, qqbar2ccbar, gg2bbbar, qqbar2bbbar). */
 #define MI_GG2GG          111
 #define MI_GG2QQBAR       112
 #define MI_QG2QG          113
 #define MI_QQ2QQ          114
 #define MI_QQBAR2GG       115
 #define MI_QQBAR2QQBARNEW 116
-#define MI_MAX_PROC       6
+#define MI_GG2CCBAR       121
+#define MI_QQBAR2CCBAR    122
+#define MI_GG2BBBAR       123
+#define MI_QQBAR2BBBAR    124
+#define MI_MAX_PROC       10
+#define MI_PROC_CODE(i)   ((i) < 6 ? 111 + (i) : 121 + (i) - 6)
 
 enum mi_mode      { MI_MODE_PARTONIC_1D = 0, MI_MODE_HADRONIC_3D = 1 };
 enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1 };
@@ -68,7 +75,7 @@ enum mi_rng       { MI_RNG_PHILOX = 0, MI_RNG_SPLITMIX_COMPAT = 1 };
  * PDFinProcess:nQuarkIn = 5, SigmaProcess:Kfactor = 1. */
 typedef struct mi_physics {
     int    mode;            /* enum mi_mode */
-    int    proc_mask;       /* bit i <-> code 111 + i (hadronic); partonic mode is gg->gg only */
+    int    proc_mask;       /* bit i <-> code MI_PROC_CODE(i) (hadronic); partonic mode is gg->gg only */
     /* partonic 1-D: fixed sHat, fixed alphaS, cos(theta) flat in [-cos_max, cos_max] */
     double sHat;            /* GeV^2 */
     double cos_max;
@@ -86,7 +93,8 @@ typedef struct mi_physics {
     double pdf_sea[2];      /* sea (every quark and antiquark) */
     double kfactor;
     int    pdf_source;      /* enum mi_pdf_source: toy PDFs above, or the grid of mi_pdf_load */
-    double fact_fac;        /* mu_F = fact_fac * pT (PDF scale with a grid) */
+    double fact_fac;        /* mu_F = fact_fac * pT (PDF scale with a grid); heavy pairs: mu_F^2 = fact_fac^2 (pT^2 + m^2) */
+    double mass_Q[2];       /* c and b masses (GeV) of codes 121..124; defaults 1.5, 4.8 (Pythia's m0) */
 } mi_physics;
 
 typedef struct mi_run {
@@ -100,7 +108,7 @@ typedef struct mi_run {
 } mi_run;
 
 /* Pythia's sigma-estimate semantics.  Index 0 = sum over processes (Info::sigmaGen(0)),
- * index 1 + i = code 111 + i.  For weighted integration nSel = nAcc = nTry; sigmaErr follows
+ * index 1 + i = code MI_PROC_CODE(i) (code[] holds them).  For weighted integration nSel = nAcc = nTry; sigmaErr follows
  * ProcessContainer::sigmaDelta.  The sum's error uses the per-point total weight (the
  * processes share points), not the quadrature sum of independent containers. */
 typedef struct mi_sigma_est {
@@ -184,17 +192,19 @@ int mi_rng_philox_fill(mi_ctx *ctx, uint64_t key, uint64_t counter0, uint32_t st
                        uint64_t n, uint32_t *out_dev);
 
 /* ---------------------------------------------------------------- sigmaHat ---- */
-/* Batched Kfactor * sigmaHatWrap(id1, id2) (mb/GeV^2) for code 111..116, FP64, massless.
- * uH_dev may be NULL (then uH = -(sH + tH) as in Sigma2Process::store2Kin); id1/id2 may be
- * NULL (only 114 depends on them; NULL means q q' with different flavours). */
+/* Batched Kfactor * sigmaHatWrap(id1, id2) (mb/GeV^2) for code 111..116 (massless) and the
+ * heavy-quark pairs 121..124, FP64.  uH_dev may be NULL for 111..116 (then uH = -(sH + tH) as in
+ * Sigma2Process::store2Kin); 121..124 need it and take m3^2 = m4^2 = (sH + tH + uH) / 2.
+ * id1/id2 may be NULL (only 114 depends on them; NULL means q q' with different flavours). */
 int mi_sigma_hat(mi_ctx *ctx, int code, uint64_t n, const double *sH_dev, const double *tH_dev,
                  const double *uH_dev, const double *alpS_dev, const int32_t *id1_dev,
                  const int32_t *id2_dev, int nQuarkNew, double kfactor, double *out_dev);
 
 /* ----------------------------------------------------------- integration ---- */
 /* Per-point weights (mb) of events counter0 .. counter0+n_points-1: w_dev is
- * n x (MI_MAX_PROC + 1) doubles (column 0 = total, 1 + i = code 111 + i); kin_dev (may be
- * NULL) is n x 4 doubles (sHat, tHat, x1, x2).  Uses run->sampler's map (flat or the
+ * n x (MI_MAX_PROC + 1) doubles (column 0 = total, 1 + i = code MI_PROC_CODE(i)); kin_dev (may
+ * be NULL) is n x 4 doubles (sHat, tHat, x1, x2) of the first family proc_mask selects
+ * (massless, c cbar, b bbar: each has its own phase-space map at the same uniforms).  Uses run->sampler's map (flat or the
  * context's VEGAS grid).  Parity hook for the integration kernels. */
 int mi_points(mi_ctx *ctx, const mi_run *run, double *w_dev, double *kin_dev);
 /* Sigma estimate over run->n_points events; fused RNG -> map -> PDFs x sigmaHat -> sum w,
@@ -235,12 +245,15 @@ int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phys, const mi
 
 /* ------------------------------------------------------------ reweighting ---- */
 /* Renormalisation-scale variations of events in one pass: out[i * nvar + k] =
- * w[i] * (alphaS(f_k^2 pT^2) / alphaS(renorm_fac^2 pT^2))^2, pT^2 = tHat uHat / sHat (massless),
- * alphaS per phys (order 0/1, FP64).  Replaces the one-pass multi-weight stage of
+ * w[i] * (alphaS(f_k^2 Q^2) / alphaS(renorm_fac^2 Q^2))^2 with the event's Pythia scale
+ * Q^2 = tHat uHat / sHat for codes 111..116 and (m^2 - tHat)(m^2 - uHat) / sHat = pT^2 + m^2 for
+ * the heavy pairs 121..124 (m = phys->mass_Q, uHat = 2 m^2 - sHat - tHat); alphaS per phys
+ * (order 0/1, FP64).  pid_dev (may be NULL: all massless) is the events' process codes, as
+ * mi_unweight_compact writes them.  Replaces the one-pass multi-weight stage of
  * cuPythia/pipeline/reweight.cu:36-51 and generate.cu:49-54.  nvar <= 64. */
 int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev,
-                       const double *tHat_dev, const double *w_dev, int nvar, const double *factors_host,
-                       double *out_dev);
+                       const double *tHat_dev, const int32_t *pid_dev, const double *w_dev, int nvar,
+                       const double *factors_host, double *out_dev);
 
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the

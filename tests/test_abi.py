@@ -53,9 +53,11 @@ def test_host_only_entry_points():
     import cupythia_amd as mi
     from cupythia_amd import _abi
     lib = _abi.load()
-    assert lib.mi_version() == _abi.MI_ABI_VERSION == 2
+    assert lib.mi_version() == _abi.MI_ABI_VERSION == 3
     p = mi.physics_default(1)
-    assert p.pdf_source == mi.MI_PDF_TOY and p.fact_fac == 1.0
+    assert p.pdf_source == mi.MI_PDF_TOY and p.fact_fac == 1.0 and tuple(p.mass_Q) == (1.5, 4.8)
+    assert mi.PROC_CODES == (111, 112, 113, 114, 115, 116, 121, 122, 123, 124) and mi.MI_MAX_PROC == 10
+    assert [cls._code for cls in mi.PROCESSES] == list(mi.PROC_CODES)
     assert p.mode == mi.MI_MODE_HADRONIC_3D and p.proc_mask == 0x1F and p.eCM == 13600.0
     assert p.pTHatMin == 20.0 and p.alphaS_value == 0.118 and p.alphaS_order == 1
     assert tuple(p.pdf_g) == (-0.1, 5.0) and tuple(p.pdf_val) == (0.5, 3.0) and tuple(p.pdf_sea) == (-0.2, 7.0)
