@@ -198,6 +198,15 @@ def workloads(ctx, mi):
     out["pdf_grid"] = {"points": 1 << 26, "kernel_ms": ge.kernel_ms, "points_per_s": (1 << 26) / (ge.kernel_ms / 1e3),
                        "sigma_mb": ge.sigmaGen(), "sigma_err_mb": ge.sigmaErr(),
                        "grid": "100 x-nodes (NNPDF2.3 layout) x 50 Q-nodes, 11 flavours, FP32 Lagrange interpolation"}
+    # the ten-process library with the heavy-quark pairs (codes 121..124: own massive map per
+    # family at the same uniforms), flat map, toy PDFs
+    hphys = mi.physics_default(1, proc_mask=0x3FF)
+    he = ctx.integrate(mi.make_run(hphys, 1 << 26, key=49))
+    out["heavy_quarks"] = {"points": 1 << 26, "kernel_ms": he.kernel_ms,
+                           "points_per_s": (1 << 26) / (he.kernel_ms / 1e3), "sigma_mb": he.sigmaGen(),
+                           "sigma_err_mb": he.sigmaErr(),
+                           "sigma_heavy_mb": {str(c): he.sigmaGen(c) for c in (121, 122, 123, 124)},
+                           "processes": "111-116 + gg->ccbar, qqbar->ccbar, gg->bbbar, qqbar->bbbar (m_c 1.5, m_b 4.8)"}
     # flow on top of the adapted grid
     ctx.flow_init(7)
     steps, batch = 40, 1 << 20

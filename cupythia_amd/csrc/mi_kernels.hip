@@ -399,52 +399,52 @@ hipError_t launch_points_c1(hipStream_t s, const C1Launch &L, double sHat, doubl
 }
 
 // ------------------------------------------------------------------------------------ reductions
-// Fold nblk block records into one, in a fixed order: thread t takes records t, t+256, ...
-// sequentially, then a fixed LDS tree.
+// Column c of an nrow x stride row-major matrix folded in a fixed order: thread t takes rows
+// t, t+256, ... sequentially, then a fixed LDS tree.  Sum, or max for the max slots.  One block
+// per column, so the columns fold in parallel (each one's order is that of a single block).
+__device__ __forceinline__ void fold_column(const double *__restrict__ parts, uint32_t nrow, uint32_t stride,
+                                            uint32_t c, bool isMax, double *__restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nrow; b += kBlock) {
+        const double x = parts[(size_t)b * stride + c];
+        acc = isMax ? fmax(acc, x) : acc + x;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kBlock / 2; off >= 1; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] = isMax ? fmax(sh[threadIdx.x], sh[threadIdx.x + off]) : sh[threadIdx.x] + sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Fold nblk block records into one: block i folds slot i.
 __global__ __launch_bounds__(kBlock) void k_reduce_records(const double *__restrict__ parts, uint32_t nblk,
                                                            double *__restrict__ out)
 {
-    __shared__ double sh[kBlock];
-    for (int i = 0; i < kRec; ++i) {
-        const bool isMax = rec_is_max(i);
-        double acc = 0.0;
-        for (uint32_t b = threadIdx.x; b < nblk; b += kBlock) {
-            const double x = parts[(size_t)b * kRec + i];
-            acc = isMax ? fmax(acc, x) : acc + x;
-        }
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int off = kBlock / 2; off >= 1; off >>= 1) {
-            if ((int)threadIdx.x < off)
-                sh[threadIdx.x] = isMax ? fmax(sh[threadIdx.x], sh[threadIdx.x + off]) : sh[threadIdx.x] + sh[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[i] = sh[0];
-        __syncthreads();
-    }
+    const uint32_t i = blockIdx.x;
+    fold_column(parts, nblk, kRec, i, rec_is_max((int)i), out + i);
 }
 
 hipError_t launch_reduce_records(hipStream_t s, const double *partials, uint32_t nblk, double *out)
 {
-    hipLaunchKernelGGL(k_reduce_records, dim3(1), dim3(kBlock), 0, s, partials, nblk, out);
+    hipLaunchKernelGGL(k_reduce_records, dim3(kRec), dim3(kBlock), 0, s, partials, nblk, out);
     return hipGetLastError();
 }
 
-// Column sums of an nblk x ncol matrix, each column summed sequentially over blocks.
+// Column sums of an nblk x ncol matrix (the VEGAS histograms of the blocks): block c folds column c.
 __global__ __launch_bounds__(kBlock) void k_reduce_columns(const double *__restrict__ parts, uint32_t nblk,
                                                            uint32_t ncol, double *__restrict__ out)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) s += parts[(size_t)b * ncol + c];
-    out[c] = s;
+    fold_column(parts, nblk, ncol, blockIdx.x, false, out + blockIdx.x);
 }
 
 hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nblk, uint32_t ncol, double *out)
 {
-    hipLaunchKernelGGL(k_reduce_columns, dim3((ncol + kBlock - 1) / kBlock), dim3(kBlock), 0, s, parts, nblk, ncol,
-                       out);
+    hipLaunchKernelGGL(k_reduce_columns, dim3(ncol), dim3(kBlock), 0, s, parts, nblk, ncol, out);
     return hipGetLastError();
 }
 
@@ -452,50 +452,68 @@ hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nb
 // Lepage, J. Comput. Phys. 27 (1978) 192: per dimension, d_i = sum of w^2 in bin i, smoothed
 // over neighbours (ends averaged with one neighbour), normalised, damped
 // m_i = ((1 - d_i) / -ln d_i)^alpha, then the edges are moved so every new bin holds an equal
-// share of sum m_i, interpolating linearly inside the old bins.  One thread per dimension.
-__global__ void k_vegas_refine(double *__restrict__ edges, const double *__restrict__ hist, int ndim, int nbins,
-                               double alpha)
+// share of sum m_i, interpolating linearly inside the old bins.  One block per dimension: the
+// histogram and edges are staged in LDS, the per-bin smoothing and damping run one thread per
+// bin, the two sums and the edge walk run in one thread in bin order.
+__global__ __launch_bounds__(kMaxBins) void k_vegas_refine(double *__restrict__ edges, const double *__restrict__ hist,
+                                                           int ndim, int nbins, double alpha)
 {
-    __shared__ double sm[3][kMaxBins];
-    __shared__ double sne[3][kMaxBins + 1];
-    const int d = threadIdx.x;
+    __shared__ double h[kMaxBins], m[kMaxBins], x[kMaxBins + 1], ne[kMaxBins + 1];
+    __shared__ int go;
+    const int d = blockIdx.x, i = threadIdx.x;
     if (d >= ndim) return;
-    double *m = sm[d], *ne = sne[d];
-    const double *h = hist + (size_t)d * nbins;
-    double *x = edges + (size_t)d * (nbins + 1);
-    double tot = 0.0;
-    for (int i = 0; i < nbins; ++i) {
-        double s;
+    const double *hd = hist + (size_t)d * nbins;
+    double *xd = edges + (size_t)d * (nbins + 1);
+    if (i < nbins) h[i] = hd[i];
+    for (int j = i; j <= nbins; j += blockDim.x) x[j] = xd[j];
+    __syncthreads();
+    double s = 0.0;
+    if (i < nbins) {
         if (nbins == 1) s = h[0];
         else if (i == 0) s = 0.5 * (h[0] + h[1]);
         else if (i == nbins - 1) s = 0.5 * (h[nbins - 2] + h[nbins - 1]);
         else s = (h[i - 1] + h[i] + h[i + 1]) / 3.0;
         m[i] = s;
-        tot += s;
     }
-    if (!(tot > 0.0)) return;                                  // nothing to adapt to
-    double msum = 0.0;
-    for (int i = 0; i < nbins; ++i) {
-        const double r = m[i] / tot;
+    __syncthreads();
+    if (i == 0) {
+        double tot = 0.0;
+        for (int j = 0; j < nbins; ++j) tot += m[j];
+        go = tot > 0.0;
+        h[0] = tot;                                                // the total, for every thread
+    }
+    __syncthreads();
+    if (!go) return;                                               // nothing to adapt to
+    const double tot = h[0];
+    if (i < nbins) {
+        const double r = s / tot;
         double mi = 0.0;
         if (r > 0.0) mi = (r < 1.0) ? pow((1.0 - r) / -log(r), alpha) : 1.0;
         m[i] = mi;
-        msum += mi;
     }
-    if (!(msum > 0.0)) return;
-    const double step = msum / nbins;
-    ne[0] = x[0];
-    ne[nbins] = x[nbins];
-    int old = 0;
-    double acc = 0.0;                                          // sum of m over bins < old
-    for (int j = 1; j < nbins; ++j) {
-        const double target = step * j;
-        while (old < nbins - 1 && acc + m[old] < target) { acc += m[old]; ++old; }
-        double frac = (m[old] > 0.0) ? (target - acc) / m[old] : 0.0;
-        frac = fmin(fmax(frac, 0.0), 1.0);
-        ne[j] = x[old] + frac * (x[old + 1] - x[old]);
+    __syncthreads();
+    if (i == 0) {
+        double msum = 0.0;
+        for (int j = 0; j < nbins; ++j) msum += m[j];
+        go = msum > 0.0;
+        if (go) {
+            const double step = msum / nbins;
+            ne[0] = x[0];
+            ne[nbins] = x[nbins];
+            int old = 0;
+            double acc = 0.0;                                      // sum of m over bins < old
+            for (int j = 1; j < nbins; ++j) {
+                const double target = step * j;
+                while (old < nbins - 1 && acc + m[old] < target) { acc += m[old]; ++old; }
+                double frac = (m[old] > 0.0) ? (target - acc) / m[old] : 0.0;
+                frac = fmin(fmax(frac, 0.0), 1.0);
+                ne[j] = x[old] + frac * (x[old + 1] - x[old]);
+            }
+        }
     }
-    for (int j = 0; j <= nbins; ++j) x[j] = ne[j];
+    __syncthreads();
+    if (!go) return;
+    for (int j = i; j <= nbins; j += blockDim.x) xd[j] = ne[j];
 }
 
 // Per-bin tables {lo, width, 1 - lo} in FP64 and FP32 from the FP64 edges.
@@ -512,7 +530,7 @@ __global__ void k_vegas_tables(const double *__restrict__ edges, int ndim, int n
 
 hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, double alpha)
 {
-    hipLaunchKernelGGL(k_vegas_refine, dim3(1), dim3(64), 0, s, g.edges, hist_sum, g.ndim, g.nbins, alpha);
+    hipLaunchKernelGGL(k_vegas_refine, dim3(g.ndim), dim3(kMaxBins), 0, s, g.edges, hist_sum, g.ndim, g.nbins, alpha);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_vegas_tables(s, g);
