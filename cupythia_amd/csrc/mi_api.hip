@@ -4,6 +4,7 @@
 // (pythia8317/src/ProcessContainer.cc:1416-1472, pythia8317/src/ProcessLevel.cc:428-472).
 #include "mi_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -308,6 +309,7 @@ int mi_ctx_destroy(mi_ctx *ctx)
     mi_flow_release(ctx);
     pdf_release(ctx);
     (void)hipFree(ctx->partials); (void)hipFree(ctx->hist); (void)hipFree(ctx->red); (void)hipFree(ctx->counts);
+    (void)hipFree(ctx->masks); (void)hipFree(ctx->wbuf);
     (void)hipFree(ctx->grid.edges); (void)hipFree(ctx->grid.tab32); (void)hipFree(ctx->grid.tab64);
     (void)hipEventDestroy(ctx->ev0); (void)hipEventDestroy(ctx->ev1); (void)hipEventDestroy(ctx->ev2);
     delete ctx;
@@ -584,55 +586,108 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
     if (run->sampler == MI_SAMPLER_FLOW) return fail(ctx, MI_ERR_ARG, "unweighting supports the flat and VEGAS maps");
     if (std::isnan(wmax)) return fail(ctx, MI_ERR_ARG, "wmax is NaN");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    const Geom g = geom_for(run->n_points, 16, (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 4);
-    rc = ensure(ctx, &ctx->partials, &ctx->partials_cap, (size_t)g.blocks * kRec);
-    if (rc) return rc;
-    rc = ensure(ctx, &ctx->counts, &ctx->counts_cap, (size_t)g.blocks + 1);
+    // Candidates in chunks of at most kUnwChunk (the stored weights of a chunk: 4 or 8 B each).
+    const uint64_t n = run->n_points;
+    const size_t tsz = (partonic || run->precision == MI_PREC_FP64) ? sizeof(double) : sizeof(float);
+    const uint64_t chunk = std::min<uint64_t>(n, kUnwChunk);
+    const uint64_t nch = (n + chunk - 1) / chunk;
+    const uint32_t ncu = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256);
+    rc = ensure(ctx, &ctx->wbuf, &ctx->wbuf_cap, (size_t)chunk * tsz);
     if (rc) return rc;
     UnwLaunch L{};
     L.partonic = partonic;
     L.compat = compat;
+    L.out = out;
+    L.mat_blocks = ncu * 16;
     const mi_physics &p = run->phys;
     if (partonic) {
-        L.c.vegas = false; L.c.key = run->key; L.c.ctr0 = run->counter0; L.c.n = run->n_points; L.c.g = g;
+        L.c.vegas = false; L.c.key = run->key;
         L.c.pref = c1_pref(p); L.c.cMax = p.cos_max; L.c.grid = &ctx->grid; L.c.partials = ctx->partials;
         L.sHat = p.sHat;
     } else {
         L.h.prec = run->precision; L.h.vegas = run->sampler == MI_SAMPLER_VEGAS; L.h.mask = p.proc_mask;
-        L.h.key = run->key; L.h.ctr0 = run->counter0; L.h.n = run->n_points; L.h.g = g;
+        L.h.key = run->key;
         fill_had_consts_f32(p, L.h.kf); fill_had_consts_f64(p, L.h.kd);
         rc = pdf_views(ctx, p, &L.h.gf, &L.h.gd);
         if (rc) return rc;
         L.h.grid = &ctx->grid; L.h.partials = ctx->partials;
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    // one wave of equal-work blocks of the eval kernel; flags and compact share its layout
+    const uint32_t maxb = ncu * (uint32_t)occ_unweight_eval(L);
+    const Geom g0 = geom_for(chunk, 16, maxb);
+    rc = ensure(ctx, &ctx->partials, &ctx->partials_cap, (size_t)g0.blocks * kRec);
+    if (rc) return rc;
+    rc = ensure(ctx, &ctx->counts, &ctx->counts_cap, (size_t)g0.blocks + 1);
+    if (rc) return rc;
+    rc = ensure(ctx, &ctx->masks, &ctx->masks_cap, (size_t)g0.blocks * g0.ppt * (kBlock / 64));
+    if (rc) return rc;
+    L.masks = ctx->masks;
+    L.counts = ctx->counts;
+    if (partonic) L.c.partials = ctx->partials;
+    else L.h.partials = ctx->partials;
+    auto set_chunk = [&](uint64_t k) -> Geom {
+        const uint64_t c0 = run->counter0 + k * chunk, nc = std::min<uint64_t>(chunk, n - k * chunk);
+        const Geom g = geom_for(nc, 16, maxb);
+        if (partonic) { L.c.ctr0 = c0; L.c.n = nc; L.c.g = g; }
+        else          { L.h.ctr0 = c0; L.h.n = nc; L.h.g = g; }
+        return g;
+    };
+    // The estimate record over all candidates: chunk records folded in chunk order.
+    double R[kRec];
+    bool haveR = false;
+    auto fold = [&](const double *r) {
+        for (int i = 0; i < kRec; ++i) R[i] = !haveR ? r[i] : (rec_is_max(i) ? std::fmax(R[i], r[i]) : R[i] + r[i]);
+        haveR = true;
+    };
     double r[kRec];
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    bool stored = false;                       // chunk 0's weights already in wbuf
     if (!(wmax > 0.0)) {
-        // exact maximum over the same candidates: a count pass that accepts nothing
-        L.wmax = HUGE_VAL;
-        HIPCHK(ctx, launch_unweight_count(ctx->stream, L, ctx->counts));
-        HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, ctx->red));
-        HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        wmax = r[kRM];
+        // the exact maximum over all candidates first (a single chunk keeps its weights)
+        for (uint64_t k = 0; k < nch; ++k) {
+            const Geom g = set_chunk(k);
+            L.w = (nch == 1) ? (void *)ctx->wbuf : nullptr;
+            HIPCHK(ctx, launch_unweight_eval(ctx->stream, L));
+            HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, ctx->red));
+            HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            fold(r);
+        }
+        stored = nch == 1;
+        wmax = R[kRM];
         if (!(wmax > 0.0)) return fail(ctx, MI_ERR_NUMERIC, "all candidate weights are zero");
     }
+    const bool recs_done = haveR;
     L.wmax = wmax;
-    HIPCHK(ctx, launch_unweight_count(ctx->stream, L, ctx->counts));
-    HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, ctx->red));
-    HIPCHK(ctx, launch_scan_counts(ctx->stream, ctx->counts, g.blocks));
-    HIPCHK(ctx, launch_unweight_write(ctx->stream, L, ctx->counts, *out));
-    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    L.w = ctx->wbuf;
     uint64_t total = 0;
-    HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&total, ctx->counts + g.blocks, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint64_t k = 0; k < nch; ++k) {
+        const Geom g = set_chunk(k);
+        L.base = total;
+        if (!stored) {
+            HIPCHK(ctx, launch_unweight_eval(ctx->stream, L));
+            if (!recs_done) {
+                HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, ctx->red));
+                HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+        HIPCHK(ctx, launch_unweight_flags(ctx->stream, L));
+        HIPCHK(ctx, launch_scan_counts(ctx->stream, ctx->counts, g.blocks));
+        HIPCHK(ctx, launch_unweight_compact(ctx->stream, L));
+        HIPCHK(ctx, launch_unweight_materialize(ctx->stream, L));
+        if (k + 1 == nch) HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        uint64_t ck = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&ck, ctx->counts + g.blocks, sizeof ck, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (!stored && !recs_done) fold(r);
+        total += ck;
+    }
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (n_acc) *n_acc = total;
     if (wmax_used) *wmax_used = wmax;
     if (est) {
-        fill_estimate(r, (int64_t)run->n_points, partonic, p.proc_mask, ms, est);
+        fill_estimate(R, (int64_t)run->n_points, partonic, p.proc_mask, ms, est);
         est->nAcc = (int64_t)total;
     }
     if (total > out->capacity)

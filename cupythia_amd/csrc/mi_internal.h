@@ -31,6 +31,7 @@ __host__ __device__ constexpr bool has_heavy(int mask) { return (mask & (kFamMas
 // Pythia process code of slot i.
 __host__ __device__ constexpr int proc_code(int i) { return i < kLightProc ? 111 + i : 121 + (i - kLightProc); }
 constexpr int kBlock = 256;
+constexpr uint64_t kUnwChunk = 1ULL << 27;   // unweighting candidates per chunk (stored weights)
 constexpr int kMaxBins = 256;
 
 struct Grid {
@@ -55,6 +56,8 @@ struct mi_ctx {
     double *hist = nullptr;      size_t hist_cap = 0;       // doubles
     double *red = nullptr;       size_t red_cap = 0;        // doubles (reduced records)
     uint64_t *counts = nullptr;  size_t counts_cap = 0;     // per-block counts / offsets
+    uint64_t *masks = nullptr;   size_t masks_cap = 0;      // unweighting accept ballots
+    unsigned char *wbuf = nullptr; size_t wbuf_cap = 0;     // unweighting per-candidate weights (bytes)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     mi::Grid grid;
     mi::FlowState *flow = nullptr;
@@ -133,6 +136,7 @@ hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, d
 hipError_t launch_vegas_tables(hipStream_t s, Grid &g);
 
 // Unweighting (mi_unweight.hip).
+// One chunk of candidates [ctr0, ctr0 + n) (h or c carries them and the chunk's geometry).
 struct UnwLaunch {
     HadLaunch h;          // hadronic candidates (h.partials receives the estimate partials)
     bool partonic;        // partonic gg->gg (C1 integrand) instead
@@ -140,11 +144,19 @@ struct UnwLaunch {
     double sHat;          // partonic
     bool compat;          // SplitMix64 reference-compat stream (partonic only)
     double wmax;
+    void *w;              // per-candidate total weights (T), chunk-local index; eval may take null
+    uint64_t *masks;      // blocks x ppt x 4 wave ballots
+    uint64_t *counts;     // blocks + 1: accepted per block, then scanned (counts[blocks] = total)
+    uint64_t base;        // rank of the chunk's first accepted event
+    uint32_t mat_blocks;  // grid of the materialize launch (grid-stride over the accepted events)
+    const mi_events *out;
 };
-hipError_t launch_unweight_count(hipStream_t s, const UnwLaunch &L, uint64_t *counts);
+hipError_t launch_unweight_eval(hipStream_t s, const UnwLaunch &L);
+int occ_unweight_eval(const UnwLaunch &L);   // resident blocks per CU of the eval kernel
+hipError_t launch_unweight_flags(hipStream_t s, const UnwLaunch &L);
 hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk);
-hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64_t *offsets,
-                                 const mi_events &out);
+hipError_t launch_unweight_compact(hipStream_t s, const UnwLaunch &L);
+hipError_t launch_unweight_materialize(hipStream_t s, const UnwLaunch &L);
 
 // Renormalisation-scale reweighting of events (mi_kernels.hip).
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,

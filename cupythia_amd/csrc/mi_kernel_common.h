@@ -4,6 +4,8 @@
 
 #include "mi_internal.h"
 
+#include <type_traits>
+
 namespace mi {
 
 // ------------------------------------------------------------------------------------ utils
@@ -42,6 +44,14 @@ __device__ __forceinline__ void block_record(const double *v, double *dst, uint6
         if (cnt && i == kRAcc) *cnt = (uint64_t)r;
     }
 }
+
+// Running maximum of non-negative weights: FP32 as the integer order of the bit patterns, FP64 as
+// fmax; MaxOf<T> is the running value's type.  Converted to FP64 once, exactly.
+__device__ __forceinline__ uint32_t run_max(uint32_t m, float w) { return max(m, __float_as_uint(w)); }
+__device__ __forceinline__ double run_max(double m, double w) { return fmax(m, w); }
+template <typename T> __device__ __forceinline__ double max_value(uint32_t m) { return (double)__uint_as_float(m); }
+template <typename T> __device__ __forceinline__ double max_value(double m) { return m; }
+template <typename T> using MaxOf = typename std::conditional<sizeof(T) == 4, uint32_t, double>::type;
 
 // a[f] of a family array (f < N) as selects, so a register array never needs a dynamic index.
 template <typename T, int N>

@@ -117,11 +117,6 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
     return hipGetLastError();
 }
 
-// Running maximum of non-negative weights: FP32 as the integer order of the bit patterns, FP64 as fmax.
-__device__ __forceinline__ uint32_t run_max(uint32_t m, float w) { return max(m, __float_as_uint(w)); }
-__device__ __forceinline__ double run_max(double m, double w) { return fmax(m, w); }
-template <typename T> __device__ __forceinline__ double max_value(uint32_t m) { return (double)__uint_as_float(m); }
-template <typename T> __device__ __forceinline__ double max_value(double m) { return m; }
 
 // ------------------------------------------------------------------------------------ hadronic integration
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.

@@ -5,13 +5,18 @@
 // Pythia ProcessContainer.cc:437), then a stable filter of the accepted candidates
 // (cub::DeviceSelect::Flagged semantics, cub_compaction.cu:76-80, generate.cu:98-103).
 //
-// Three launches, no spin-waits:
-//   count:  the integrand of every candidate, accept flags, per-block accepted counts and the
-//           weighted-estimate partials;
-//   scan:   exclusive scan of the block counts (one block);
-//   write:  the integrand again (cheaper than storing and re-reading every candidate), per-step
-//           wave ballot + popcount for lane ranks, LDS prefix over the 4 waves, and a
-//           contiguous, coalesced SoA write of the accepted events at their global rank.
+// Six launches per chunk of candidates, no spin-waits or atomics; the integrand runs once per candidate
+// and once more per accepted event:
+//   eval:        the integrand of every candidate; its total weight is stored (FP32 or FP64, 4 or
+//                8 B per candidate) and the weighted-estimate partials are written per block;
+//   flags:       the accept uniform (Philox stream 1) against the stored weight: a 64-bit
+//                wave ballot per 64 candidates; then the accepted count of each eval block from
+//                its ballots;
+//   scan:        exclusive scan of the block counts (one block);
+//   compact:     the candidate index of every accepted event at its global rank (the block's
+//                ballots as a bitset in candidate order, popcounts scanned over the block);
+//   materialize: one thread per accepted event re-evaluates its candidate (same arithmetic, so the
+//                same weights) for the kinematics and the process choice, and writes the SoA.
 // Block b owns candidates [b*256*ppt, (b+1)*256*ppt) and walks them in 256-wide steps, so the
 // global rank order is the candidate order.
 #include "mi_kernel_common.h"
@@ -22,14 +27,13 @@ namespace {
 
 struct Cand {
     double w;          // total weight (mb), the FP32 value promoted exactly in FP32 mode
-    double ws[kMaxProc];
     double sH, tH, x1, x2;
     bool acc;
     int pid;
 };
 
-// Accept stream: Philox stream 1 of the same event: u = 53-bit uniform from words (0, 1),
-// process choice from the 24-bit uniform of word 2 against the cumulative FP32/FP64 weights.
+// Process choice from the 24-bit uniform of word 2 of the accept stream (Philox stream 1)
+// against the cumulative FP32/FP64 weights; the accept decision from words 0, 1.
 template <typename T>
 __device__ __forceinline__ void decide(uint64_t key, uint64_t e, T tot, const T *ws, int mask, double wmax, Cand &c)
 {
@@ -48,60 +52,12 @@ __device__ __forceinline__ void decide(uint64_t key, uint64_t e, T tot, const T 
     c.pid = pid;
 }
 
-// HEAVY: the event's kinematics are those of the chosen process's family (each family has its
-// own map at the same uniforms).
-template <typename T, bool VEGAS, bool HEAVY>
-__device__ __forceinline__ void had_cand(const HadConst<T> &k, int mask, uint64_t key, uint64_t e, const T *stab,
-                                         int nbins, double wmax, const GridPdf<T> *gp, Cand &c)
-{
-    T ws[kMaxProc];
-    int bins[3];
-    constexpr int NF = HEAVY ? 3 : 1;
-    T sH[NF], tH[NF], x1[NF], x2[NF];
-    const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, e, stab, nbins, (T)nbins, ws, bins, sH, tH, x1, x2, gp);
-    c.w = (double)tot;
-#pragma unroll
-    for (int i = 0; i < kMaxProc; ++i) c.ws[i] = (double)ws[i];
-    decide<T>(key, e, tot, ws, mask, wmax, c);
-    int f = 0;
-    if (HEAVY) {
-        f = c.pid >= 123 ? 2 : (c.pid >= 121 ? 1 : 0);
-        if (c.pid == 0) f = (mask & kFamMask[0]) ? 0 : ((mask & kFamMask[1]) ? 1 : 2);
-    }
-    c.sH = (double)pick_fam<T, NF>(sH, f); c.tH = (double)pick_fam<T, NF>(tH, f);
-    c.x1 = (double)pick_fam<T, NF>(x1, f); c.x2 = (double)pick_fam<T, NF>(x2, f);
-}
-
-// Partonic gg -> gg candidate.  Philox: stream 0 for cos(theta), stream 1 for u.  Compat:
-// the reference kernel-10 stream, ctr = seed + e * 0x100000001B3, draws cos(theta) then u.
-__device__ __forceinline__ void c1_cand(double pref, double cMax, double sHat, uint64_t key, uint64_t e, bool compat,
-                                        double wmax, Cand &c)
-{
-    double u, uacc;
-    if (compat) {
-        const uint64_t ctr = key + e * 0x100000001B3ULL;
-        u = u01(splitmix64(ctr));
-        uacc = u01(splitmix64(ctr + 1));
-    } else {
-        const U4 r = philox_event(key, e, 0u);
-        u = u53(r.x, r.y);
-        const U4 a = philox_event(key, e, 1u);
-        uacc = u53(a.x, a.y);
-    }
-    const double w = c1_weight(pref, cMax, u);
-    const double cth = cMax * (2.0 * u - 1.0);
-    c.w = w;
-    for (int i = 0; i < kMaxProc; ++i) c.ws[i] = 0.0;
-    c.ws[0] = w;
-    c.sH = sHat; c.tH = -0.5 * sHat * (1.0 - cth); c.x1 = 1.0; c.x2 = 1.0;
-    c.acc = uacc * wmax < w;
-    c.pid = 111;
-}
-
-// KIND 0: hadronic flat, 1: hadronic VEGAS, 2: partonic, 3 / 4: hadronic flat / VEGAS with the
-// heavy-quark codes selected.
-constexpr bool kind_vegas(int kind) { return kind == 1 || kind == 4; }
-constexpr bool kind_heavy(int kind) { return kind >= 3; }
+// KIND: kPartonic (partonic gg -> gg), or for the hadronic integrand the bits kVegas (VEGAS map),
+// kHeavy (heavy-quark codes selected), kGrid (PDF grid instead of the toy PDFs).
+constexpr int kVegas = 1, kHeavy = 2, kGrid = 4, kPartonic = 8;
+constexpr bool kind_vegas(int kind) { return kind != kPartonic && (kind & kVegas); }
+constexpr bool kind_heavy(int kind) { return kind != kPartonic && (kind & kHeavy); }
+constexpr bool kind_grid(int kind) { return kind != kPartonic && (kind & kGrid); }
 
 template <int KIND, typename T>
 struct Params {
@@ -117,46 +73,185 @@ struct Params {
     GridPdf<T> gpdf;       // PDF grid (active = 0: toy PDFs)
 };
 
-template <int KIND, typename T>
-__device__ __forceinline__ void candidate(const Params<KIND, T> &P, const T *stab, uint64_t e, Cand &c)
+// Partonic gg -> gg: the cos(theta) uniform (Philox stream 0; compat: the reference kernel-10
+// stream, ctr = seed + e * 0x100000001B3, first draw) and the accept uniform (stream 1; compat:
+// the second draw).
+__device__ __forceinline__ double c1_u(uint64_t key, uint64_t e, bool compat)
 {
-    const GridPdf<T> *gp = P.gpdf.active ? &P.gpdf : nullptr;
-    if (KIND == 2) c1_cand(P.pref, P.cMax, P.sHat, P.key, e, P.compat != 0, P.wmax, c);
-    else had_cand<T, kind_vegas(KIND), kind_heavy(KIND)>(P.k, P.mask, P.key, e, stab, P.nbins, P.wmax, gp, c);
+    if (compat) return u01(splitmix64(key + e * 0x100000001B3ULL));
+    const U4 r = philox_event(key, e, 0u);
+    return u53(r.x, r.y);
+}
+__device__ __forceinline__ double c1_uacc(uint64_t key, uint64_t e, bool compat)
+{
+    if (compat) return u01(splitmix64(key + e * 0x100000001B3ULL + 1));
+    const U4 a = philox_event(key, e, 1u);
+    return u53(a.x, a.y);
+}
+
+// Total weight (and per-process weights) of candidate e.
+template <int KIND, typename T>
+__device__ __forceinline__ T eval_cand(const Params<KIND, T> &P, const T *stab, uint64_t e, T *ws)
+{
+    if (KIND == kPartonic) {
+        const double w = c1_weight(P.pref, P.cMax, c1_u(P.key, e, P.compat != 0));
+#pragma unroll
+        for (int i = 0; i < kMaxProc; ++i) ws[i] = (T)0;
+        ws[0] = (T)w;
+        return (T)w;
+    }
+    int bins[3];
+    const GridPdf<T> *gp = kind_grid(KIND) ? &P.gpdf : nullptr;
+    return had_candidate<T, kind_vegas(KIND), kind_heavy(KIND)>(P.k, P.mask, P.key, e, stab, P.nbins, (T)P.nbins, ws,
+                                                                bins, (T *)nullptr, (T *)nullptr, (T *)nullptr,
+                                                                (T *)nullptr, gp);
+}
+
+// The full candidate for an accepted event: weights, kinematics of the chosen process's family
+// (each family has its own map at the same uniforms), accept flag and process code.
+template <int KIND, typename T>
+__device__ __forceinline__ void full_cand(const Params<KIND, T> &P, const T *stab, uint64_t e, Cand &c)
+{
+    if (KIND == kPartonic) {
+        const bool compat = P.compat != 0;
+        const double u = c1_u(P.key, e, compat);
+        const double w = c1_weight(P.pref, P.cMax, u);
+        const double cth = P.cMax * (2.0 * u - 1.0);
+        c.w = w;
+        c.sH = P.sHat; c.tH = -0.5 * P.sHat * (1.0 - cth); c.x1 = 1.0; c.x2 = 1.0;
+        c.acc = c1_uacc(P.key, e, compat) * P.wmax < w;
+        c.pid = 111;
+        return;
+    }
+    constexpr bool HEAVY = kind_heavy(KIND);
+    T ws[kMaxProc];
+    int bins[3];
+    constexpr int NF = HEAVY ? 3 : 1;
+    T sH[NF], tH[NF], x1[NF], x2[NF];
+    const GridPdf<T> *gp = kind_grid(KIND) ? &P.gpdf : nullptr;
+    const T tot = had_candidate<T, kind_vegas(KIND), HEAVY>(P.k, P.mask, P.key, e, stab, P.nbins, (T)P.nbins, ws,
+                                                            bins, sH, tH, x1, x2, gp);
+    c.w = (double)tot;
+    decide<T>(P.key, e, tot, ws, P.mask, P.wmax, c);
+    int f = 0;
+    if (HEAVY) {
+        f = c.pid >= 123 ? 2 : (c.pid >= 121 ? 1 : 0);
+        if (c.pid == 0) f = (P.mask & kFamMask[0]) ? 0 : ((P.mask & kFamMask[1]) ? 1 : 2);
+    }
+    c.sH = (double)pick_fam<T, NF>(sH, f); c.tH = (double)pick_fam<T, NF>(tH, f);
+    c.x1 = (double)pick_fam<T, NF>(x1, f); c.x2 = (double)pick_fam<T, NF>(x2, f);
 }
 
 template <int KIND, typename T>
-__global__ __launch_bounds__(kBlock) void k_unw_count(Params<KIND, T> P, double *__restrict__ partials,
-                                                      uint64_t *__restrict__ counts)
+__device__ __forceinline__ void stage_tables(const Params<KIND, T> &P, T *stab)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    T *stab = reinterpret_cast<T *>(smem);
     if (kind_vegas(KIND)) {
         for (int i = threadIdx.x; i < 9 * P.nbins; i += blockDim.x) stab[i] = P.gtab[i];
         __syncthreads();
     }
+}
+
+// eval: the integrand of every candidate, the stored total (wout may be null: estimate and
+// maximum only) and one kRec record per block.
+template <int KIND, typename T>
+__global__ __launch_bounds__(kBlock) void k_unw_eval(Params<KIND, T> P, T *__restrict__ wout,
+                                                     double *__restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *stab = reinterpret_cast<T *>(smem);
+    stage_tables<KIND, T>(P, stab);
+    constexpr int NP = (KIND == kPartonic) ? 1 : (kind_heavy(KIND) ? kMaxProc : kLightProc);
     double v[kRec];
 #pragma unroll
     for (int i = 0; i < kRec; ++i) v[i] = 0.0;
+    MaxOf<T> mx[NP + 1];
+#pragma unroll
+    for (int i = 0; i <= NP; ++i) mx[i] = 0;
+    uint32_t nnz = 0;
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * P.ppt;
     for (uint32_t j = 0; j < P.ppt; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
         if (i >= P.n) break;
-        Cand c;
-        candidate<KIND, T>(P, stab, P.ctr0 + i, c);
-        v[kRS] += c.w;
-        v[kRQ] = fma(c.w, c.w, v[kRQ]);
-        v[kRM] = fmax(v[kRM], c.w);
-        v[kRNz] += (c.w > 0.0) ? 1.0 : 0.0;
-        v[kRAcc] += c.acc ? 1.0 : 0.0;
+        T ws[kMaxProc];
+        const T tot = eval_cand<KIND, T>(P, stab, P.ctr0 + i, ws);
+        if (wout) wout[i] = tot;
+        const double w = (double)tot;
+        v[kRS] += w;
+        v[kRQ] = fma(w, w, v[kRQ]);
+        mx[0] = run_max(mx[0], tot);
+        nnz += (tot > (T)0) ? 1u : 0u;
 #pragma unroll
-        for (int p = 0; p < (KIND < 2 ? kLightProc : kMaxProc); ++p) {
-            v[kRS + 1 + p] += c.ws[p];
-            v[kRQ + 1 + p] = fma(c.ws[p], c.ws[p], v[kRQ + 1 + p]);
-            v[kRM + 1 + p] = fmax(v[kRM + 1 + p], c.ws[p]);
+        for (int p = 0; p < NP; ++p) {
+            const double d = (double)ws[p];
+            v[kRS + 1 + p] += d;
+            v[kRQ + 1 + p] = fma(d, d, v[kRQ + 1 + p]);
+            mx[1 + p] = run_max(mx[1 + p], ws[p]);
         }
     }
-    block_record(v, partials + (size_t)blockIdx.x * kRec, counts + blockIdx.x);
+#pragma unroll
+    for (int i = 0; i <= NP; ++i) v[kRM + i] = max_value<T>(mx[i]);
+    v[kRNz] = (double)nnz;
+    block_record(v, partials + (size_t)blockIdx.x * kRec);
+}
+
+// flags: accept decisions from the stored totals over the padded range of nwords x 64 candidates
+// (blocks x ppt x 256 of the eval layout; candidates at or beyond n are rejected), so that
+// masks[k] is the ballot of candidates 64 k ... 64 k + 63: the bitset of the whole chunk in
+// candidate order.  Each wave takes 4 consecutive words: its 4 loads are issued before the
+// Philox blocks, and lanes 0..3 store the 4 ballots.
+template <int KIND, typename T>
+__global__ __launch_bounds__(kBlock) void k_unw_flags(Params<KIND, T> P, const T *__restrict__ w, uint64_t nwords,
+                                                      uint64_t *__restrict__ masks)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t word0 = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 4;
+    if (word0 >= nwords) return;                       // whole waves: nwords is a multiple of 4
+    T wv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t i = (word0 + r) * 64 + lane;
+        wv[r] = (i < P.n) ? w[i] : (T)0;
+    }
+    uint64_t mine = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t i = (word0 + r) * 64 + lane;
+        bool acc = false;
+        if (i < P.n) {
+            const uint64_t e = P.ctr0 + i;
+            double u;
+            if (KIND == kPartonic) {
+                u = c1_uacc(P.key, e, P.compat != 0);
+            } else {
+                const U4 a = philox_event(P.key, e, 1u);
+                u = u53(a.x, a.y);
+            }
+            acc = u * P.wmax < (double)wv[r];
+        }
+        const uint64_t m = __ballot(acc);
+        mine = (lane == r) ? m : mine;
+    }
+    if (lane < 4) masks[word0 + lane] = mine;
+}
+
+// counts[b] = accepted candidates of eval block b (its 4 ppt mask words), one block each.
+__global__ __launch_bounds__(kBlock) void k_unw_count(const uint64_t *__restrict__ masks, uint32_t ppt,
+                                                      uint64_t *__restrict__ counts)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t words = ppt * (kBlock / 64);
+    const uint64_t *mb = masks + (size_t)blockIdx.x * words;
+    uint32_t c = 0;
+    for (uint32_t k = threadIdx.x; k < words; k += kBlock) c += (uint32_t)__popcll(mb[k]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int q = 0; q < kBlock / 64; ++q) t += wsum[q];
+        counts[blockIdx.x] = t;
+    }
 }
 
 // Exclusive scan of nblk counts in place; counts[nblk] receives the total.  One block of 1024.
@@ -180,50 +275,67 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint64_t *__restrict__ cou
     for (uint32_t b = lo; b < hi; ++b) { const uint64_t x = counts[b]; counts[b] = run; run += x; }
 }
 
+// compact: the candidate index of every accepted event at rank base + offsets[b] + (rank in the
+// block), written if below cap.  The block's ballots, in mask order, are a bitset over its
+// candidates in candidate order (word k = candidates 64k .. 64k + 63 of the block); thread t
+// takes W consecutive words, the block scans the threads' popcounts, and each thread writes the
+// set bits of its words in order.
+__global__ __launch_bounds__(kBlock) void k_unw_compact(const uint64_t *__restrict__ masks,
+                                                        const uint64_t *__restrict__ offsets, uint64_t ctr0,
+                                                        uint32_t ppt, uint64_t base, uint64_t cap,
+                                                        uint64_t *__restrict__ index)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t words = ppt * (kBlock / 64);
+    const uint32_t W = (words + kBlock - 1) / kBlock;
+    const uint32_t w0 = min(words, threadIdx.x * W), w1 = min(words, w0 + W);
+    const uint64_t *mb = masks + (size_t)blockIdx.x * words;
+    uint32_t c = 0;
+    for (uint32_t k = w0; k < w1; ++k) c += (uint32_t)__popcll(mb[k]);
+    // exclusive scan of c over the block: within the wave, then the wave totals
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = incl - c;
+    for (int q = 0; q < wave; ++q) before += wsum[q];
+    uint64_t pos = base + offsets[blockIdx.x] + before;
+    const uint64_t first = ctr0 + (uint64_t)blockIdx.x * kBlock * ppt;
+    for (uint32_t k = w0; k < w1; ++k) {
+        uint64_t m = mb[k];
+        while (m) {
+            const int bit = __ffsll((unsigned long long)m) - 1;
+            if (pos < cap) index[pos] = first + 64ull * k + (uint64_t)bit;
+            ++pos;
+            m &= m - 1;
+        }
+    }
+}
+
+// materialize: events [base, min(base + *chunk_total, cap)) from their candidate indices.
 template <int KIND, typename T>
-__global__ __launch_bounds__(kBlock) void k_unw_write(Params<KIND, T> P, const uint64_t *__restrict__ offsets,
-                                                      mi_events out)
+__global__ __launch_bounds__(kBlock) void k_unw_materialize(Params<KIND, T> P, const uint64_t *__restrict__ chunk_total,
+                                                            uint64_t base, mi_events out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *stab = reinterpret_cast<T *>(smem);
-    __shared__ uint32_t wcnt[kBlock / 64];
-    if (kind_vegas(KIND)) {
-        for (int i = threadIdx.x; i < 9 * P.nbins; i += blockDim.x) stab[i] = P.gtab[i];
-        __syncthreads();
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-    uint64_t pos0 = offsets[blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * kBlock * P.ppt;
-    for (uint32_t j = 0; j < P.ppt; ++j) {                 // block-uniform trip count (barriers inside)
-        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    stage_tables<KIND, T>(P, stab);
+    const uint64_t hi = min(base + *chunk_total, out.capacity);
+    for (uint64_t j = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < hi;
+         j += (uint64_t)gridDim.x * blockDim.x) {
         Cand c;
-        c.acc = false;
-        if (i < P.n) candidate<KIND, T>(P, stab, P.ctr0 + i, c);
-        const uint64_t m = __ballot(c.acc);
-        if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t before = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) {
-            const uint32_t x = wcnt[w];
-            before += (w < wave) ? x : 0u;
-            total += x;
-        }
-        if (c.acc) {
-            const uint64_t pos = pos0 + before + (uint64_t)__popcll(m & lt);
-            if (pos < out.capacity) {
-                out.sHat[pos] = c.sH;
-                out.tHat[pos] = c.tH;
-                out.x1[pos] = c.x1;
-                out.x2[pos] = c.x2;
-                out.weight[pos] = fmax(c.w, P.wmax);
-                out.pid[pos] = c.pid;
-                out.index[pos] = P.ctr0 + i;
-            }
-        }
-        pos0 += total;
-        __syncthreads();
+        full_cand<KIND, T>(P, stab, out.index[j], c);
+        out.sHat[j] = c.sH;
+        out.tHat[j] = c.tH;
+        out.x1[j] = c.x1;
+        out.x2[j] = c.x2;
+        out.weight[j] = fmax(c.w, P.wmax);
+        out.pid[j] = c.pid;
     }
 }
 
@@ -249,56 +361,105 @@ Params<KIND, T> make_params(const UnwLaunch &L, const HadConst<T> &k, const T *t
     return P;
 }
 
-template <int KIND, typename T>
-hipError_t count_t(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, uint64_t *counts)
-{
-    const Params<KIND, T> P = make_params<KIND, T>(L, k, tab);
-    const Geom g = L.partonic ? L.c.g : L.h.g;
-    double *parts = L.partonic ? L.c.partials : L.h.partials;
-    const size_t sm = kind_vegas(KIND) ? sizeof(T) * 9 * P.nbins : 0;
-    hipLaunchKernelGGL((k_unw_count<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, parts, counts);
-    return hipGetLastError();
-}
+enum class Stage { Eval, Flags, Materialize };
 
 template <int KIND, typename T>
-hipError_t write_t(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, const uint64_t *offsets,
-                   const mi_events &out)
+hipError_t stage_t(hipStream_t s, Stage st, const UnwLaunch &L, const HadConst<T> &k, const T *tab)
 {
     const Params<KIND, T> P = make_params<KIND, T>(L, k, tab);
     const Geom g = L.partonic ? L.c.g : L.h.g;
     const size_t sm = kind_vegas(KIND) ? sizeof(T) * 9 * P.nbins : 0;
-    hipLaunchKernelGGL((k_unw_write<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, offsets, out);
+    switch (st) {
+    case Stage::Eval: {
+        double *parts = L.partonic ? L.c.partials : L.h.partials;
+        hipLaunchKernelGGL((k_unw_eval<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, (T *)L.w, parts);
+        break;
+    }
+    case Stage::Flags: {
+        const uint64_t nwords = (uint64_t)g.blocks * g.ppt * (kBlock / 64);     // a multiple of 4
+        const uint64_t fb = (nwords + 4 * (kBlock / 64) - 1) / (4 * (kBlock / 64));
+        hipLaunchKernelGGL((k_unw_flags<KIND, T>), dim3((uint32_t)fb), dim3(kBlock), 0, s, P, (const T *)L.w, nwords,
+                           L.masks);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_unw_count, dim3(g.blocks), dim3(kBlock), 0, s, L.masks, g.ppt, L.counts);
+        break;
+    }
+    case Stage::Materialize:
+        hipLaunchKernelGGL((k_unw_materialize<KIND, T>), dim3(L.mat_blocks), dim3(kBlock), sm, s, P,
+                           L.counts + g.blocks, L.base, *L.out);
+        break;
+    }
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t count_h(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, uint64_t *counts)
+// The hadronic KIND of a launch.
+inline int had_kind(const UnwLaunch &L, bool grid)
 {
-    const bool hv = has_heavy(L.h.mask);
-    if (L.h.vegas) return hv ? count_t<4, T>(s, L, k, tab, counts) : count_t<1, T>(s, L, k, tab, counts);
-    return hv ? count_t<3, T>(s, L, k, nullptr, counts) : count_t<0, T>(s, L, k, nullptr, counts);
+    return (L.h.vegas ? kVegas : 0) | (has_heavy(L.h.mask) ? kHeavy : 0) | (grid ? kGrid : 0);
 }
 
 template <typename T>
-hipError_t write_h(hipStream_t s, const UnwLaunch &L, const HadConst<T> &k, const T *tab, const uint64_t *offsets,
-                   const mi_events &out)
+hipError_t stage_h(hipStream_t s, Stage st, const UnwLaunch &L, const HadConst<T> &k, const T *tab)
 {
-    const bool hv = has_heavy(L.h.mask);
-    if (L.h.vegas) return hv ? write_t<4, T>(s, L, k, tab, offsets, out) : write_t<1, T>(s, L, k, tab, offsets, out);
-    return hv ? write_t<3, T>(s, L, k, nullptr, offsets, out) : write_t<0, T>(s, L, k, nullptr, offsets, out);
+    switch (had_kind(L, grid_of(L.h, (T *)nullptr).active != 0)) {
+    case 0: return stage_t<0, T>(s, st, L, k, nullptr);
+    case 1: return stage_t<1, T>(s, st, L, k, tab);
+    case 2: return stage_t<2, T>(s, st, L, k, nullptr);
+    case 3: return stage_t<3, T>(s, st, L, k, tab);
+    case 4: return stage_t<4, T>(s, st, L, k, nullptr);
+    case 5: return stage_t<5, T>(s, st, L, k, tab);
+    case 6: return stage_t<6, T>(s, st, L, k, nullptr);
+    default: return stage_t<7, T>(s, st, L, k, tab);
+    }
+}
+
+hipError_t stage(hipStream_t s, Stage st, const UnwLaunch &L)
+{
+    if (L.partonic) {
+        HadConst<double> kd{};
+        return stage_t<kPartonic, double>(s, st, L, kd, nullptr);
+    }
+    if (L.h.prec == MI_PREC_FP32) return stage_h<float>(s, st, L, L.h.kf, L.h.vegas ? L.h.grid->tab32 : nullptr);
+    return stage_h<double>(s, st, L, L.h.kd, L.h.vegas ? L.h.grid->tab64 : nullptr);
+}
+
+template <int KIND, typename T>
+int occ_eval_t(size_t sm)
+{
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_unw_eval<KIND, T>, kBlock, sm);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+template <typename T>
+int occ_eval_h(const UnwLaunch &L)
+{
+    const size_t sm = L.h.vegas ? sizeof(T) * 9 * L.h.grid->nbins : 0;
+    switch (had_kind(L, grid_of(L.h, (T *)nullptr).active != 0)) {
+    case 0: return occ_eval_t<0, T>(sm);
+    case 1: return occ_eval_t<1, T>(sm);
+    case 2: return occ_eval_t<2, T>(sm);
+    case 3: return occ_eval_t<3, T>(sm);
+    case 4: return occ_eval_t<4, T>(sm);
+    case 5: return occ_eval_t<5, T>(sm);
+    case 6: return occ_eval_t<6, T>(sm);
+    default: return occ_eval_t<7, T>(sm);
+    }
 }
 
 }  // namespace
 
-hipError_t launch_unweight_count(hipStream_t s, const UnwLaunch &L, uint64_t *counts)
+// Resident blocks per CU of the eval kernel the launch will use.
+int occ_unweight_eval(const UnwLaunch &L)
 {
-    if (L.partonic) {
-        HadConst<double> kd{};
-        return count_t<2, double>(s, L, kd, nullptr, counts);
-    }
-    if (L.h.prec == MI_PREC_FP32) return count_h<float>(s, L, L.h.kf, L.h.vegas ? L.h.grid->tab32 : nullptr, counts);
-    return count_h<double>(s, L, L.h.kd, L.h.vegas ? L.h.grid->tab64 : nullptr, counts);
+    if (L.partonic) return occ_eval_t<kPartonic, double>(0);
+    return L.h.prec == MI_PREC_FP32 ? occ_eval_h<float>(L) : occ_eval_h<double>(L);
 }
+
+hipError_t launch_unweight_eval(hipStream_t s, const UnwLaunch &L) { return stage(s, Stage::Eval, L); }
+hipError_t launch_unweight_flags(hipStream_t s, const UnwLaunch &L) { return stage(s, Stage::Flags, L); }
+hipError_t launch_unweight_materialize(hipStream_t s, const UnwLaunch &L) { return stage(s, Stage::Materialize, L); }
 
 hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk)
 {
@@ -306,15 +467,13 @@ hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk)
     return hipGetLastError();
 }
 
-hipError_t launch_unweight_write(hipStream_t s, const UnwLaunch &L, const uint64_t *offsets, const mi_events &out)
+hipError_t launch_unweight_compact(hipStream_t s, const UnwLaunch &L)
 {
-    if (L.partonic) {
-        HadConst<double> kd{};
-        return write_t<2, double>(s, L, kd, nullptr, offsets, out);
-    }
-    if (L.h.prec == MI_PREC_FP32)
-        return write_h<float>(s, L, L.h.kf, L.h.vegas ? L.h.grid->tab32 : nullptr, offsets, out);
-    return write_h<double>(s, L, L.h.kd, L.h.vegas ? L.h.grid->tab64 : nullptr, offsets, out);
+    const Geom g = L.partonic ? L.c.g : L.h.g;
+    const uint64_t ctr0 = L.partonic ? L.c.ctr0 : L.h.ctr0;
+    hipLaunchKernelGGL(k_unw_compact, dim3(g.blocks), dim3(kBlock), 0, s, L.masks, L.counts, ctr0, g.ppt, L.base,
+                       L.out->capacity, L.out->index);
+    return hipGetLastError();
 }
 
 }  // namespace mi

@@ -155,3 +155,28 @@ def test_unweight_c3_workload(ctx, O):
     sig_unw = ev.n / (1 << 26) * ev.wmax
     assert abs(sig_unw - ev.estimate.sigmaGen()) < 5 * ev.estimate.sigmaGen() / math.sqrt(ev.n)
     print(f"C3 unweighting: {ev.n} of 2^26 accepted (eta {ev.n / (1 << 26):.4f}), {ev.estimate.kernel_ms:.2f} ms")
+
+
+@pytest.mark.parametrize("exact_max", [False, True])
+def test_unweight_chunked_equals_split_calls(ctx, exact_max):
+    """Above 2^27 candidates the weights are stored chunk by chunk: the events of one call over
+    2^27 + 2^22 candidates equal, array for array, those of two calls over [0, 2^27) and
+    [2^27, 2^27 + 2^22) at the same wMax; with the exact maximum, wMax is the maximum of the two
+    calls' maxima and the weighted estimate folds both chunks."""
+    phys = mi.physics_default(1)
+    n1, n2 = 1 << 27, 1 << 22
+    wm = 0.0 if exact_max else 2.7
+    cap = 1 << 23
+    full = ctx.unweight_compact(mi.make_run(phys, n1 + n2, key=8, counter0=3), wmax=wm, capacity=cap)
+    w_used = full.wmax
+    a = ctx.unweight_compact(mi.make_run(phys, n1, key=8, counter0=3), wmax=w_used, capacity=cap)
+    b = ctx.unweight_compact(mi.make_run(phys, n2, key=8, counter0=3 + n1), wmax=w_used, capacity=cap)
+    if exact_max:
+        ea = ctx.unweight_compact(mi.make_run(phys, n1, key=8, counter0=3), capacity=1 << 12, wmax=1e30)
+        eb = ctx.unweight_compact(mi.make_run(phys, n2, key=8, counter0=3 + n1), capacity=1 << 12, wmax=1e30)
+        assert w_used == max(ea.estimate.wMax(), eb.estimate.wMax())
+        assert full.estimate.sumW() == pytest.approx(ea.estimate.sumW() + eb.estimate.sumW(), rel=1e-13)
+    assert full.n == a.n + b.n and b.n > 0
+    for name in ("index", "sHat", "tHat", "x1", "x2", "weight", "pid"):
+        assert torch.equal(getattr(full, name), torch.cat([getattr(a, name), getattr(b, name)])), name
+    assert full.estimate.nTried() == n1 + n2 and full.estimate.nAccepted() == full.n
