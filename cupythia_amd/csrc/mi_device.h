@@ -353,6 +353,97 @@ __host__ __device__ inline void grid_xf(const GridPdf<T> &g, T x, T lnx, int m3q
     }
 }
 
+// FP32 (device layout): the same arithmetic with each node's 12 values and each stencil start's
+// 12 reciprocals fetched as three 16-byte words (the FP32 tables are 16-byte aligned).
+__host__ __device__ inline void lagrange4_v(float t, const float *n, const float *inv, float *w)
+{
+    const float4 *iv = reinterpret_cast<const float4 *>(inv);
+    const float4 a = iv[0], b = iv[1], c = iv[2];
+    const float r[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        w[i] = 1.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j != i) { w[i] *= (t - n[j]) * r[k]; ++k; }
+    }
+}
+
+__host__ __device__ inline void node_vals(const float *p, float *v)
+{
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+    const float4 a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+    v[6] = b.z; v[7] = b.w; v[8] = c.x; v[9] = c.y; v[10] = c.z;
+}
+
+__host__ __device__ inline void grid_qw_f32(const GridPdf<float> &g, float q, float lnq, int &m3q, int &n3q, float *wq)
+{
+    wq[0] = wq[1] = wq[2] = wq[3] = 1.0f;
+    if (q <= g.qMin) { n3q = 1; m3q = 0; return; }
+    if (q >= g.qMax) { n3q = 1; m3q = g.nq - 1; return; }
+    int iqDiv = 0;
+    for (int s = 1; s < g.nqSub; ++s)
+        if (q > g.qDiv[s - 1]) iqDiv = s;
+    const int minS = iqDiv == 0 ? 0 : g.nqSum[iqDiv - 1], maxS = g.nqSum[iqDiv] - 1;
+    int minq = minS, maxq = maxS;
+    while (maxq - minq > 1) {
+        const int mid = (minq + maxq) >> 1;
+        if (q < g.qk[mid]) maxq = mid;
+        else               minq = mid;
+    }
+    if (maxS - minS < 3) {
+        n3q = 2;
+        m3q = minq;
+        wq[1] = (lnq - g.lnq[minq]) * g.iqinv[kGridNF * minq];
+        wq[0] = 1.0f - wq[1];
+    } else {
+        n3q = 4;
+        m3q = (minq == minS) ? minS : ((maxq == maxS) ? maxS - 3 : minq - 1);
+        lagrange4_v(lnq, g.lnq + m3q, g.iqinv + kGridNF * m3q, wq);
+    }
+}
+
+// The 4 x 4 stencil (n3q rows of 4 x nodes) folded node by node with the combined weight
+// wq * wx into FP32 fmas: 16 products and 11 fmas per node, one node's values live at a time.
+__host__ __device__ inline void grid_xf_f32(const GridPdf<float> &g, float x, float lnx, int m3q, int n3q,
+                                            const float *wq, float *out)
+{
+#pragma unroll
+    for (int f = 0; f < 11; ++f) out[f] = 0.0f;
+    if (x <= g.xMin) {
+        for (int i3q = 0; i3q < n3q; ++i3q) {
+            float v[11];
+            node_vals(g.val + (size_t)(m3q + i3q) * g.nx * kGridNF, v);
+#pragma unroll
+            for (int f = 0; f < 11; ++f) out[f] = __builtin_fmaf(wq[i3q], v[f], out[f]);
+        }
+        return;
+    }
+    if (x >= g.xMax) return;
+    int minx = 0, maxx = g.nx - 1;
+    while (maxx - minx > 1) {
+        const int mid = (minx + maxx) >> 1;
+        if (x < g.xk[mid]) maxx = mid;
+        else               minx = mid;
+    }
+    const int m3x = (minx == 0) ? 0 : ((maxx == g.nx - 1) ? g.nx - 4 : minx - 1);
+    float wx[4];
+    lagrange4_v(lnx, g.lnx + m3x, g.ixinv + kGridNF * m3x, wx);
+    for (int i3q = 0; i3q < n3q; ++i3q) {
+        const float *p = g.val + ((size_t)(m3q + i3q) * g.nx + m3x) * kGridNF;
+#pragma unroll
+        for (int ix = 0; ix < 4; ++ix) {
+            const float w = wq[i3q] * wx[ix];
+            float v[11];
+            node_vals(p + ix * kGridNF, v);
+#pragma unroll
+            for (int f = 0; f < 11; ++f) out[f] = __builtin_fmaf(w, v[f], out[f]);
+        }
+    }
+}
+
 // Parton luminosities of the five flux types from the grid at the factorization scale
 // (PDF values clipped at 0 as Pythia's PDF::xf does): gg, qg, all quark pairs QQ, same-flavour
 // pairs (q q and qbar qbar), and q qbar pairs of one flavour; nQ flavours enter.
@@ -364,9 +455,15 @@ __host__ __device__ inline Lumi<T> grid_lumi(const GridPdf<T> &g, int nQ, T x1, 
 {
     int m3q, n3q;
     T wq[4], f1[11], f2[11];
-    grid_qw<T>(g, q, lnq, m3q, n3q, wq);
-    grid_xf<T>(g, x1, lx1, m3q, n3q, wq, f1);
-    grid_xf<T>(g, x2, lx2, m3q, n3q, wq, f2);
+    if constexpr (sizeof(T) == 4) {
+        grid_qw_f32(g, q, lnq, m3q, n3q, wq);
+        grid_xf_f32(g, x1, lx1, m3q, n3q, wq, f1);
+        grid_xf_f32(g, x2, lx2, m3q, n3q, wq, f2);
+    } else {
+        grid_qw<T>(g, q, lnq, m3q, n3q, wq);
+        grid_xf<T>(g, x1, lx1, m3q, n3q, wq, f1);
+        grid_xf<T>(g, x2, lx2, m3q, n3q, wq, f2);
+    }
 #pragma unroll
     for (int f = 0; f < 11; ++f) {
         f1[f] = f1[f] > (T)0 ? f1[f] : (T)0;

@@ -25,7 +25,8 @@ struct PdfState {
     double qDiv[kGridMaxSub] = {};
     // device
     double *d64 = nullptr;    // xk | lnx | qk | lnq | val
-    float *d32 = nullptr;     // xk | lnx | qk | lnq | val | ixinv | iqinv
+    float *d32 = nullptr;     // val | ixinv | iqinv | xk | lnx | qk | lnq (val and the reciprocal
+                              // tables 16-byte aligned: 12 floats per node / stencil start)
     GridPdf<double> g64;      // views into d64
     GridPdf<float> g32;       // views into d32
     GridPdf<double> host64;   // views into the host copies
@@ -239,11 +240,17 @@ int mi_pdf_load(mi_ctx *ctx, const char *path)
     h64.insert(h64.end(), st->qk.begin(), st->qk.end());
     h64.insert(h64.end(), st->lnq.begin(), st->lnq.end());
     h64.insert(h64.end(), st->val.begin(), st->val.end());
-    std::vector<float> h32(h64.size());
-    for (size_t i = 0; i < h64.size(); ++i) h32[i] = (float)h64[i];
-    // the FP32 logs are the FP64 logs rounded (not logs of the rounded nodes)
+    // FP32 copy: rounded from FP64 (the FP32 logs are the FP64 logs rounded, not logs of the rounded
+    // nodes), values first so that every node's 12 values are three aligned 16-byte words
+    std::vector<float> h32;
+    h32.reserve(nv + kGridNF * (nx + nq) + 2 * nx + 2 * nq);
+    for (size_t i = 0; i < nv; ++i) h32.push_back((float)st->val[i]);
     h32.insert(h32.end(), ixinv.begin(), ixinv.end());
     h32.insert(h32.end(), iqinv.begin(), iqinv.end());
+    for (double v : st->x) h32.push_back((float)v);
+    for (double v : st->lnx) h32.push_back((float)v);
+    for (double v : st->qk) h32.push_back((float)v);
+    for (double v : st->lnq) h32.push_back((float)v);
     if (e == hipSuccess) e = hipMalloc(&st->d64, sizeof(double) * h64.size());
     if (e == hipSuccess) e = hipMalloc(&st->d32, sizeof(float) * h32.size());
     if (e == hipSuccess) e = hipMemcpy(st->d64, h64.data(), sizeof(double) * h64.size(), hipMemcpyHostToDevice);
@@ -257,10 +264,13 @@ int mi_pdf_load(mi_ctx *ctx, const char *path)
     st->g64.xk = st->d64; st->g64.lnx = st->d64 + nx; st->g64.qk = st->d64 + 2 * nx;
     st->g64.lnq = st->d64 + 2 * nx + nq; st->g64.val = st->d64 + 2 * nx + 2 * nq;
     fill_view(st->g32, *st);
-    st->g32.xk = st->d32; st->g32.lnx = st->d32 + nx; st->g32.qk = st->d32 + 2 * nx;
-    st->g32.lnq = st->d32 + 2 * nx + nq; st->g32.val = st->d32 + 2 * nx + 2 * nq;
-    st->g32.ixinv = st->d32 + 2 * nx + 2 * nq + nv;
+    st->g32.val = st->d32;
+    st->g32.ixinv = st->d32 + nv;
     st->g32.iqinv = st->g32.ixinv + kGridNF * nx;
+    st->g32.xk = st->g32.iqinv + kGridNF * nq;
+    st->g32.lnx = st->g32.xk + nx;
+    st->g32.qk = st->g32.lnx + nx;
+    st->g32.lnq = st->g32.qk + nq;
     fill_view(st->host64, *st);
     st->host64.xk = st->x.data(); st->host64.lnx = st->lnx.data(); st->host64.qk = st->qk.data();
     st->host64.lnq = st->lnq.data(); st->host64.val = st->val.data();

@@ -240,7 +240,7 @@ static void grid_xf32(const orc32_grid *g, float x, float lnx, int m3q, int n3q,
     if (x <= g->xMin) {
         for (int i3q = 0; i3q < n3q; ++i3q) {
             const float *p = g->val + (size_t)(m3q + i3q) * g->nx * ORC_GRID_NF;
-            for (int f = 0; f < 11; ++f) out[f] += wq[i3q] * p[f];
+            for (int f = 0; f < 11; ++f) out[f] = fmaf(wq[i3q], p[f], out[f]);
         }
         return;
     }
@@ -254,11 +254,13 @@ static void grid_xf32(const orc32_grid *g, float x, float lnx, int m3q, int n3q,
     int m3x = (minx == 0) ? 0 : ((maxx == g->nx - 1) ? g->nx - 4 : minx - 1);
     float wx[4];
     lagrange4_32(lnx, g->lnx + m3x, g->ixinv + ORC_GRID_NF * m3x, wx);
+    /* node by node with the combined weight wq * wx, one fma per flavour */
     for (int i3q = 0; i3q < n3q; ++i3q) {
         const float *p = g->val + ((size_t)(m3q + i3q) * g->nx + m3x) * ORC_GRID_NF;
-        for (int f = 0; f < 11; ++f)
-            out[f] += wq[i3q] * (wx[0] * p[f] + wx[1] * p[ORC_GRID_NF + f] + wx[2] * p[2 * ORC_GRID_NF + f]
-                                 + wx[3] * p[3 * ORC_GRID_NF + f]);
+        for (int ix = 0; ix < 4; ++ix) {
+            float w = wq[i3q] * wx[ix];
+            for (int f = 0; f < 11; ++f) out[f] = fmaf(w, p[ix * ORC_GRID_NF + f], out[f]);
+        }
     }
 }
 

@@ -145,8 +145,9 @@ def test_grid_point_fp64_vs_long_double(O, tmp_path):
 def test_fp32_grid_mirror_accuracy(O, tmp_path):
     """The FP32 mirror of the device's grid evaluation against the long-double Pythia-form oracle
     (same grid, same points): median 1.2e-6 (the FP32 resolution of ln x and ln q in the Lagrange
-    weights), below 3e-5 for every weight above 1e-8 of the mean; the cubic interpolation of
-    the steep large-x tail cancels to larger relative errors only on weights ~1e-20 of the mean."""
+    weights), 99.9% of the weights above 1e-8 of the mean within 1e-5 and all of them within 5e-5;
+    the cubic interpolation of the steep large-x tail cancels to larger relative errors only on
+    weights ~1e-20 of the mean."""
     xs = PG.nnpdf_like_x_grid(40, 40, 1e-7)
     qs = PG.q_grid(1.0, 1.0e4, 30)
     path = str(tmp_path / "toyq.dat")
@@ -156,11 +157,12 @@ def test_fp32_grid_mirror_accuracy(O, tmp_path):
     n = 1 << 14
     wld, _ = O.had_points_ld(cfg, 42, 0, n)
     w32, _ = O.had_points32(cfg, 42, 0, n)
-    for col in range(6):
+    for col in range(1, 7):
         m = wld[:, col] > 1e-30
         if not m.any():
             continue
         rel = np.abs(w32[m, col].astype(np.float64) - wld[m, col]) / wld[m, col]
         big = wld[m, col] > 1e-8 * wld[m, col].mean()
-        assert rel[big].max() < 3e-5, (col, rel[big].max())
+        assert rel[big].max() < 5e-5, (col, rel[big].max())
+        assert np.quantile(rel[big], 0.999) < 1e-5, col
         assert np.median(rel) < 3e-6
