@@ -31,6 +31,7 @@ MI_SAMPLER_FLAT, MI_SAMPLER_VEGAS, MI_SAMPLER_FLOW = 0, 1, 2
 MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT = 0, 1
 MI_PDF_TOY, MI_PDF_GRID = 0, 1
 MI_ABI_VERSION = 3
+MI_PDF_SLOTS = 8
 
 MI_FLOW_LAYERS, MI_FLOW_BINS, MI_FLOW_HIDDEN, MI_FLOW_NOUT = 6, 16, 64, 49
 MI_FLOW_NPARAMS = 45222
@@ -70,6 +71,10 @@ class MiEvents(C.Structure):
                 ("weight", C.c_void_p), ("pid", C.c_void_p), ("index", C.c_void_p), ("capacity", U64)]
 
 
+class MiWeightVar(C.Structure):
+    _fields_ = [("renorm_fac", D), ("fact_fac", D), ("pdf_slot", I)]
+
+
 class MiSigmaGrad(C.Structure):
     _fields_ = [("sigma", D), ("sigma_err", D), ("d_alphaS_MZ", D), ("d_alphaS_MZ_err", D),
                 ("d_renorm_fac", D), ("d_renorm_fac_err", D), ("d_pdf_g_a", D), ("d_pdf_g_a_err", D),
@@ -105,6 +110,8 @@ PROTOTYPES = {
     "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, VP, I, P(D), VP], I),
     "mi_adjoint": ([VP, P(MiRun), P(MiSigmaGrad)], I),
     "mi_pdf_load": ([VP, C.c_char_p], I),
+    "mi_pdf_load_slot": ([VP, I, C.c_char_p], I),
+    "mi_reweight": ([VP, P(MiPhysics), U64, VP, VP, VP, VP, VP, VP, I, P(MiWeightVar), VP], I),
     "mi_pdf_xf": ([VP, U64, VP, VP, VP], I),
     "mi_flow_init": ([VP, U64], I),
     "mi_flow_train": ([VP, P(MiRun), I, U64, D, P(MiFlowInfo)], I),

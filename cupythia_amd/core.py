@@ -260,11 +260,25 @@ class Context:
                                                 len(factors), fac, _ptr(out)), "mi_reweight_scales")
         return out
 
+    def reweight(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, x1: torch.Tensor,
+                 x2: torch.Tensor, pid: torch.Tensor, w: torch.Tensor, variations) -> torch.Tensor:
+        """Weights under (renorm_fac, fact_fac, pdf_slot) variations (n x nvar): each event's own
+        process weight recomputed at the varied scales / PDFs over the run's (mi_reweight)."""
+        vs = (A.MiWeightVar * len(variations))(*[A.MiWeightVar(float(r), float(f), int(sl))
+                                                 for (r, f, sl) in variations])
+        n = sHat.numel()
+        out = torch.empty((n, len(variations)), dtype=torch.float64, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_reweight(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat), _ptr(x1), _ptr(x2),
+                                         _ptr(pid), _ptr(w), len(variations), vs, _ptr(out)), "mi_reweight")
+        return out
+
     # -------------------------------------------------------------- PDF grids
-    def pdf_load(self, path: str) -> None:
-        """Proton PDF from an LHAPDF6 lhagrid1 file (Pythia's LHAGrid1); runs with
-        physics_default(..., pdf_source=MI_PDF_GRID) use it."""
-        self._check(self.lib.mi_pdf_load(self.h, path.encode()), "mi_pdf_load")
+    def pdf_load(self, path: str, slot: int = 0) -> None:
+        """Proton PDF from an LHAPDF6 lhagrid1 file (Pythia's LHAGrid1) into a PDF slot: slot 0 is
+        the grid of runs with physics_default(..., pdf_source=MI_PDF_GRID); slots 1.. hold the
+        grids of reweight()'s PDF variations."""
+        self._check(self.lib.mi_pdf_load_slot(self.h, slot, path.encode()), "mi_pdf_load_slot")
 
     def pdf_xf(self, x: torch.Tensor, Q2: torch.Tensor) -> torch.Tensor:
         """Raw interpolated xf (n x 12 FP64: g, d, u, s, c, b, dbar, ubar, sbar, cbar, bbar, 0)."""

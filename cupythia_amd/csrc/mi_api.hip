@@ -724,4 +724,54 @@ int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const do
     return MI_OK;
 }
 
+int mi_reweight(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev, const double *tHat_dev,
+                const double *x1_dev, const double *x2_dev, const int32_t *pid_dev, const double *w_dev, int nvar,
+                const mi_weight_var *vars_host, double *out_dev)
+{
+    if (!ctx) return MI_ERR_ARG;
+    if (!phys) return fail(ctx, MI_ERR_ARG, "phys is NULL");
+    if (nvar < 1 || nvar > 64) return fail(ctx, MI_ERR_RANGE, "nvar must be 1..64");
+    if (!vars_host) return fail(ctx, MI_ERR_ARG, "vars is NULL");
+    if (n && (!sHat_dev || !tHat_dev || !x1_dev || !x2_dev || !pid_dev || !w_dev || !out_dev))
+        return fail(ctx, MI_ERR_ARG, "NULL array");
+    if (!(phys->alphaS_value > 0.0) || !(phys->renorm_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "bad alphaS / renorm_fac");
+    const bool grid = phys->pdf_source == MI_PDF_GRID;
+    if (!grid && phys->pdf_source != MI_PDF_TOY) return fail(ctx, MI_ERR_ARG, "unknown pdf_source");
+    if (grid && !(phys->fact_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "fact_fac must be > 0");
+    std::vector<RwVar> vars((size_t)nvar);
+    GridSet64 gs{};
+    if (grid) {
+        const int rc = pdf_slot_view(ctx, 0, &gs.g[0]);
+        if (rc) return rc;
+    }
+    for (int v = 0; v < nvar; ++v) {
+        const mi_weight_var &q = vars_host[v];
+        if (!(q.renorm_fac > 0.0) || !(q.fact_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "scale factors must be > 0");
+        if (q.pdf_slot < 0 || q.pdf_slot >= MI_PDF_SLOTS) return fail(ctx, MI_ERR_ARG, "pdf_slot out of range");
+        if (!grid && q.pdf_slot != 0) return fail(ctx, MI_ERR_ARG, "PDF slots other than 0 need pdf_source = MI_PDF_GRID");
+        if (grid && !gs.g[q.pdf_slot].active) {
+            const int rc = pdf_slot_view(ctx, q.pdf_slot, &gs.g[q.pdf_slot]);
+            if (rc) return rc;
+        }
+        vars[v] = RwVar{q.renorm_fac * q.renorm_fac, q.fact_fac * q.fact_fac, q.pdf_slot, 0};
+    }
+    if (n == 0) return MI_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HadConst<double> k;
+    mi_physics p = *phys;
+    if (p.eCM <= 2.0 * p.pTHatMin || p.pTHatMin <= 0.0) { p.eCM = 13600.0; p.pTHatMin = 20.0; }
+    if (p.nQuarkIn < 2 || p.nQuarkIn > 5) return fail(ctx, MI_ERR_ARG, "nQuarkIn must be 2..5");
+    fill_had_consts_f64(p, k);
+    const size_t nd = (sizeof(RwVar) * vars.size() + sizeof(double) - 1) / sizeof(double);
+    int rc = ensure(ctx, &ctx->hist, &ctx->hist_cap, nd);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hist, vars.data(), sizeof(RwVar) * vars.size(), hipMemcpyHostToDevice, ctx->stream));
+    const double ff2 = grid ? phys->fact_fac * phys->fact_fac : 1.0;
+    HIPCHK(ctx, launch_reweight_vars(ctx->stream, k, gs, grid, n, sHat_dev, tHat_dev, x1_dev, x2_dev, pid_dev, w_dev,
+                                     nvar, reinterpret_cast<const RwVar *>(ctx->hist), p.renorm_fac * p.renorm_fac, ff2,
+                                     out_dev));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
 }  // extern "C"

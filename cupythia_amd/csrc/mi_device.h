@@ -658,6 +658,81 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
 // ------------------------------------------------------------------ FP64 hadronic point
 // Same map and algebra in FP64 with the device libm (exp, log, expm1, sqrt); the per-point
 // parity target against the host Pythia-form oracle is 1e-12 relative.
+// alphaS of Pythia's order-0/1 AlphaStrong at Q2 (FP64); cf = 12 pi / b0 of the active flavours.
+__host__ __device__ inline double alphas_f64(const HadConst<double>& k, double Q2, double* cfo = nullptr)
+{
+    if (k.as_order == 0.0) { if (cfo) *cfo = 0.0; return k.as_fixed; }
+    if (Q2 < k.scale2Min) Q2 = k.scale2Min;
+    const double lq = log(Q2);
+    double lam, cf;
+    if (Q2 > k.mt2 && k.nf6 != 0.0) { lam = k.lnL6s; cf = k.c6; }
+    else if (Q2 > k.mb2)            { lam = k.lnL5s; cf = k.c5; }
+    else if (Q2 > k.mc2)            { lam = k.lnL4s; cf = k.c4; }
+    else                            { lam = k.lnL3s; cf = k.c3; }
+    if (cfo) *cfo = cf;
+    return cf / (lq - lam);
+}
+
+// Luminosities of the toy PDFs at (x1, x2) = (e^lx1, e^lx2) (FP64); l1a, l1b = ln(1 - x).
+__host__ __device__ inline Lumi<double> toy_lumi_f64(const HadConst<double>& k, double lx1, double lx2, double& l1a,
+                                                     double& l1b)
+{
+    Lumi<double> L;
+    l1a = log(-expm1(lx1)); l1b = log(-expm1(lx2));
+    const double g1 = exp(fma(k.bg, l1a, fma(k.ag, lx1, k.lnAg)));
+    const double g2 = exp(fma(k.bg, l1b, fma(k.ag, lx2, k.lnAg)));
+    const double S1 = exp(fma(k.bs, l1a, fma(k.as, lx1, k.lnAs)));
+    const double S2 = exp(fma(k.bs, l1b, fma(k.as, lx2, k.lnAs)));
+    const double V1 = exp(fma(k.bv, l1a, fma(k.av, lx1, k.lnAuv)));
+    const double V2 = exp(fma(k.bv, l1b, fma(k.av, lx2, k.lnAuv)));
+    const double Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
+    const double U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
+    const double Q1 = fma(k.nSea, S1, V1 + Vd1), Q2s = fma(k.nSea, S2, V2 + Vd2);
+    const double SS = S1 * S2;
+    L.g1 = g1; L.g2 = g2; L.Q1 = Q1; L.Q2 = Q2s;
+    L.gg = g1 * g2;
+    L.qg = fma(Q1, g2, g1 * Q2s);
+    L.QQ = Q1 * Q2s;
+    L.same = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
+    L.opp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+    return L;
+}
+
+// Bracket times luminosity of each process of a family (FP64): FAM 0 fills w[0..5] (codes
+// 111..116), FAM 1, 2 fill w[0..1] (the family's gg and q qbar codes); a = (m^2 - tH)/sH,
+// b = (m^2 - uH)/sH, mu = m^2 / sH (0 for FAM 0).  d sigma / dt = pi alpS^2 / sH^2 * bracket.
+template <int FAM>
+__host__ __device__ inline void brackets_f64(const HadConst<double>& k, double a, double b, double mu,
+                                             const Lumi<double>& L, double* w)
+{
+    const double ab = a * b;
+    const double inv_ab = 1.0 / ab;
+    const double ia = b * inv_ab, ib = a * inv_ab;
+    const double q2 = fma(a, a, b * b);
+    if (FAM == 0) {
+        w[0] = L.gg * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
+        w[1] = L.gg * (k.nQuarkNew * q2 * fma(inv_ab, 1.0 / 6.0, -0.375));
+        w[2] = L.qg * (fma(b, b, 1.0) * fma(ia, ia, (4.0 / 9.0) * ib));
+        {
+            const double sigT = (4.0 / 9.0) * fma(b, b, 1.0) * (ia * ia);
+            const double sigU = (4.0 / 9.0) * fma(a, a, 1.0) * (ib * ib);
+            const double sigTU = -(8.0 / 27.0) * inv_ab;
+            const double sigST = (8.0 / 27.0) * (b * b) * ia;
+            const double csame = 0.5 * (sigU + sigTU - sigT);
+            w[3] = fma(sigT, L.QQ, fma(csame, L.same, sigST * L.opp));
+        }
+        w[4] = L.opp * (q2 * fma(inv_ab, 16.0 / 27.0, -4.0 / 3.0));
+        w[5] = L.opp * (k.nQuarkNew * (4.0 / 9.0) * q2);
+    } else {
+        const double cT = fma(a, fma(4.5, b, -0.5) + mu, -4.0 * mu);
+        const double cU = fma(b, fma(4.5, a, -0.5) + mu, -4.0 * mu);
+        const double sTS = fma(b, ia, fma(-2.25 * b, b, (mu * (ia * ia)) * cT));
+        const double sUS = fma(a, ib, fma(-2.25 * a, a, (mu * (ib * ib)) * cU));
+        w[0] = L.gg * ((sTS + sUS) / 6.0);
+        w[1] = L.opp * ((4.0 / 9.0) * fma(2.0, mu, q2));
+    }
+}
+
 template <int FAM>
 __device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int mask,
                                                  double u0, double om0, double u1, double om1,
@@ -678,102 +753,40 @@ __device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int 
     const double A = fma(twoz, om2, omz);
     const double B = fma(twoz, u2, omz);
     const double a = 0.5 * A, b = 0.5 * B;
-    const double ab = a * b;
-    const double inv_ab = 1.0 / ab;
-    const double ia = b * inv_ab, ib = a * inv_ab;
     const double sH = (FAM == 0 ? k.fourPT2 : k.fam[FAM - 1].fourMT2) / r;
-    const double pT2 = sH * ab;
+    const double pT2 = sH * (a * b);
 
-    double alpS, cfo = 0.0;
-    if (k.as_order == 0.0) {
-        alpS = k.as_fixed;
-    } else {
-        double Q2 = k.xiR2 * pT2;
-        if (Q2 < k.scale2Min) Q2 = k.scale2Min;
-        const double lq = log(Q2);
-        double lam, cf;
-        if (Q2 > k.mt2 && k.nf6 != 0.0) { lam = k.lnL6s; cf = k.c6; }
-        else if (Q2 > k.mb2)            { lam = k.lnL5s; cf = k.c5; }
-        else if (Q2 > k.mc2)            { lam = k.lnL4s; cf = k.c4; }
-        else                            { lam = k.lnL3s; cf = k.c3; }
-        alpS = cf / (lq - lam);
-        cfo = cf;
-    }
+    double cfo = 0.0;
+    const double alpS = alphas_f64(k, k.xiR2 * pT2, &cfo);
 
-    double Lgg, Lqg, LQQ, Lsame, Lopp, l1a = 0.0, l1b = 0.0, g1, g2, Q1, Q2s;
+    Lumi<double> L;
+    double l1a = 0.0, l1b = 0.0;
     if (gp) {
         const double Q2F = gp->factFac2 * pT2;
         const double q = sqrt(Q2F);
-        const Lumi<double> L = grid_lumi<double>(*gp, (int)(0.5 * k.nSea), x1, lx1, x2, lx2, q, log(q));
-        Lgg = L.gg; Lqg = L.qg; LQQ = L.QQ; Lsame = L.same; Lopp = L.opp;
-        g1 = L.g1; g2 = L.g2; Q1 = L.Q1; Q2s = L.Q2;
+        L = grid_lumi<double>(*gp, (int)(0.5 * k.nSea), x1, lx1, x2, lx2, q, log(q));
     } else {
-        l1a = log(-expm1(lx1)); l1b = log(-expm1(lx2));
-        g1 = exp(fma(k.bg, l1a, fma(k.ag, lx1, k.lnAg)));
-        g2 = exp(fma(k.bg, l1b, fma(k.ag, lx2, k.lnAg)));
-        const double S1 = exp(fma(k.bs, l1a, fma(k.as, lx1, k.lnAs)));
-        const double S2 = exp(fma(k.bs, l1b, fma(k.as, lx2, k.lnAs)));
-        const double V1 = exp(fma(k.bv, l1a, fma(k.av, lx1, k.lnAuv)));
-        const double V2 = exp(fma(k.bv, l1b, fma(k.av, lx2, k.lnAuv)));
-        const double Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
-        const double U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
-        Q1 = fma(k.nSea, S1, V1 + Vd1); Q2s = fma(k.nSea, S2, V2 + Vd2);
-        const double SS = S1 * S2;
-        Lgg = g1 * g2;
-        Lqg = fma(Q1, g2, g1 * Q2s);
-        LQQ = Q1 * Q2s;
-        Lsame = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
-        Lopp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+        L = toy_lumi_f64(k, lx1, lx2, l1a, l1b);
     }
 
-    const double q2 = fma(a, a, b * b);
     const double common = (FAM == 0 ? k.pref : k.fam[FAM - 1].pref) * (alpS * alpS) * (LT * om0) * zMax * r;
     double tot = 0.0;
-    if (FAM == 0) {
-        double w[6];
-        w[0] = Lgg * (2.25 * fma(b * ia, ia, fma(a * ib, ib, 3.0 - ab)));
-        w[1] = Lgg * (k.nQuarkNew * q2 * fma(inv_ab, 1.0 / 6.0, -0.375));
-        w[2] = Lqg * (fma(b, b, 1.0) * fma(ia, ia, (4.0 / 9.0) * ib));
-        {
-            const double sigT = (4.0 / 9.0) * fma(b, b, 1.0) * (ia * ia);
-            const double sigU = (4.0 / 9.0) * fma(a, a, 1.0) * (ib * ib);
-            const double sigTU = -(8.0 / 27.0) * inv_ab;
-            const double sigST = (8.0 / 27.0) * (b * b) * ia;
-            const double csame = 0.5 * (sigU + sigTU - sigT);
-            w[3] = fma(sigT, LQQ, fma(csame, Lsame, sigST * Lopp));
-        }
-        w[4] = Lopp * (q2 * fma(inv_ab, 16.0 / 27.0, -4.0 / 3.0));
-        w[5] = Lopp * (k.nQuarkNew * (4.0 / 9.0) * q2);
+    constexpr int NW = FAM == 0 ? 6 : 2, lo = FAM == 0 ? 0 : 4 + 2 * FAM;
+    double w[NW];
+    brackets_f64<FAM>(k, a, b, FAM == 0 ? 0.0 : k.fam[FAM - 1].mu * r, L, w);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            double wi = (mask & (1 << i)) ? common * w[i] : 0.0;
-            if (!(wi > 0.0)) wi = 0.0;
-            ws[i] = wi;
-            tot += wi;
-        }
-        if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
-    } else {
-        const double mu = k.fam[FAM - 1].mu * r;
-        const double cT = fma(a, fma(4.5, b, -0.5) + mu, -4.0 * mu);
-        const double cU = fma(b, fma(4.5, a, -0.5) + mu, -4.0 * mu);
-        const double sTS = fma(b, ia, fma(-2.25 * b, b, (mu * (ia * ia)) * cT));
-        const double sUS = fma(a, ib, fma(-2.25 * a, a, (mu * (ib * ib)) * cU));
-        double w[2];
-        w[0] = Lgg * ((sTS + sUS) / 6.0);
-        w[1] = Lopp * ((4.0 / 9.0) * fma(2.0, mu, q2));
-        constexpr int lo = 4 + 2 * FAM;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            double wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0;
-            if (!(wi > 0.0)) wi = 0.0;
-            ws[lo + i] = wi;
-            tot += wi;
-        }
-        if (sHo) { *sHo = sH; *tHo = fma(-sH, a, k.fam[FAM - 1].m2); *x1o = x1; *x2o = x2; }
+    for (int i = 0; i < NW; ++i) {
+        double wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0;
+        if (!(wi > 0.0)) wi = 0.0;
+        ws[lo + i] = wi;
+        tot += wi;
+    }
+    if (sHo) {
+        *sHo = sH; *tHo = FAM == 0 ? -(sH * a) : fma(-sH, a, k.fam[FAM - 1].m2); *x1o = x1; *x2o = x2;
     }
     if (aux) {
         aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
-        aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
+        aux->g1 = L.g1; aux->g2 = L.g2; aux->Q1 = L.Q1; aux->Q2 = L.Q2;
     }
     return tot;
 }

@@ -61,7 +61,8 @@ struct mi_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     mi::Grid grid;
     mi::FlowState *flow = nullptr;
-    mi::PdfState *pdf = nullptr;
+    mi::PdfState *pdf = nullptr;                          // slot 0: the grid of pdf_source = MI_PDF_GRID
+    mi::PdfState *pdf_var[MI_PDF_SLOTS] = {};            // slots 1..: variation grids (mi_reweight)
 };
 
 namespace mi {
@@ -77,6 +78,9 @@ int pdf_views(mi_ctx *ctx, const mi_physics &p, GridPdf<float> *g32, GridPdf<dou
 // Host (FP64) view for host-side evaluation; false when p does not use a grid.
 bool pdf_host_view(const mi_ctx *ctx, const mi_physics &p, GridPdf<double> *g);
 void pdf_release(mi_ctx *ctx);
+// FP64 device view of PDF slot s (0 = the grid of mi_pdf_load); MI_ERR_STATE if none is loaded.
+int pdf_slot_view(mi_ctx *ctx, int slot, GridPdf<double> *g64);
+constexpr int kPdfSlots = MI_PDF_SLOTS;
 
 // ------------------------------------------------------------------ launch geometry
 struct Geom { uint32_t blocks; uint32_t ppt; };
@@ -158,6 +162,13 @@ hipError_t launch_scan_counts(hipStream_t s, uint64_t *counts, uint32_t nblk);
 hipError_t launch_unweight_compact(hipStream_t s, const UnwLaunch &L);
 hipError_t launch_unweight_materialize(hipStream_t s, const UnwLaunch &L);
 
+// Scale and PDF variations of events (mi_kernels.hip): per variation xi_R^2, xi_F^2 and a PDF slot.
+struct RwVar { double xiR2, xiF2; int slot, pad; };
+struct GridSet64 { GridPdf<double> g[MI_PDF_SLOTS]; };
+hipError_t launch_reweight_vars(hipStream_t s, const HadConst<double> &k, const GridSet64 &gs, bool grid, uint64_t n,
+                                const double *sH, const double *tH, const double *x1, const double *x2,
+                                const int32_t *pid, const double *w, int nvar, const RwVar *vars_dev,
+                                double xiR2_0, double xiF2_0, double *out);
 // Renormalisation-scale reweighting of events (mi_kernels.hip).
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
                            const int32_t *pid, const double *w, int nvar, const double *fac, double *out);

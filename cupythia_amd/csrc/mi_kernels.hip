@@ -585,6 +585,74 @@ int occ_integrate_c1(bool vegas, bool hist, int nbins)
     return (e == hipSuccess && b > 0) ? b : 1;
 }
 
+// General variations: the event's own process weight recomputed at (xi_R^2 Q^2, xi_F^2 Q^2) with
+// the PDFs of a slot, Q^2 = sH a b (pT^2, or pT^2 + m^2 for the heavy pairs), from the event's
+// (sHat, tHat, x1, x2, code); out = w * W(variation) / W(the run's own scales, slot 0), where
+// W = alphaS^2 * bracket * luminosity of the code (the phase-space Jacobian cancels).
+__device__ __forceinline__ double proc_weight(const HadConst<double> &k, const GridSet64 &gs, bool grid, int fam, int j,
+                                              double a, double b, double mu, double Q2, double x1, double lx1,
+                                              double x2, double lx2, const Lumi<double> &toyL, double xiR2, double xiF2,
+                                              int slot)
+{
+    const double alpS = alphas_f64(k, xiR2 * Q2);
+    Lumi<double> L = toyL;
+    if (grid) {
+        const double q = sqrt(xiF2 * Q2);
+        L = grid_lumi<double>(gs.g[slot], (int)(0.5 * k.nSea), x1, lx1, x2, lx2, q, log(q));
+    }
+    double wb[6];
+    if (fam == 0) brackets_f64<0>(k, a, b, mu, L, wb);
+    else if (fam == 1) brackets_f64<1>(k, a, b, mu, L, wb);
+    else brackets_f64<2>(k, a, b, mu, L, wb);
+    const double br = wb[j] > 0.0 ? wb[j] : 0.0;
+    return alpS * alpS * br;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reweight_vars(HadConst<double> k, GridSet64 gs, int grid, uint64_t n,
+                                                          const double *__restrict__ sH, const double *__restrict__ tH,
+                                                          const double *__restrict__ x1v, const double *__restrict__ x2v,
+                                                          const int32_t *__restrict__ pid, const double *__restrict__ w,
+                                                          int nvar, const RwVar *__restrict__ vars, double xiR2_0,
+                                                          double xiF2_0, double *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int code = pid[i];
+    int fam = -1, j = 0;
+    if (code >= 111 && code <= 116) { fam = 0; j = code - 111; }
+    else if (code == 121 || code == 122) { fam = 1; j = code - 121; }
+    else if (code == 123 || code == 124) { fam = 2; j = code - 123; }
+    if (fam < 0) {                                     // no process (a zero-weight candidate)
+        for (int v = 0; v < nvar; ++v) out[i * nvar + v] = 0.0;
+        return;
+    }
+    const double m2 = fam == 0 ? 0.0 : k.fam[fam - 1].m2;
+    const double s = sH[i], t = tH[i];
+    const double a = (m2 - t) / s, b = (s - m2 + t) / s;  // (m^2 - tH)/sH, (m^2 - uH)/sH
+    const double Q2 = s * (a * b), mu = m2 / s;
+    const double x1 = x1v[i], x2 = x2v[i], lx1 = log(x1), lx2 = log(x2);
+    double l1a, l1b;
+    const Lumi<double> toyL = grid ? Lumi<double>{} : toy_lumi_f64(k, lx1, lx2, l1a, l1b);
+    const double W0 = proc_weight(k, gs, grid != 0, fam, j, a, b, mu, Q2, x1, lx1, x2, lx2, toyL, xiR2_0, xiF2_0, 0);
+    for (int v = 0; v < nvar; ++v) {
+        const RwVar q = vars[v];
+        const double Wv = proc_weight(k, gs, grid != 0, fam, j, a, b, mu, Q2, x1, lx1, x2, lx2, toyL, q.xiR2, q.xiF2,
+                                      q.slot);
+        out[i * nvar + v] = W0 > 0.0 ? w[i] * (Wv / W0) : 0.0;
+    }
+}
+
+hipError_t launch_reweight_vars(hipStream_t s, const HadConst<double> &k, const GridSet64 &gs, bool grid, uint64_t n,
+                                const double *sH, const double *tH, const double *x1, const double *x2,
+                                const int32_t *pid, const double *w, int nvar, const RwVar *vars_dev,
+                                double xiR2_0, double xiF2_0, double *out)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reweight_vars, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, k, gs,
+                       grid ? 1 : 0, n, sH, tH, x1, x2, pid, w, nvar, vars_dev, xiR2_0, xiF2_0, out);
+    return hipGetLastError();
+}
+
 }  // namespace mi
 
 namespace mi {

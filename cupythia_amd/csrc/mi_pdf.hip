@@ -179,13 +179,30 @@ __global__ void k_pdf_xf(GridPdf<double> g, uint64_t n, const double *__restrict
 
 }  // namespace
 
+void release_state(PdfState *&st)
+{
+    if (!st) return;
+    (void)hipFree(st->d64);
+    (void)hipFree(st->d32);
+    delete st;
+    st = nullptr;
+}
+
 void pdf_release(mi_ctx *ctx)
 {
-    if (!ctx || !ctx->pdf) return;
-    (void)hipFree(ctx->pdf->d64);
-    (void)hipFree(ctx->pdf->d32);
-    delete ctx->pdf;
-    ctx->pdf = nullptr;
+    if (!ctx) return;
+    release_state(ctx->pdf);
+    for (int s = 1; s < kPdfSlots; ++s) release_state(ctx->pdf_var[s]);
+}
+
+int pdf_slot_view(mi_ctx *ctx, int slot, GridPdf<double> *g64)
+{
+    PdfState *st = slot == 0 ? ctx->pdf : (slot > 0 && slot < kPdfSlots ? ctx->pdf_var[slot] : nullptr);
+    if (slot < 0 || slot >= kPdfSlots) return pfail(ctx, MI_ERR_ARG, "PDF slot out of range");
+    if (!st) return pfail(ctx, MI_ERR_STATE, "no PDF grid loaded in slot " + std::to_string(slot));
+    *g64 = st->g64;
+    g64->active = 1;
+    return MI_OK;
 }
 
 int pdf_views(mi_ctx *ctx, const mi_physics &p, GridPdf<float> *g32, GridPdf<double> *g64)
@@ -215,10 +232,13 @@ using namespace mi;
 
 extern "C" {
 
-int mi_pdf_load(mi_ctx *ctx, const char *path)
+int mi_pdf_load(mi_ctx *ctx, const char *path) { return mi_pdf_load_slot(ctx, 0, path); }
+
+int mi_pdf_load_slot(mi_ctx *ctx, int slot, const char *path)
 {
     if (!ctx) return MI_ERR_ARG;
     if (!path) return pfail(ctx, MI_ERR_ARG, "path is NULL");
+    if (slot < 0 || slot >= kPdfSlots) return pfail(ctx, MI_ERR_ARG, "PDF slot out of range");
     PdfState *st = new PdfState();
     int rc = parse_lhagrid1(ctx, path, *st);
     if (rc) { delete st; return rc; }
@@ -274,8 +294,9 @@ int mi_pdf_load(mi_ctx *ctx, const char *path)
     fill_view(st->host64, *st);
     st->host64.xk = st->x.data(); st->host64.lnx = st->lnx.data(); st->host64.qk = st->qk.data();
     st->host64.lnq = st->lnq.data(); st->host64.val = st->val.data();
-    pdf_release(ctx);
-    ctx->pdf = st;
+    PdfState *&dst = slot == 0 ? ctx->pdf : ctx->pdf_var[slot];
+    release_state(dst);
+    dst = st;
     return MI_OK;
 }
 

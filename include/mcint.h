@@ -67,6 +67,7 @@ enum mi_mode      { MI_MODE_PARTONIC_1D = 0, MI_MODE_HADRONIC_3D = 1 };
 enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1 };
 enum mi_sampler   { MI_SAMPLER_FLAT = 0, MI_SAMPLER_VEGAS = 1, MI_SAMPLER_FLOW = 2 };
 enum mi_pdf_source { MI_PDF_TOY = 0, MI_PDF_GRID = 1 };
+#define MI_PDF_SLOTS      8    /* PDF grids a context holds: slot 0 (runs) + variation slots 1..7 */
 enum mi_rng       { MI_RNG_PHILOX = 0, MI_RNG_SPLITMIX_COMPAT = 1 };
 
 /* Physics configuration.  Defaults (mi_physics_default) mirror Pythia's settings where the
@@ -255,6 +256,26 @@ int mi_reweight_scales(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const do
                        const double *tHat_dev, const int32_t *pid_dev, const double *w_dev, int nvar,
                        const double *factors_host, double *out_dev);
 
+/* One weight variation: mu_R = renorm_fac * Q, mu_F = fact_fac * Q (Q^2 = pT^2, or pT^2 + m^2 for
+ * the heavy pairs) and the PDFs of a slot (0 = the grid of mi_pdf_load; slots 1..7 from
+ * mi_pdf_load_slot, e.g. the members of an LHAPDF set). */
+typedef struct mi_weight_var {
+    double renorm_fac;
+    double fact_fac;
+    int    pdf_slot;
+} mi_weight_var;
+
+/* Scale and PDF variations of events in one pass: out[i * nvar + k] = w[i] * W_k / W_0, where W is
+ * the event's own process weight -- alphaS(mu_R)^2 times its sigmaHat bracket times its flux sum
+ * over the flavours (Pythia's sigmaPDF for that code) -- at the event's (sHat, tHat, x1, x2),
+ * W_0 at phys's renorm_fac, fact_fac and slot 0.  FP64.  With the toy PDFs (phys->pdf_source =
+ * MI_PDF_TOY) fact_fac has no effect and pdf_slot must be 0.  pid_dev: the events' codes (as
+ * mi_unweight_compact writes them); nvar <= 64.  Replaces the multi-weight stage of
+ * cuPythia/pipeline/reweight.cu:36-67 (there: mu_R only) for scales and PDF members. */
+int mi_reweight(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *sHat_dev, const double *tHat_dev,
+                const double *x1_dev, const double *x2_dev, const int32_t *pid_dev, const double *w_dev,
+                int nvar, const mi_weight_var *vars_host, double *out_dev);
+
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
  * same points, FP64 accumulation.  Hadronic mode with alphaS_order 1 (or 0 for the fixed-
@@ -269,6 +290,9 @@ int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out);
  * (:648-745) with PDF:extrapolate = off.  Replaces cuPythia's grid fill and interpolator
  * (cuPythia/pipeline/genpdf.cc:18-70, pdf.cuh:21-49).  A later call replaces the grid. */
 int mi_pdf_load(mi_ctx *ctx, const char *path);
+/* The same into PDF slot `slot` (0..MI_PDF_SLOTS-1; 0 is mi_pdf_load's): grids for mi_reweight's
+ * PDF variations.  A later call replaces the slot's grid. */
+int mi_pdf_load_slot(mi_ctx *ctx, int slot, const char *path);
 /* Batched PDF evaluation (FP64, device arrays): xf_dev[i * 12 + slot] for slots g, d, u, s, c,
  * b, dbar, ubar, sbar, cbar, bbar (11 = photon, written 0), raw interpolated values (Pythia's
  * PDF::xf clips them at 0, PartonDistributions.cc:122-133).  The counterpart of Pythia's
