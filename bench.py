@@ -168,6 +168,16 @@ def workloads(ctx, mi):
     import torch
     phys = mi.physics_default(1)
     out = {}
+    # partonic gg -> gg flat MC in FP64 (sHat = (100 GeV)^2, alphaS 0.118, |cos| < 0.9, 2^24 points)
+    c1 = ctx.integrate(mi.make_run(mi.physics_default(0), 1 << 24, precision=mi.MI_PREC_FP64, key=42))
+    c1 = ctx.integrate(mi.make_run(mi.physics_default(0), 1 << 24, precision=mi.MI_PREC_FP64, key=42))
+    c1_flops = 23                                   # 20 in the FP64 weight + 3 accumulating
+    out["c1_partonic_fp64"] = {
+        "points": 1 << 24, "kernel_ms": c1.kernel_ms, "points_per_s": (1 << 24) / (c1.kernel_ms / 1e3),
+        "sigma_mb": c1.sigmaGen(), "sigma_err_mb": c1.sigmaErr(),
+        # the reference's Simpson value (03_fused_resident_me/README.md:13) in Pythia's CONVERT2MB
+        "reference_simpson_mb": 1.323768e-4 * (0.389380 / 0.3893793721),
+        "fp64_tflops": c1_flops * (1 << 24) / (c1.kernel_ms / 1e3) / 1e12, "fp64_peak_tflops": 78.6}
     # VEGAS + unweighting (3-D grid, 64 bins, alpha 1.5)
     ctx.vegas_init(3, 64)
     vi = ctx.vegas_adapt(mi.make_run(phys, 1 << 24, key=42), 12, 1.5)
