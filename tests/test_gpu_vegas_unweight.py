@@ -180,3 +180,25 @@ def test_unweight_chunked_equals_split_calls(ctx, exact_max):
     for name in ("index", "sHat", "tHat", "x1", "x2", "weight", "pid"):
         assert torch.equal(getattr(full, name), torch.cat([getattr(a, name), getattr(b, name)])), name
     assert full.estimate.nTried() == n1 + n2 and full.estimate.nAccepted() == full.n
+
+
+def test_unweight_fp64_matches_long_double_decisions(ctx, O):
+    """FP64 unweighting (stored FP64 weights): the accepted indices are the candidates with
+    u * wMax < w for the long-double Pythia-form weights w and the accept uniforms of Philox
+    stream 1, except where w and u * wMax agree to 1e-10 (the FP64 vs long-double tolerance)."""
+    phys = mi.physics_default(1, proc_mask=0x3FF, pTHatMin=10.0)
+    n, wmax = 1 << 16, 5.0
+    ev = ctx.unweight_compact(mi.make_run(phys, n, precision=mi.MI_PREC_FP64, key=31, counter0=77), wmax=wmax)
+    wld, _ = O.had_points_ld(O.hadcfg_default(procMask=0x3FF, pTHatMin=10.0), 31, 77, n)
+    words = O.philox_fill(31, 77, n, 1).reshape(n, 4).astype(np.uint64)
+    u = (((words[:, 1] << np.uint64(32)) | words[:, 0]) >> np.uint64(11)).astype(np.float64) / 2.0 ** 53
+    acc = u * wmax < wld[:, 0]
+    got = np.zeros(n, dtype=bool)
+    got[ev.index.cpu().numpy() - 77] = True
+    close = np.abs(u * wmax - wld[:, 0]) <= 1e-10 * np.maximum(wld[:, 0], 1e-300)
+    assert np.array_equal(got[~close], acc[~close])
+    assert ev.n == int(got.sum()) and ev.n > 100
+    # weights carried: max(w, wMax) of the FP64 weights
+    w_ev = ev.weight.cpu().numpy()
+    ref = np.maximum(wld[ev.index.cpu().numpy() - 77, 0], wmax)
+    assert np.allclose(w_ev, ref, rtol=1e-11, atol=0)
