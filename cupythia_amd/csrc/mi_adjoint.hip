@@ -26,39 +26,46 @@ struct AdjConst {
     T xi;
 };
 
-// Record layout: [kRS] sum w, [kRQ] sum w^2, [kRS + 1 + k] sum d_k, [kRQ + 1 + k] sum d_k^2,
-// [kRNz] nonzero.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> ac, int mask, uint64_t key,
-                                                    uint64_t ctr0, uint64_t n, uint32_t ppt,
-                                                    double *__restrict__ partials)
+// One family's weights and the intermediate values of its point (its own map, x1, x2, alphaS).
+template <typename T, int F> struct FamAux;
+template <int F> struct FamAux<float, F> {
+    static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0, float u1,
+                                                 float o1, float u2, float o2, float *ws, PointAux<float> *ax,
+                                                 const GridPdf<float> *gp)
+    { return had_family_f32<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp); }
+};
+template <int F> struct FamAux<double, F> {
+    static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
+                                                  double u1, double o1, double u2, double o2, double *ws,
+                                                  PointAux<double> *ax, const GridPdf<double> *gp)
+    { return had_family_f64<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp); }
+};
+
+// Derivatives of one family's weights at a point: the coupling terms hold for every code
+// (Q^2 = sH a b for all of them); the gluon-shape terms need the toy gluon (GRID = false):
+// g g channels (111, 112, and 121 / 123) scale with g1 g2, q g (113) with Q1 g2 + g1 Q2.
+template <typename T, int F, bool GRID>
+__device__ __forceinline__ void fam_derivs(const HadConst<T> &k, const AdjConst<T> &ac, int mask, const MapOut<T> &m0,
+                                           const MapOut<T> &m1, const MapOut<T> &m2, const GridPdf<T> *gp, T &tot,
+                                           T &d_as, T &d_xi, T &d_a, T &d_b)
 {
-    double v[kRec];
-#pragma unroll
-    for (int i = 0; i < kRec; ++i) v[i] = 0.0;
-    const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
-    for (uint32_t j = 0; j < ppt; ++j) {
-        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        if (i >= n) break;
-        const U4 r = philox_event(key, ctr0 + i, 0u);
-        const MapOut<T> m0 = map_flat<T>(k24(r.x)), m1 = map_flat<T>(k24(r.y)), m2 = map_flat<T>(k24(r.z));
-        T ws[kMaxProc];
-        PointAux<T> ax;
-        const T tot = HadPointAux<T>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, &ax);
-        // coupling
-        T d_as, d_xi;
-        if (k.as_order == (T)0) {
-            d_as = ac.as_fac * tot / ax.alpS;
-            d_xi = (T)0;
-        } else {
-            d_as = ac.as_fac * tot * ax.alpS;
-            d_xi = -(T)4 * tot * ax.alpS / (ax.cf * ac.xi);
-        }
-        // gluon shape: gg channels (111, 112) scale with g1 g2, qg (113) with Q1 g2 + g1 Q2
-        const T la1 = ax.lx1 + ac.dlnAg_da, la2 = ax.lx2 + ac.dlnAg_da;
-        const T lb1 = ax.l1a + ac.dlnAg_db, lb2 = ax.l1b + ac.dlnAg_db;
-        const T wgg = ws[0] + ws[1];
-        T d_a = wgg * (la1 + la2), d_b = wgg * (lb1 + lb2);
+    T ws[kMaxProc];
+    PointAux<T> ax;
+    const T tf = FamAux<T, F>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, &ax, gp);
+    tot += tf;
+    if (k.as_order == (T)0) {
+        d_as += ac.as_fac * tf / ax.alpS;
+    } else {
+        d_as += ac.as_fac * tf * ax.alpS;
+        d_xi += -(T)4 * tf * ax.alpS / (ax.cf * ac.xi);
+    }
+    if (GRID) return;
+    const T la1 = ax.lx1 + ac.dlnAg_da, la2 = ax.lx2 + ac.dlnAg_da;
+    const T lb1 = ax.l1a + ac.dlnAg_db, lb2 = ax.l1b + ac.dlnAg_db;
+    const T wgg = F == 0 ? ws[0] + ws[1] : ws[4 + 2 * F];
+    d_a += wgg * (la1 + la2);
+    d_b += wgg * (lb1 + lb2);
+    if (F == 0) {
         const T Lqg = ax.Q1 * ax.g2 + ax.g1 * ax.Q2;
         if (ws[2] > (T)0 && Lqg > (T)0) {
             const T fa = (ax.Q1 * ax.g2 * la2 + ax.g1 * ax.Q2 * la1) / Lqg;
@@ -66,6 +73,34 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> a
             d_a += ws[2] * fa;
             d_b += ws[2] * fb;
         }
+    }
+}
+
+// Record layout: [kRS] sum w, [kRQ] sum w^2, [kRS + 1 + k] sum d_k, [kRQ + 1 + k] sum d_k^2,
+// [kRNz] nonzero.  HEAVY: the families of the heavy-quark codes as well; GRID: parton densities
+// from the PDF grid (the gluon-shape derivatives are then 0).
+template <typename T, bool HEAVY, bool GRID>
+__global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> ac, int mask, uint64_t key,
+                                                    uint64_t ctr0, uint64_t n, uint32_t ppt,
+                                                    double *__restrict__ partials, GridPdf<T> gpdf)
+{
+    double v[kRec];
+#pragma unroll
+    for (int i = 0; i < kRec; ++i) v[i] = 0.0;
+    const GridPdf<T> *gp = GRID ? &gpdf : nullptr;
+    const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
+    for (uint32_t j = 0; j < ppt; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i >= n) break;
+        const U4 r = philox_event(key, ctr0 + i, 0u);
+        const MapOut<T> m0 = map_flat<T>(k24(r.x)), m1 = map_flat<T>(k24(r.y)), m2 = map_flat<T>(k24(r.z));
+        T tot = (T)0, d_as = (T)0, d_xi = (T)0, d_a = (T)0, d_b = (T)0;
+        if (!HEAVY || (mask & kFamMask[0]))
+            fam_derivs<T, 0, GRID>(k, ac, mask, m0, m1, m2, gp, tot, d_as, d_xi, d_a, d_b);
+        if (HEAVY && (mask & kFamMask[1]))
+            fam_derivs<T, 1, GRID>(k, ac, mask, m0, m1, m2, gp, tot, d_as, d_xi, d_a, d_b);
+        if (HEAVY && (mask & kFamMask[2]))
+            fam_derivs<T, 2, GRID>(k, ac, mask, m0, m1, m2, gp, tot, d_as, d_xi, d_a, d_b);
         const double dt = (double)tot;
         v[kRS] += dt;
         v[kRQ] = fma(dt, dt, v[kRQ]);
@@ -80,13 +115,39 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> a
     block_record(v, partials + (size_t)blockIdx.x * kRec);
 }
 
-int occ_adjoint(int prec)
+template <typename T, bool HEAVY, bool GRID>
+int occ_adjoint_t()
 {
     int nb = 0;
-    hipError_t e = (prec == MI_PREC_FP32)
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<float>, kBlock, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<double>, kBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint<T, HEAVY, GRID>, kBlock, 0);
     return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+template <typename T>
+int occ_adjoint_p(bool heavy, bool grid)
+{
+    if (heavy) return grid ? occ_adjoint_t<T, true, true>() : occ_adjoint_t<T, true, false>();
+    return grid ? occ_adjoint_t<T, false, true>() : occ_adjoint_t<T, false, false>();
+}
+
+int occ_adjoint(int prec, bool heavy, bool grid)
+{
+    return prec == MI_PREC_FP32 ? occ_adjoint_p<float>(heavy, grid) : occ_adjoint_p<double>(heavy, grid);
+}
+
+template <typename T>
+void launch_adjoint_t(hipStream_t s, dim3 g, const HadConst<T> &k, const AdjConst<T> &ac, int mask, uint64_t key,
+                      uint64_t ctr0, uint64_t n, uint32_t ppt, double *partials, const GridPdf<T> &gp)
+{
+    const bool hv = has_heavy(mask), gr = gp.active != 0;
+    if (hv && gr)
+        hipLaunchKernelGGL((k_adjoint<T, true, true>), g, dim3(kBlock), 0, s, k, ac, mask, key, ctr0, n, ppt, partials, gp);
+    else if (hv)
+        hipLaunchKernelGGL((k_adjoint<T, true, false>), g, dim3(kBlock), 0, s, k, ac, mask, key, ctr0, n, ppt, partials, gp);
+    else if (gr)
+        hipLaunchKernelGGL((k_adjoint<T, false, true>), g, dim3(kBlock), 0, s, k, ac, mask, key, ctr0, n, ppt, partials, gp);
+    else
+        hipLaunchKernelGGL((k_adjoint<T, false, false>), g, dim3(kBlock), 0, s, k, ac, mask, key, ctr0, n, ppt, partials, gp);
 }
 
 }  // namespace mi
@@ -117,18 +178,32 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     if (run->sampler != MI_SAMPLER_FLAT) { ctx->err = "the adjoint samples the flat map"; return MI_ERR_ARG; }
     if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64) { ctx->err = "bad precision"; return MI_ERR_ARG; }
     if (run->n_points == 0 || run->n_points > (1ULL << 40)) { ctx->err = "n_points must be 1..2^40"; return MI_ERR_RANGE; }
-    if ((p.proc_mask & 0x3F) == 0 || (p.proc_mask & ~0x3F)) {
-        ctx->err = "the adjoint covers the massless codes 111..116 (proc_mask bits 0..5)";
+    if (p.proc_mask == 0 || (p.proc_mask & ~0x3FF)) {
+        ctx->err = "proc_mask must select codes among 111..116, 121..124 (bits 0..9)";
         return MI_ERR_ARG;
     }
-    if (p.pdf_source != MI_PDF_TOY) { ctx->err = "the adjoint's PDF-shape derivatives need the toy PDFs"; return MI_ERR_ARG; }
+    for (int f = 0; f < 2; ++f)
+        if ((p.proc_mask & kFamMask[1 + f]) &&
+            (!(p.mass_Q[f] > 0.0) || !(p.eCM * p.eCM > 4.0 * (p.mass_Q[f] * p.mass_Q[f] + p.pTHatMin * p.pTHatMin)))) {
+            ctx->err = "heavy-quark codes need mass_Q > 0 and eCM > 2 mT";
+            return MI_ERR_ARG;
+        }
+    if (p.pdf_source != MI_PDF_TOY && p.pdf_source != MI_PDF_GRID) { ctx->err = "unknown pdf_source"; return MI_ERR_ARG; }
+    if (p.pdf_source == MI_PDF_GRID && !(p.fact_fac > 0.0)) { ctx->err = "fact_fac must be > 0"; return MI_ERR_ARG; }
+    GridPdf<float> gf;
+    GridPdf<double> gd;
+    {
+        const int rc = pdf_views(ctx, p, &gf, &gd);
+        if (rc) return rc;
+    }
     if (!(p.alphaS_value > 0.0) || !(p.renorm_fac > 0.0) || !(p.eCM > 2.0 * p.pTHatMin) || !(p.pTHatMin > 0.0) ||
         p.nQuarkIn < 2 || p.nQuarkIn > 5 || !(p.pdf_g[0] > -1.0) || !(p.pdf_g[1] > -1.0)) {
         ctx->err = "physics configuration out of range";
         return MI_ERR_ARG;
     }
     if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return MI_ERR_HIP; }
-    const Geom g = geom_for(run->n_points, 16, (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ_adjoint(run->precision));
+    const Geom g = geom_for(run->n_points, 16, (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) *
+                                                   occ_adjoint(run->precision, has_heavy(p.proc_mask), gf.active != 0));
     if (g.blocks * (size_t)kRec > ctx->partials_cap) {
         (void)hipFree(ctx->partials);
         ctx->partials = nullptr;
@@ -149,14 +224,14 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
         HadConst<float> k;
         fill_had_consts_f32(p, k);
         AdjConst<float> ac{(float)dA_da, (float)dA_db, (float)as_fac, (float)p.renorm_fac};
-        hipLaunchKernelGGL(k_adjoint<float>, dim3(g.blocks), dim3(kBlock), 0, ctx->stream, k, ac, p.proc_mask, run->key,
-                           run->counter0, run->n_points, g.ppt, ctx->partials);
+        launch_adjoint_t<float>(ctx->stream, dim3(g.blocks), k, ac, p.proc_mask, run->key, run->counter0, run->n_points,
+                                g.ppt, ctx->partials, gf);
     } else {
         HadConst<double> k;
         fill_had_consts_f64(p, k);
         AdjConst<double> ac{dA_da, dA_db, as_fac, p.renorm_fac};
-        hipLaunchKernelGGL(k_adjoint<double>, dim3(g.blocks), dim3(kBlock), 0, ctx->stream, k, ac, p.proc_mask, run->key,
-                           run->counter0, run->n_points, g.ppt, ctx->partials);
+        launch_adjoint_t<double>(ctx->stream, dim3(g.blocks), k, ac, p.proc_mask, run->key, run->counter0, run->n_points,
+                                 g.ppt, ctx->partials, gd);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);

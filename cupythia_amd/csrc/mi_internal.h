@@ -173,12 +173,4 @@ hipError_t launch_reweight_vars(hipStream_t s, const HadConst<double> &k, const 
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
                            const int32_t *pid, const double *w, int nvar, const double *fac, double *out);
 
-// Adjoint (mi_adjoint.hip): partials of kRec doubles per block, see that file for the layout.
-struct AdjLaunch {
-    HadLaunch h;
-    double dlnA_dasMZ[4];   // d ln(Lambda_nf^2) / d alphaS(MZ) for nf = 6, 5, 4, 3
-    double lnxiR2;
-};
-hipError_t launch_adjoint(hipStream_t s, const AdjLaunch &L);
-
 }  // namespace mi

@@ -94,18 +94,6 @@ __device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbin
     return m;
 }
 
-template <typename T> struct HadPointAux;
-template <> struct HadPointAux<float> {
-    static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0, float u1,
-                                                 float o1, float u2, float o2, float *ws, PointAux<float> *ax)
-    { return had_point_f32<false>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
-};
-template <> struct HadPointAux<double> {
-    static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0, double u1,
-                                                  double o1, double u2, double o2, double *ws, PointAux<double> *ax)
-    { return had_point_f64<false>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax); }
-};
-
 // gp: the PDF grid, or nullptr for the toy PDFs.  HEAVY: see had_point_f32.
 template <typename T, bool HEAVY> struct HadPoint;
 template <bool HEAVY> struct HadPoint<float, HEAVY> {

@@ -278,8 +278,10 @@ int mi_reweight(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *s
 
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
- * same points, FP64 accumulation.  Hadronic mode with alphaS_order 1 (or 0 for the fixed-
- * coupling alphaS^2 check) and the toy PDFs (the gluon shape parameters are theirs). */
+ * same points, FP64 accumulation.  Hadronic mode, flat sampler, any proc_mask within bits 0..9
+ * (each heavy-quark family at its own scale pT^2 + m^2), alphaS_order 1 (or 0 for the fixed-
+ * coupling alphaS^2 check).  The gluon shape parameters a, b are the toy PDFs' own: with
+ * pdf_source = MI_PDF_GRID their derivatives (and errors) are 0. */
 int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out);
 
 /* ------------------------------------------------------------- PDF grids ---- */

@@ -1,6 +1,8 @@
 """GPU adjoint (mi_adjoint) against the oracle, finite differences with common random numbers,
 the alphaS^2 scaling, and the torch.autograd harness."""
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -27,6 +29,49 @@ def test_adjoint_vs_oracle(ctx, O, prec, tol):
     got = np.array([g.sigma, g.d_alphaS_MZ, g.d_renorm_fac, g.d_pdf_g_a, g.d_pdf_g_b]) * n
     rel = np.abs(got - r[:5]) / np.abs(r[:5])
     assert rel.max() < tol, rel
+
+
+@pytest.mark.parametrize("mask", [0x3FF, 0x3C0, 0x100, 0x200])
+@pytest.mark.parametrize("prec,tol", [(mi.MI_PREC_FP64, 1e-7), (mi.MI_PREC_FP32, 3e-5)])
+def test_adjoint_heavy_vs_oracle(ctx, O, mask, prec, tol):
+    """The heavy-quark families (codes 121..124, pT > 5 GeV): every family's coupling at its own
+    scale pT^2 + m^2, the gluon shape through the g g -> Q Qbar luminosity."""
+    n = 1 << 16
+    g = ctx.adjoint(mi.make_run(mi.physics_default(1, proc_mask=mask, pTHatMin=5.0), n, precision=prec, key=42))
+    r = oracle_adj(O, O.hadcfg_default(procMask=mask, pTHatMin=5.0), n)
+    got = np.array([g.sigma, g.d_alphaS_MZ, g.d_renorm_fac, g.d_pdf_g_a, g.d_pdf_g_b]) * n
+    k = 3 if mask == 0x200 else 5
+    rel = np.abs(got[:k] - r[:k]) / np.abs(r[:k])
+    assert rel.max() < tol, rel
+    if mask == 0x200:                      # q qbar -> b bbar alone: no gluon in the initial state
+        assert r[3] == 0.0 and r[4] == 0.0
+        assert g.d_pdf_g_a == 0.0 and g.d_pdf_g_b == 0.0
+
+
+@pytest.fixture(scope="module")
+def grid_file(tmp_path_factory, O):
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import pdfgrid_util as PG
+    path = str(tmp_path_factory.mktemp("adjgrid") / "toyq.dat")
+    PG.write_lhagrid1(path, PG.nnpdf_like_x_grid(50, 50, 1e-9), [PG.q_grid(1.0, 1.0e4, 50)], PG.toy_fn(O, Qdep=0.05))
+    return path
+
+
+@pytest.mark.parametrize("mask", [0x3F, 0x3FF])
+@pytest.mark.parametrize("prec,tol", [(mi.MI_PREC_FP64, 1e-7), (mi.MI_PREC_FP32, 3e-5)])
+def test_adjoint_grid_vs_oracle(ctx, O, grid_file, mask, prec, tol):
+    """Parton densities from a PDF grid: the coupling derivatives as with the toy PDFs, the toy
+    gluon's shape parameters then enter nothing (derivatives exactly 0)."""
+    ctx.pdf_load(grid_file)
+    n = 1 << 16
+    phys = mi.physics_default(1, proc_mask=mask, pTHatMin=5.0, pdf_source=mi.MI_PDF_GRID, fact_fac=1.3)
+    g = ctx.adjoint(mi.make_run(phys, n, precision=prec, key=42))
+    r = oracle_adj(O, O.hadcfg_default(procMask=mask, pTHatMin=5.0, grid=O.LhaGrid(grid_file), factFac=1.3), n)
+    got = np.array([g.sigma, g.d_alphaS_MZ, g.d_renorm_fac]) * n
+    rel = np.abs(got - r[:3]) / np.abs(r[:3])
+    assert rel.max() < tol, rel
+    assert r[3] == 0.0 and r[4] == 0.0
+    assert g.d_pdf_g_a == 0.0 and g.d_pdf_g_b == 0.0 and g.d_pdf_g_a_err == 0.0
 
 
 def test_alphas_squared_scaling(ctx):

@@ -235,8 +235,8 @@ def test_heavy_errors(ctx):
         ctx.integrate(mi.make_run(mi.physics_default(1, proc_mask=0x400), 1024))
     with pytest.raises(mi.MIError):                       # 2 pTmin < eCM < 2 mT of the b pair
         ctx.integrate(mi.make_run(mi.physics_default(1, proc_mask=0x300, eCM=12.0, pTHatMin=5.0), 1024))
-    with pytest.raises(mi.MIError):
-        ctx.adjoint(mi.make_run(phys_h(0x3FF), 1 << 16))
+    with pytest.raises(mi.MIError):                       # the adjoint checks the heavy masses as well
+        ctx.adjoint(mi.make_run(phys_h(0x0C0, mass_Q=(0.0, 4.8)), 1 << 16))
     # the light codes of the same physics are unaffected by the heavy masses
     a = ctx.integrate(mi.make_run(phys_h(0x3F, mass_Q=(1.2, 4.5)), 1 << 16, key=3))
     b = ctx.integrate(mi.make_run(phys_h(0x3F), 1 << 16, key=3))

@@ -120,6 +120,6 @@ def test_grid_errors(ctx, O, grid_files):
             fresh.pdf_load("/nonexistent/file.dat")
     ctx.pdf_load(grid_files[0])
     with pytest.raises(mi.MIError):
-        ctx.adjoint(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID), 1 << 16))
+        ctx.adjoint(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, fact_fac=0.0), 1 << 16))
     with pytest.raises(mi.MIError):
         ctx.integrate(mi.make_run(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, fact_fac=0.0), 1024))
