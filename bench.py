@@ -191,6 +191,19 @@ def workloads(ctx, mi):
         "unweight_candidates": 1 << 26, "accepted": ev.n, "unweight_ms": ev.estimate.kernel_ms,
         "candidates_per_s": (1 << 26) / (ev.estimate.kernel_ms / 1e3), "event_bytes": nb,
         "note": "unweight_ms covers the exact-max pass, count, scan and write launches"}
+    # device event record of the accepted events (flavours, colour flow, lab four-momenta); the
+    # second call is timed (wall clock of the entry point, synchronised; the kernel time is in the
+    # rocprofv3 trace)
+    rec = ctx.build_events(phys, ev, key=42)
+    del rec
+    t0 = time.perf_counter()
+    rec = ctx.build_events(phys, ev, key=42)
+    wall = time.perf_counter() - t0
+    eb = ev.n * (4 * 8 + 4 + 8) + ev.n * (4 * (5 * 8 + 8 * 4) + 4 + 8)
+    out["event_record"] = {"events": ev.n, "wall_ms": wall * 1e3, "events_per_s_wall": ev.n / wall,
+                           "algorithmic_bytes": eb,
+                           "note": "reads sHat, tHat, x1, x2, pid, index (44 B), writes 4 partons x 72 B + 12 B"}
+    del rec
     # adjoint
     g = ctx.adjoint(mi.make_run(phys, 1 << 26, key=42))
     out["adjoint"] = {"points": 1 << 26, "kernel_ms": g.kernel_ms, "points_per_s": (1 << 26) / (g.kernel_ms / 1e3),

@@ -23,7 +23,7 @@ from ._abi import (  # noqa: F401
     MI_QQBAR2CCBAR, MI_GG2BBBAR, MI_QQBAR2BBBAR, PROC_CODES,
 )
 from .core import (  # noqa: F401
-    Context, Events, MIError, SigmaEstimate, make_run, physics_default,
+    Context, EventRecord, Events, MIError, SigmaEstimate, make_run, physics_default,
 )
 from .processes import (  # noqa: F401
     PROCESSES, Sigma2gg2gg, Sigma2gg2qqbar, Sigma2qg2qg, Sigma2qq2qq, Sigma2qqbar2gg,

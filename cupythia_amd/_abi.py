@@ -30,7 +30,7 @@ MI_PREC_FP32, MI_PREC_FP64 = 0, 1
 MI_SAMPLER_FLAT, MI_SAMPLER_VEGAS, MI_SAMPLER_FLOW = 0, 1, 2
 MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT = 0, 1
 MI_PDF_TOY, MI_PDF_GRID = 0, 1
-MI_ABI_VERSION = 3
+MI_ABI_VERSION = 4
 MI_PDF_SLOTS = 8
 
 MI_FLOW_LAYERS, MI_FLOW_BINS, MI_FLOW_HIDDEN, MI_FLOW_NOUT = 6, 16, 64, 49
@@ -71,6 +71,14 @@ class MiEvents(C.Structure):
                 ("weight", C.c_void_p), ("pid", C.c_void_p), ("index", C.c_void_p), ("capacity", U64)]
 
 
+class MiParticles(C.Structure):
+    _fields_ = [("px", C.c_void_p), ("py", C.c_void_p), ("pz", C.c_void_p), ("e", C.c_void_p), ("m", C.c_void_p),
+                ("pdg", C.c_void_p), ("status", C.c_void_p), ("col", C.c_void_p), ("acol", C.c_void_p),
+                ("mo1", C.c_void_p), ("mo2", C.c_void_p), ("d1", C.c_void_p), ("d2", C.c_void_p),
+                ("n_part", C.c_void_p), ("scale", C.c_void_p), ("capacity", U64), ("max_part", I),
+                ("reserved", I)]
+
+
 class MiWeightVar(C.Structure):
     _fields_ = [("renorm_fac", D), ("fact_fac", D), ("pdf_slot", I)]
 
@@ -107,6 +115,7 @@ PROTOTYPES = {
     "mi_vegas_set_grid": ([VP, P(D), I, I], I),
     "mi_unweight_compact": ([VP, P(MiRun), D, P(MiEvents), P(U64), P(D), P(MiSigmaEst)], I),
     "mi_write_lhe": ([VP, C.c_char_p, P(MiPhysics), P(MiEvents), U64, U64, P(MiSigmaEst)], I),
+    "mi_build_events": ([VP, P(MiPhysics), P(MiEvents), U64, U64, P(MiParticles)], I),
     "mi_reweight_scales": ([VP, P(MiPhysics), U64, VP, VP, VP, VP, I, P(D), VP], I),
     "mi_adjoint": ([VP, P(MiRun), P(MiSigmaGrad)], I),
     "mi_pdf_load": ([VP, C.c_char_p], I),

@@ -99,6 +99,31 @@ class Events:
     estimate: Optional[SigmaEstimate] = None
 
 
+@dataclass
+class EventRecord:
+    """Device event record of n events (mi_build_events): per-particle tensors of shape
+    (n, max_part), particles 0, 1 incoming (status -21), 2, 3 outgoing (status 23)."""
+    px: torch.Tensor
+    py: torch.Tensor
+    pz: torch.Tensor
+    e: torch.Tensor
+    m: torch.Tensor
+    pdg: torch.Tensor
+    status: torch.Tensor
+    col: torch.Tensor
+    acol: torch.Tensor
+    mo1: torch.Tensor
+    mo2: torch.Tensor
+    d1: torch.Tensor
+    d2: torch.Tensor
+    n_part: torch.Tensor
+    scale: torch.Tensor
+
+    def p4(self) -> torch.Tensor:
+        """(n, max_part, 4) four-momenta (px, py, pz, E)."""
+        return torch.stack([self.px, self.py, self.pz, self.e], dim=-1)
+
+
 def _ptr(t: Optional[torch.Tensor]):
     if t is None:
         return None
@@ -245,6 +270,24 @@ class Context:
         self.sync_stream()
         self._check(self.lib.mi_write_lhe(self.h, path.encode(), C.byref(phys), C.byref(raw), ev.n, key,
                                           C.byref(est)), "mi_write_lhe")
+
+    def build_events(self, phys: A.MiPhysics, ev: "Events", key: int, max_part: int = 4) -> EventRecord:
+        """The partons of compacted events on the device: flavours, colour flow, lab four-momenta
+        (Philox streams 3 and 4 of each event's index under key)."""
+        n, d = ev.n, self.dev
+        f64, i32 = dict(dtype=torch.float64, device=d), dict(dtype=torch.int32, device=d)
+        shp = (n, max_part)
+        r = EventRecord(*[torch.zeros(shp, **f64) for _ in range(5)], *[torch.zeros(shp, **i32) for _ in range(8)],
+                        torch.zeros(n, **i32), torch.zeros(n, **f64))
+        raw = A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
+                         _ptr(ev.index), n)
+        out = A.MiParticles(_ptr(r.px), _ptr(r.py), _ptr(r.pz), _ptr(r.e), _ptr(r.m), _ptr(r.pdg), _ptr(r.status),
+                            _ptr(r.col), _ptr(r.acol), _ptr(r.mo1), _ptr(r.mo2), _ptr(r.d1), _ptr(r.d2),
+                            _ptr(r.n_part), _ptr(r.scale), n, max_part, 0)
+        self.sync_stream()
+        self._check(self.lib.mi_build_events(self.h, C.byref(phys), C.byref(raw), n, key, C.byref(out)),
+                    "mi_build_events")
+        return r
 
     # -------------------------------------------------------------- reweighting
     def reweight_scales(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, w: torch.Tensor,

@@ -169,6 +169,20 @@ hipError_t launch_reweight_vars(hipStream_t s, const HadConst<double> &k, const 
                                 const double *sH, const double *tH, const double *x1, const double *x2,
                                 const int32_t *pid, const double *w, int nvar, const RwVar *vars_dev,
                                 double xiR2_0, double xiF2_0, double *out);
+// Device event record (mi_events.hip).
+struct EvConst {
+    double eBeam;                        // per beam (partonic: sqrt(sHat) / 2)
+    double renorm_fac, nQuarkNew;
+    double Ag, ag, bg, Auv, Adv, av, bv, As, as, bs;   // toy xf = A x^a (1 - x)^b
+    double mQ[2];
+    int nQ, partonic, grid, max_part;
+};
+// Constants of the event build for p (validates p; the grid's device view when p uses one).
+int event_consts(mi_ctx *ctx, const mi_physics &p, int max_part, EvConst *c, GridPdf<double> *g);
+// Builds events first .. first + n - 1 of ev into record slots 0 .. n - 1 of out.
+hipError_t launch_build_events(hipStream_t s, const EvConst &c, const GridPdf<double> &g, uint64_t n, uint64_t key,
+                               const mi_events &ev, uint64_t first, const mi_particles &out);
+
 // Renormalisation-scale reweighting of events (mi_kernels.hip).
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
                            const int32_t *pid, const double *w, int nvar, const double *fac, double *out);

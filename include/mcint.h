@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 3
+#define MI_ABI_VERSION 4
 
 typedef struct mi_ctx mi_ctx;   /* opaque */
 
@@ -234,12 +234,43 @@ int mi_vegas_set_grid(mi_ctx *ctx, const double *edges_host, int ndim, int nbins
 int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_events *out,
                         uint64_t *n_acc, double *wmax_used, mi_sigma_est *est);
 
+/* ------------------------------------------------------ device event record ---- */
+/* Partons of the compacted events on the device, structure of arrays in caller-owned device
+ * memory: particle p of event e at e * max_part + p (max_part >= 4; slots beyond n_part[e] are
+ * left untouched, headroom for later stages).  Replaces cuPythia/pipeline/event.cuh:20-31
+ * (DeviceEvents); x1, x2 and the weight stay in mi_events. */
+typedef struct mi_particles {
+    double   *px, *py, *pz, *e, *m;        /* [capacity * max_part], GeV, lab frame */
+    int32_t  *pdg, *status;                /* PDG id; -21 incoming, 23 outgoing (Pythia's codes) */
+    int32_t  *col, *acol;                  /* colour tags 501, 502, ... (0 = none) */
+    int32_t  *mo1, *mo2, *d1, *d2;         /* mothers / daughters as record slots, -1 = none */
+    int32_t  *n_part;                      /* [capacity] particles written (4; 0 for an unknown code) */
+    double   *scale;                       /* [capacity] renorm_fac sqrt(pT^2 + m^2), GeV */
+    uint64_t capacity;                     /* events */
+    int32_t  max_part;
+    int32_t  reserved;
+} mi_particles;
+
+/* Builds the record of n compacted events (ev, e.g. from mi_unweight_compact) on the device, one
+ * thread per event, as Pythia completes a hard process:
+ *   incoming flavours: a pair from the code's flux list (SigmaProcess::initFlux) drawn with
+ *     weight xf1 xf2 dsigmaHat/dtHat (SigmaProcess::pickInState, SigmaProcess.cc:486-508);
+ *   outgoing flavours and colour flow: the code's setIdColAcol (SigmaQCD.cc:135-394, 768-781,
+ *     894-905);
+ *   momenta: incoming partons along +-z with E = x sqrt(S)/2, the outgoing pair back to back in
+ *     the partonic frame at the record's tHat and a uniform azimuth, boosted to the lab
+ *     (PhaseSpace.cc:2236-2251); partonic mode: sqrt(sHat)/2 per beam, x = 1.
+ * Random numbers: Philox stream 3 (words 0..3) and stream 4 (word 0) of ev->index[i] under `key`,
+ * so the record is a function of (key, event index).  phys: the run's physics (PDFs for the
+ * flavour draw, masses, renorm_fac).  Replaces cuPythia/pipeline/build_events.cu:22-41 and
+ * cuPythia/12_phase_space/phase_space.cu:37-53. */
+int mi_build_events(mi_ctx *ctx, const mi_physics *phys, const mi_events *ev, uint64_t n, uint64_t key,
+                    const mi_particles *out);
+
 /* -------------------------------------------------------------- LHE output ---- */
-/* LHEF 3.0 file of n compacted events (device SoA, e.g. from mi_unweight_compact): incoming
- * flavours drawn from the process's flux list with weight xf1 xf2 dsigma/dt, outgoing flavours and
- * colour flow per Pythia's setIdColAcol, azimuth uniform -- all from Philox streams 3 and 4 of
- * the event index under `key`; momenta in the lab; weights and cross sections (est->sigmaGen per
- * process code) in pb.  Replaces cuPythia/pipeline/lhe_writer.cu:46-70 and
+/* LHEF 3.0 file of n compacted events (device SoA, e.g. from mi_unweight_compact): the partons
+ * of mi_build_events (same key: flavours, colour flow, lab momenta), weights and cross sections
+ * (est->sigmaGen per process code) in pb.  Replaces cuPythia/pipeline/lhe_writer.cu:46-70 and
  * cuPythia/07_unweighting_lhe/unweight_lhe.cu:74-102. */
 int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phys, const mi_events *ev, uint64_t n,
                  uint64_t key, const mi_sigma_est *est);

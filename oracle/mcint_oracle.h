@@ -248,6 +248,26 @@ void orc_had_mc(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, in
    Pinned: FD of a closed form (fixed alphaS: d ln sigma / d ln alphaS = 2 exactly). */
 void orc_adjoint(const orc_hadcfg *c, uint64_t key, uint64_t ctr0, uint64_t n, double out[9]);
 
+/* ------------------------------------------------------------- event record ---- */
+/* The partons completing a compacted event (oracle_events.c): incoming pair by pickInState over
+   the code's flux list, outgoing flavours and colour flow by setIdColAcol, lab momenta by
+   finalKin (rotation + boost), hard scale renormFac sqrt(pT^2 + m^2).  Colour tags 501.. (0 none),
+   status -21 / 23, mothers / daughters as slots (-1 none); p[k] = (px, py, pz, e, m).  partonic:
+   g g -> g g at sqrt(sHatPartonic) per collision, x1 = x2 = 1.  An unknown code gives nPart = 0.
+   Pinned: tests/test_oracle_events.py. */
+typedef struct {
+  int pdg[4], status[4], col[4], acol[4], mo1[4], mo2[4], d1[4], d2[4];
+  double p[4][5];
+  double scale;
+  int nPart;
+} orc_event4;
+void orc_build_event(const orc_hadcfg *c, int partonic, double sHatPartonic, uint64_t key, uint64_t index, int code,
+                     double sH, double tH, double x1, double x2, orc_event4 *out);
+/* n events; ints[i][k][8] = (pdg, status, col, acol, mo1, mo2, d1, d2), mom[i][k][5]. */
+void orc_build_events(const orc_hadcfg *c, int partonic, double sHatPartonic, uint64_t key, uint64_t n,
+                      const uint64_t *index, const int32_t *code, const double *sH, const double *tH,
+                      const double *x1, const double *x2, int32_t *ints, double *mom, double *scale, int32_t *nPart);
+
 #ifdef __cplusplus
 }
 #endif

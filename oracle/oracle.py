@@ -163,6 +163,8 @@ def _bind_extras(L):
         "orc_had_mc": ([P(OrcHadCfg), u64, u64, u64, i, P(d)], None),
         "orc_sigma_hat_massive": ([i, d, d, d, d, d], d),
         "orc_sigma_tot_massive_textbook": ([i, d, d, d], d),
+        "orc_build_events": ([P(OrcHadCfg), i, d, u64, u64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], None),
     }
     for name, (argt, rest) in specs.items():
         if hasattr(L, name):
@@ -174,6 +176,31 @@ def _bind_extras(L):
 # ------------------------------------------------------------------------- helpers
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+EVENT_INT_FIELDS = ("pdg", "status", "col", "acol", "mo1", "mo2", "d1", "d2")
+
+
+def build_events(cfg, key, index, code, sH, tH, x1=None, x2=None, partonic=False, sHat_partonic=0.0):
+    """The partons of n compacted events (oracle_events.c): a dict of (n, 4) arrays for
+    EVENT_INT_FIELDS, p (n, 4, 4) = (px, py, pz, e), m (n, 4), scale (n,), n_part (n,)."""
+    index = np.ascontiguousarray(index, dtype=np.uint64)
+    n = index.size
+    code = np.ascontiguousarray(code, dtype=np.int32)
+    sH = np.ascontiguousarray(sH, dtype=np.float64)
+    tH = np.ascontiguousarray(tH, dtype=np.float64)
+    x1 = None if x1 is None else np.ascontiguousarray(x1, dtype=np.float64)
+    x2 = None if x2 is None else np.ascontiguousarray(x2, dtype=np.float64)
+    ints = np.zeros((n, 4, 8), dtype=np.int32)
+    mom = np.zeros((n, 4, 5))
+    scale = np.zeros(n)
+    npart = np.zeros(n, dtype=np.int32)
+    ptr = (lambda a: None if a is None else a.ctypes.data)
+    lib().orc_build_events(C.byref(cfg), int(partonic), float(sHat_partonic), key, n, ptr(index), ptr(code), ptr(sH),
+                           ptr(tH), ptr(x1), ptr(x2), ptr(ints), ptr(mom), ptr(scale), ptr(npart))
+    out = {f: ints[:, :, j] for j, f in enumerate(EVENT_INT_FIELDS)}
+    out.update(p=mom[:, :, :4], m=mom[:, :, 4], scale=scale, n_part=npart)
+    return out
 
 
 def philox_fill(key: int, counter0: int, n: int, stream: int = 0) -> np.ndarray:

@@ -18,7 +18,7 @@ def declared_symbols():
 def test_header_declares_the_abi():
     syms = declared_symbols()
     for must in ["mi_ctx_create", "mi_integrate", "mi_sigma_hat", "mi_vegas_adapt", "mi_unweight_compact",
-                 "mi_adjoint", "mi_flow_train", "mi_rng_philox_fill"]:
+                 "mi_adjoint", "mi_flow_train", "mi_rng_philox_fill", "mi_build_events", "mi_write_lhe"]:
         assert must in syms
 
 
@@ -35,9 +35,10 @@ def test_struct_layouts_match_header():
     import subprocess
     import tempfile
     from cupythia_amd import _abi as A
-    src = '#include <stdio.h>\n#include "mcint.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n",' \
+    src = '#include <stdio.h>\n#include "mcint.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",' \
           'sizeof(mi_physics),sizeof(mi_run),sizeof(mi_sigma_est),sizeof(mi_vegas_info),sizeof(mi_events),' \
-          'sizeof(mi_sigma_grad),sizeof(mi_flow_info),offsetof(mi_run,n_points));return 0;}'
+          'sizeof(mi_sigma_grad),sizeof(mi_flow_info),offsetof(mi_run,n_points),sizeof(mi_particles),' \
+          'offsetof(mi_particles,max_part));return 0;}'
     src = src.replace('#include <stdio.h>', '#include <stdio.h>\n#include <stddef.h>')
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "t.c"), "w").write(src)
@@ -45,7 +46,8 @@ def test_struct_layouts_match_header():
                        check=True)
         out = subprocess.run([os.path.join(d, "t")], check=True, capture_output=True, text=True).stdout.split()
     got = [C.sizeof(A.MiPhysics), C.sizeof(A.MiRun), C.sizeof(A.MiSigmaEst), C.sizeof(A.MiVegasInfo),
-           C.sizeof(A.MiEvents), C.sizeof(A.MiSigmaGrad), C.sizeof(A.MiFlowInfo), A.MiRun.n_points.offset]
+           C.sizeof(A.MiEvents), C.sizeof(A.MiSigmaGrad), C.sizeof(A.MiFlowInfo), A.MiRun.n_points.offset,
+           C.sizeof(A.MiParticles), A.MiParticles.max_part.offset]
     assert [int(x) for x in out] == got
 
 
@@ -53,7 +55,7 @@ def test_host_only_entry_points():
     import cupythia_amd as mi
     from cupythia_amd import _abi
     lib = _abi.load()
-    assert lib.mi_version() == _abi.MI_ABI_VERSION == 3
+    assert lib.mi_version() == _abi.MI_ABI_VERSION == 4
     p = mi.physics_default(1)
     assert p.pdf_source == mi.MI_PDF_TOY and p.fact_fac == 1.0 and tuple(p.mass_Q) == (1.5, 4.8)
     assert mi.PROC_CODES == (111, 112, 113, 114, 115, 116, 121, 122, 123, 124) and mi.MI_MAX_PROC == 10
