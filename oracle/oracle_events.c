@@ -25,9 +25,9 @@
  *   hard scale           renormFac^2 (pT^2 + m^2) = Q2Ren of SigmaProcess::store2Kin with
  *                        renormScale2 = 2 (SigmaProcess.cc:846-925); the record holds its root.
  * Random numbers (this library's definition): Philox4x32-10 with counter (index lo, index hi,
- * stream, 0); stream 3 word 0 the pair, word 1 the colour topology, word 2 the g g -> g g
- * orientation, word 3 the new flavour of 112 / 116; stream 4 word 0 the azimuth; uniforms
- * (w >> 8) 2^-24.
+ * stream, 0); stream 3 words (0, 1) the pair and (2, 3) the colour topology, stream 4 words (0, 1)
+ * the azimuth, as 53-bit uniforms orc_u53; stream 4 word 2 the g g -> g g orientation and word 3
+ * the new flavour of 112 / 116 as 24-bit uniforms (w >> 8) 2^-24.
  * Pinned by tests/test_oracle_events.py: colour-topology and flavour frequencies at fixed
  * kinematics against Pythia's probability ratios, colour-line validity, four-momentum conservation,
  * on-shell masses and the invariants (p1 + p2)^2 = x1 x2 S, (p1 - p3)^2 = tHat. */
@@ -130,7 +130,7 @@ void orc_build_event(const orc_hadcfg *c, int partonic, double sHatPartonic, uin
             ps[i] = c->kfactor * ORC_CONVERT2MB * sig * beam_xf(c, ia[i], X1, Q2F) * beam_xf(c, ib[i], X2, Q2F);
             sum += ps[i];
         }
-        double r = sum * uni24(r3[0]);
+        double r = sum * orc_u53(r3[0], r3[1]);
         int pick = np - 1;
         for (int i = 0; i < np; ++i) {
             r -= ps[i];
@@ -141,7 +141,7 @@ void orc_build_event(const orc_hadcfg *c, int partonic, double sHatPartonic, uin
     }
 
     /* ---- outgoing flavours and colour flow (setIdColAcol) */
-    const double uc = uni24(r3[1]), us = uni24(r3[2]), un = uni24(r3[3]);
+    const double uc = orc_u53(r3[2], r3[3]), us = uni24(r4[2]), un = uni24(r4[3]);
     const double sH2 = s * s, tH2 = t * t, uH2 = u * u;
     int id3 = 21, id4 = 21;
     if (code == 111) {
@@ -213,7 +213,7 @@ void orc_build_event(const orc_hadcfg *c, int partonic, double sHatPartonic, uin
     const long double betaQ = sqrtl(fmaxl(0.0L, 1.0L - 4.0L * m2 / (long double)s));
     long double z = (1.0L - 2.0L * ((long double)m2 - t) / (long double)s) / (heavy ? betaQ : 1.0L);
     z = fminl(1.0L, fmaxl(-1.0L, z));
-    const long double theta = acosl(z), phi = 6.283185307179586476925286766559L * uni24(r4[0]);   /* 2 pi u */
+    const long double theta = acosl(z), phi = 6.283185307179586476925286766559L * orc_u53(r4[0], r4[1]);   /* 2 pi u */
     const long double betaZ = ((long double)X1 - X2) / ((long double)X1 + X2);
     long double p[4][4] = {{0.0L, 0.0L, 0.5L * eCM * X1, 0.5L * eCM * X1},
                            {0.0L, 0.0L, -0.5L * eCM * X2, 0.5L * eCM * X2},
