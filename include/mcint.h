@@ -260,8 +260,9 @@ typedef struct mi_particles {
  *   momenta: incoming partons along +-z with E = x sqrt(S)/2, the outgoing pair back to back in
  *     the partonic frame at the record's tHat and a uniform azimuth, boosted to the lab
  *     (PhaseSpace.cc:2236-2251); partonic mode: sqrt(sHat)/2 per beam, x = 1.
- * Random numbers: Philox stream 3 (words 0..3) and stream 4 (word 0) of ev->index[i] under `key`,
- * so the record is a function of (key, event index).  phys: the run's physics (PDFs for the
+ * Random numbers: Philox streams 3 and 4 of ev->index[i] under `key` (stream 3: the pair and the
+ * colour topology as 53-bit uniforms; stream 4: the azimuth, the g g -> g g orientation, the new
+ * flavour of 112 / 116), so the record is a function of (key, event index).  phys: the run's physics (PDFs for the
  * flavour draw, masses, renorm_fac).  Replaces cuPythia/pipeline/build_events.cu:22-41 and
  * cuPythia/12_phase_space/phase_space.cu:37-53. */
 int mi_build_events(mi_ctx *ctx, const mi_physics *phys, const mi_events *ev, uint64_t n, uint64_t key,
