@@ -216,20 +216,45 @@ namespace mi {
 void fill_had_consts_f32(const mi_physics &p, HadConst<float> &k) { fill_consts<float>(p, k); }
 void fill_had_consts_f64(const mi_physics &p, HadConst<double> &k) { fill_consts<double>(p, k); }
 
-Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks)
+Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks, uint32_t nt)
 {
-    uint64_t blocks = (n + (uint64_t)kBlock * min_ppt - 1) / ((uint64_t)kBlock * min_ppt);
+    uint64_t blocks = (n + (uint64_t)nt * min_ppt - 1) / ((uint64_t)nt * min_ppt);
     if (blocks > max_blocks) blocks = max_blocks;
     if (blocks == 0) blocks = 1;
-    uint64_t ppt = (n + blocks * kBlock - 1) / (blocks * kBlock);
+    uint64_t ppt = (n + blocks * nt - 1) / (blocks * nt);
     if (ppt == 0) ppt = 1;
     return Geom{(uint32_t)blocks, (uint32_t)ppt};
 }
 
+// The LDS window of an FP32 grid-PDF integration (no VEGAS histogram): fills g's window and the
+// launch's dynamic LDS size; false when the run has none or the window does not fit one block's
+// LDS next to the VEGAS table and the reduction's static LDS.
+bool had_window(const mi_ctx *ctx, const mi_run *run, bool vegas, bool hist, GridPdf<float> *g, size_t *smem)
+{
+    if (hist || run->precision != MI_PREC_FP32 || run->phys.mode != MI_MODE_HADRONIC_3D ||
+        run->phys.pdf_source != MI_PDF_GRID || ctx->lds_max <= 0)
+        return false;
+    const size_t tab = vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 7) & ~(size_t)7 : 0;
+    const size_t off = (tab + 15) & ~(size_t)15;
+    const size_t stat = (size_t)(kWinBlock / 64) * kRec * sizeof(double) + 1024;   // block_record + slack
+    if ((size_t)ctx->lds_max <= stat) return false;
+    size_t bytes = 0;
+    if (!pdf_window(ctx, run->phys, off, (size_t)ctx->lds_max - stat, g, &bytes)) return false;
+    *smem = off + bytes;
+    return true;
+}
+
 // One wave of equal-work blocks over all CUs: every block takes ceil(n / (blocks * 256))
-// points per thread, so the grid drains together and the block records stay few.
+// points per thread, so the grid drains together and the block records stay few.  With the LDS
+// window: one 1024-thread block per CU.
 Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas, bool hist)
 {
+    {
+        GridPdf<float> g;
+        size_t sm = 0;
+        if (had_window(ctx, run, vegas, hist, &g, &sm))
+            return geom_for(n, 16, (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256), kWinBlock);
+    }
     const int nb = ctx->grid.nbins;
     const int occ = run->phys.mode == MI_MODE_HADRONIC_3D
                         ? occ_integrate_had(run->precision, vegas, hist, nb, run->phys.pdf_source == MI_PDF_GRID,
@@ -254,6 +279,7 @@ int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, boo
         fill_had_consts_f32(p, L.kf); fill_had_consts_f64(p, L.kd);
         rc = pdf_views(ctx, p, &L.gf, &L.gd);
         if (rc) return rc;
+        L.win = had_window(ctx, run, vegas, hist_parts != nullptr, &L.gf, &L.win_smem);
         L.grid = &ctx->grid; L.partials = ctx->partials; L.hist = hist_parts;
         HIPCHK(ctx, launch_integrate_had(ctx->stream, L));
     } else {
@@ -287,7 +313,8 @@ int mi_ctx_create(int device, mi_ctx **out)
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
         hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess ||
-        hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
         delete ctx;
         return MI_ERR_HIP;
     }

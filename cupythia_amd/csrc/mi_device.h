@@ -266,6 +266,14 @@ struct GridPdf {
     const T *qk = nullptr, *lnq = nullptr;       // nq: nodes with subgrid borders averaged; logs as read
     const T *val = nullptr;                      // nq x nx x 12
     const T *ixinv = nullptr, *iqinv = nullptr;  // FP32 only
+    // FP32 only: node guesses on a uniform ln x (ln q) binning, xlut[j] = the last node at or
+    // below bin j's left edge; the search then steps with the bisection's own comparisons
+    const unsigned short *xlut = nullptr, *qlut = nullptr;
+    float xl0 = 0, xih = 0, ql0 = 0, qih = 0;
+    int nbx = 0, nbq = 0;
+    // FP32 only: a window of val (rows wq0..wq1, columns wx0..nx-1, wstride = nx - wx0 nodes per
+    // row) copied to the kernel's dynamic LDS at byte offset woff (-1: no window)
+    int woff = -1, wq0 = 0, wq1 = -1, wx0 = 0, wstride = 0;
     T qDiv[kGridMaxSub] = {};                    // last q of each subgrid, as read
     T xMin = 0, xMax = 0, qMin = 0, qMax = 0;
     T factFac2 = 1;                              // Q2_F = factFac2 * pT2
@@ -353,6 +361,9 @@ __host__ __device__ inline void grid_xf(const GridPdf<T> &g, T x, T lnx, int m3q
     }
 }
 
+// The dynamic LDS of the running kernel (the integration kernel keeps its PDF-grid window there).
+extern __shared__ __align__(16) unsigned char mi_lds[];
+
 // FP32 (device layout): the same arithmetic with each node's 12 values and each stencil start's
 // 12 reciprocals fetched as three 16-byte words (the FP32 tables are 16-byte aligned).
 __host__ __device__ inline void lagrange4_v(float t, const float *n, const float *inv, float *w)
@@ -378,21 +389,31 @@ __host__ __device__ inline void node_vals(const float *p, float *v)
     v[6] = b.z; v[7] = b.w; v[8] = c.x; v[9] = c.y; v[10] = c.z;
 }
 
-__host__ __device__ inline void grid_qw_f32(const GridPdf<float> &g, float q, float lnq, int &m3q, int &n3q, float *wq)
+// Bin of a log value in a node-guess table (clamped to the table).
+__host__ __device__ inline int lut_bin(float l, float l0, float ih, int nb)
+{
+    const float t = (l - l0) * ih;
+    return t <= 0.0f ? 0 : (t >= (float)(nb - 1) ? nb - 1 : (int)t);
+}
+
+__device__ inline void grid_qw_f32(const GridPdf<float> &g, float q, float lnq, int &m3q, int &n3q, float *wq)
 {
     wq[0] = wq[1] = wq[2] = wq[3] = 1.0f;
     if (q <= g.qMin) { n3q = 1; m3q = 0; return; }
     if (q >= g.qMax) { n3q = 1; m3q = g.nq - 1; return; }
-    int iqDiv = 0;
-    for (int s = 1; s < g.nqSub; ++s)
-        if (q > g.qDiv[s - 1]) iqDiv = s;
-    const int minS = iqDiv == 0 ? 0 : g.nqSum[iqDiv - 1], maxS = g.nqSum[iqDiv] - 1;
-    int minq = minS, maxq = maxS;
-    while (maxq - minq > 1) {
-        const int mid = (minq + maxq) >> 1;
-        if (q < g.qk[mid]) maxq = mid;
-        else               minq = mid;
-    }
+    // the subgrid holding q (the last one whose predecessor ends below q), with constant
+    // indices only, so the view's arrays stay in scalar registers
+    int minS = 0, maxS = g.nqSum[0] - 1;
+#pragma unroll
+    for (int s = 1; s < kGridMaxSub; ++s)
+        if (s < g.nqSub && q > g.qDiv[s - 1]) { minS = g.nqSum[s - 1]; maxS = g.nqSum[s] - 1; }
+    // the interval the bisection over [minS, maxS] finds: the last node in [minS, maxS - 1] at or
+    // below q (minS if none), from the table's guess in a few steps
+    int minq = g.qlut[lut_bin(lnq, g.ql0, g.qih, g.nbq)];
+    minq = minq < minS ? minS : (minq > maxS - 1 ? maxS - 1 : minq);
+    while (minq > minS && q < g.qk[minq]) --minq;
+    while (minq < maxS - 1 && q >= g.qk[minq + 1]) ++minq;
+    const int maxq = minq + 1;
     if (maxS - minS < 3) {
         n3q = 2;
         m3q = minq;
@@ -405,43 +426,60 @@ __host__ __device__ inline void grid_qw_f32(const GridPdf<float> &g, float q, fl
     }
 }
 
+// Folds n3q rows of 4 nodes (row stride rs floats from p) with the combined weights wq * wx.
+template <typename P>
+__device__ __forceinline__ void fold_stencil(P p, size_t rs, int n3q, const float *wq, const float *wx, float *out)
+{
+    for (int i3q = 0; i3q < n3q; ++i3q) {
+#pragma unroll
+        for (int ix = 0; ix < 4; ++ix) {
+            const float w = wq[i3q] * wx[ix];
+            float v[11];
+            node_vals(p + i3q * rs + ix * kGridNF, v);
+#pragma unroll
+            for (int f = 0; f < 11; ++f) out[f] = __builtin_fmaf(w, v[f], out[f]);
+        }
+    }
+}
+
 // The 4 x 4 stencil (n3q rows of 4 x nodes) folded node by node with the combined weight
 // wq * wx into FP32 fmas: 16 products and 11 fmas per node, one node's values live at a time.
-__host__ __device__ inline void grid_xf_f32(const GridPdf<float> &g, float x, float lnx, int m3q, int n3q,
-                                            const float *wq, float *out)
+// Node values come from the LDS window when the stencil lies inside it, else from global memory
+// (the same values, the same order of operations).
+__device__ inline void grid_xf_f32(const GridPdf<float> &g, float x, float lnx, int m3q, int n3q,
+                                   const float *wq, float *out)
 {
 #pragma unroll
     for (int f = 0; f < 11; ++f) out[f] = 0.0f;
     if (x <= g.xMin) {
         for (int i3q = 0; i3q < n3q; ++i3q) {
             float v[11];
-            node_vals(g.val + (size_t)(m3q + i3q) * g.nx * kGridNF, v);
+            const bool lds = g.woff >= 0 && g.wx0 == 0 && m3q + i3q >= g.wq0 && m3q + i3q <= g.wq1;
+            if (lds)
+                node_vals(reinterpret_cast<const float *>(mi_lds + g.woff) +
+                              (size_t)(m3q + i3q - g.wq0) * g.wstride * kGridNF, v);
+            else
+                node_vals(g.val + (size_t)(m3q + i3q) * g.nx * kGridNF, v);
 #pragma unroll
             for (int f = 0; f < 11; ++f) out[f] = __builtin_fmaf(wq[i3q], v[f], out[f]);
         }
         return;
     }
     if (x >= g.xMax) return;
-    int minx = 0, maxx = g.nx - 1;
-    while (maxx - minx > 1) {
-        const int mid = (minx + maxx) >> 1;
-        if (x < g.xk[mid]) maxx = mid;
-        else               minx = mid;
-    }
+    // xk[minx] <= x < xk[minx + 1] (xMin < x < xMax), as the bisection finds it
+    int minx = g.xlut[lut_bin(lnx, g.xl0, g.xih, g.nbx)];
+    while (minx > 0 && x < g.xk[minx]) --minx;
+    while (minx < g.nx - 2 && x >= g.xk[minx + 1]) ++minx;
+    const int maxx = minx + 1;
     const int m3x = (minx == 0) ? 0 : ((maxx == g.nx - 1) ? g.nx - 4 : minx - 1);
     float wx[4];
     lagrange4_v(lnx, g.lnx + m3x, g.ixinv + kGridNF * m3x, wx);
-    for (int i3q = 0; i3q < n3q; ++i3q) {
-        const float *p = g.val + ((size_t)(m3q + i3q) * g.nx + m3x) * kGridNF;
-#pragma unroll
-        for (int ix = 0; ix < 4; ++ix) {
-            const float w = wq[i3q] * wx[ix];
-            float v[11];
-            node_vals(p + ix * kGridNF, v);
-#pragma unroll
-            for (int f = 0; f < 11; ++f) out[f] = __builtin_fmaf(w, v[f], out[f]);
-        }
-    }
+    if (g.woff >= 0 && m3x >= g.wx0 && m3q >= g.wq0 && m3q + n3q - 1 <= g.wq1)
+        fold_stencil(reinterpret_cast<const float *>(mi_lds + g.woff) +
+                         ((size_t)(m3q - g.wq0) * g.wstride + (m3x - g.wx0)) * kGridNF,
+                     (size_t)g.wstride * kGridNF, n3q, wq, wx, out);
+    else
+        fold_stencil(g.val + ((size_t)m3q * g.nx + m3x) * kGridNF, (size_t)g.nx * kGridNF, n3q, wq, wx, out);
 }
 
 // Parton luminosities of the five flux types from the grid at the factorization scale

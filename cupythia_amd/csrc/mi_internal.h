@@ -49,6 +49,7 @@ struct PdfState;               // mi_pdf.hip
 struct mi_ctx {
     int device = 0;
     int num_cu = 0;
+    int lds_max = 0;             // LDS bytes a block may use (hipDeviceAttributeMaxSharedMemoryPerBlock)
     hipStream_t stream = nullptr;
     std::string err;
     // scratch, grown on demand (never shrunk)
@@ -75,6 +76,12 @@ void fill_had_consts_f64(const mi_physics &p, HadConst<double> &k);
 // Views of the loaded grid for p (active only when p.pdf_source == MI_PDF_GRID); MI_ERR_STATE
 // when a grid is asked for and none is loaded.
 int pdf_views(mi_ctx *ctx, const mi_physics &p, GridPdf<float> *g32, GridPdf<double> *g64);
+// The rows and columns of the FP32 grid that an integration of p reaches (x >= 4 pTmin^2 / S,
+// q from fact_fac pTmin to fact_fac sqrt(S) / 2) as an LDS window of the integration kernel at
+// byte offset off (sets g's woff, wq0, wq1, wx0, wstride); false, g unchanged, when p uses no grid
+// or the window would end beyond max_bytes.  *bytes: the window's size.
+bool pdf_window(const mi_ctx *ctx, const mi_physics &p, size_t off, size_t max_bytes, GridPdf<float> *g,
+                size_t *bytes);
 // Host (FP64) view for host-side evaluation; false when p does not use a grid.
 bool pdf_host_view(const mi_ctx *ctx, const mi_physics &p, GridPdf<double> *g);
 void pdf_release(mi_ctx *ctx);
@@ -85,7 +92,7 @@ constexpr int kPdfSlots = MI_PDF_SLOTS;
 // ------------------------------------------------------------------ launch geometry
 struct Geom { uint32_t blocks; uint32_t ppt; };
 // Points per thread >= min_ppt, blocks <= max_blocks.
-Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks);
+Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks, uint32_t nt = kBlock);
 // Resident blocks per CU of the integration kernels (hipOccupancyMaxActiveBlocksPerMultiprocessor).
 int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid, bool heavy);
 int occ_integrate_c1(bool vegas, bool hist, int nbins);
@@ -112,7 +119,10 @@ struct HadLaunch {
     double *hist;
     GridPdf<float> gf;     // PDF grid (active = 0: toy PDFs)
     GridPdf<double> gd;
+    bool win = false;      // FP32 grid with its LDS window (gf.woff): 1024-thread blocks
+    size_t win_smem = 0;   // dynamic LDS bytes of the window launch (tables + window)
 };
+constexpr int kWinBlock = 512;                 // threads per block of the LDS-window launch
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L);
 
 struct C1Launch {

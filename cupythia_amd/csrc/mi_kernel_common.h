@@ -24,9 +24,10 @@ __device__ __forceinline__ double wave_max(double x)
 
 // Combine kRec per-thread values into one block record (sum except the max slots).
 // If cnt is given, it receives slot kRAcc (the unweighting accept count) as an integer.
+template <int NT = kBlock>
 __device__ __forceinline__ void block_record(const double *v, double *dst, uint64_t *cnt = nullptr)
 {
-    __shared__ double red[kBlock / 64][kRec];
+    __shared__ double red[NT / 64][kRec];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < kRec; ++i) {
@@ -39,9 +40,40 @@ __device__ __forceinline__ void block_record(const double *v, double *dst, uint6
         const int i = threadIdx.x;
         const bool isMax = rec_is_max(i);
         double r = red[0][i];
-        for (int w = 1; w < kBlock / 64; ++w) r = isMax ? fmax(r, red[w][i]) : r + red[w][i];
+        for (int w = 1; w < NT / 64; ++w) r = isMax ? fmax(r, red[w][i]) : r + red[w][i];
         dst[i] = r;
         if (cnt && i == kRAcc) *cnt = (uint64_t)r;
+    }
+}
+
+// block_record for a record whose only non-zero slots are the NP + 1 sums, sums of squares and
+// maxima and the non-zero count (the integration kernels): the same wave reductions and fold
+// over waves per slot, zeros elsewhere, from small per-thread arrays instead of a full record.
+template <int NT, int NP>
+__device__ __forceinline__ void block_record_sums(const double *sw, const double *sw2, const double *mx,
+                                                  double nz, double *dst)
+{
+    __shared__ double red[NT / 64][kRec];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p <= NP; ++p) {
+        const double a = wave_sum(sw[p]), b = wave_sum(sw2[p]), c = wave_max(mx[p]);
+        if (lane == 0) { red[wave][kRS + p] = a; red[wave][kRQ + p] = b; red[wave][kRM + p] = c; }
+    }
+    const double z = wave_sum(nz);
+    if (lane == 0) red[wave][kRNz] = z;
+    __syncthreads();
+    if (threadIdx.x < kRec) {
+        const int i = threadIdx.x;
+        const int slot = i < kRQ ? i - kRS : (i < kRM ? i - kRQ : i - kRM);
+        const bool used = (i < kRNz && slot <= NP) || i == kRNz;
+        double r = 0.0;
+        if (used) {
+            const bool isMax = rec_is_max(i);
+            r = red[0][i];
+            for (int w = 1; w < NT / 64; ++w) r = isMax ? fmax(r, red[w][i]) : r + red[w][i];
+        }
+        dst[i] = r;
     }
 }
 

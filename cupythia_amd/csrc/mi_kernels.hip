@@ -122,42 +122,54 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
 // GRID: parton densities from the PDF grid gpdf instead of the toy PDFs.  HEAVY: mask may
 // select the heavy-quark codes 121..124 (their families are evaluated); otherwise only the
-// massless codes are computed and accumulated.
-template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY>
-__global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
-                                                          uint64_t ctr0, uint64_t n, uint32_t ppt,
-                                                          const T *__restrict__ gtab, int nbins,
-                                                          double *__restrict__ partials,
-                                                          double *__restrict__ hist, GridPdf<T> gpdf)
+// massless codes are computed and accumulated.  NT threads per block.  WIN (FP32 grid, no
+// histogram): the block first copies the window of the grid's node values that gpdf describes
+// (rows wq0..wq1, columns wx0..nx-1) into its dynamic LDS at gpdf.woff, so the interpolation
+// stencils read LDS instead of gathering from L2.
+template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, int NT = kBlock, bool WIN = false>
+__global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
+                                                      uint64_t ctr0, uint64_t n, uint32_t ppt,
+                                                      const T *__restrict__ gtab, int nbins,
+                                                      double *__restrict__ partials,
+                                                      double *__restrict__ hist, GridPdf<T> gpdf)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+    static_assert(!WIN || (GRID && !HIST && sizeof(T) == 4), "the LDS window is for FP32 grid integration");
+    unsigned char *smem = mi_lds;
     T *stab = reinterpret_cast<T *>(smem);                                   // 3 dims x nbins x {lo, wd, 1-lo}
     const size_t tabBytes = (sizeof(T) * 9 * (VEGAS ? nbins : 0) + 7) & ~(size_t)7;
-    double *shist = reinterpret_cast<double *>(smem + tabBytes);             // 4 waves x 3 dims x nbins
+    double *shist = reinterpret_cast<double *>(smem + tabBytes);             // NT/64 waves x 3 dims x nbins
     const int wave = threadIdx.x >> 6;
     if (VEGAS) {
-        for (int i = threadIdx.x; i < 9 * nbins; i += blockDim.x) stab[i] = gtab[i];
+        for (int i = threadIdx.x; i < 9 * nbins; i += NT) stab[i] = gtab[i];
         if (HIST)
-            for (int i = threadIdx.x; i < 4 * 3 * nbins; i += blockDim.x) shist[i] = 0.0;
-        __syncthreads();
+            for (int i = threadIdx.x; i < (NT / 64) * 3 * nbins; i += NT) shist[i] = 0.0;
     }
+    if (WIN) {
+        float4 *dst = reinterpret_cast<float4 *>(smem + gpdf.woff);
+        const int rowq = gpdf.wstride * (kGridNF / 4);                       // 16-byte words per row
+        const int total = (gpdf.wq1 - gpdf.wq0 + 1) * rowq;
+        for (int t = threadIdx.x; t < total; t += NT) {
+            const int r = t / rowq, c = t - r * rowq;
+            dst[t] = reinterpret_cast<const float4 *>(
+                gpdf.val + ((size_t)(gpdf.wq0 + r) * gpdf.nx + gpdf.wx0) * kGridNF)[c];
+        }
+    }
+    if (VEGAS || WIN) __syncthreads();
     const T nbf = (T)nbins;
-    double v[kRec];
-#pragma unroll
-    for (int i = 0; i < kRec; ++i) v[i] = 0.0;
     // Running maxima as the weights' own type: the weights are >= 0 (never NaN), so for FP32 the
     // maximum is an integer max of the bit patterns.  Converted to FP64 once at the end, which is
     // exact and order-preserving.  The nonzero count is an integer.
     using MaxT = typename std::conditional<sizeof(T) == 4, uint32_t, double>::type;
     constexpr int NP = HEAVY ? kMaxProc : kLightProc;
     MaxT mx[NP + 1];
+    double sw[NP + 1], sw2[NP + 1];                                          // sum w, sum w^2 per slot
 #pragma unroll
-    for (int i = 0; i <= NP; ++i) mx[i] = 0;
+    for (int i = 0; i <= NP; ++i) { mx[i] = 0; sw[i] = 0.0; sw2[i] = 0.0; }
     uint32_t nnz = 0;
 
-    const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
+    const uint64_t base = (uint64_t)blockIdx.x * NT * ppt;
     for (uint32_t j = 0; j < ppt; ++j) {
-        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        const uint64_t i = base + (uint64_t)j * NT + threadIdx.x;
         if (i >= n) break;
         T ws[kMaxProc];
         int bins[3];
@@ -165,15 +177,15 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
                                                      (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
                                                      GRID ? &gpdf : nullptr);
         const double dt = (double)tot;
-        v[kRS] += dt;
-        v[kRQ] = fma(dt, dt, v[kRQ]);
+        sw[0] += dt;
+        sw2[0] = fma(dt, dt, sw2[0]);
         mx[0] = run_max(mx[0], tot);
         nnz += (tot > (T)0) ? 1u : 0u;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             const double d = (double)ws[p];
-            v[kRS + 1 + p] += d;
-            v[kRQ + 1 + p] = fma(d, d, v[kRQ + 1 + p]);
+            sw[1 + p] += d;
+            sw2[1 + p] = fma(d, d, sw2[1 + p]);
             mx[1 + p] = run_max(mx[1 + p], ws[p]);
         }
         if (HIST) {
@@ -184,15 +196,15 @@ __global__ __launch_bounds__(kBlock) void k_integrate_had(HadConst<T> k, int mas
             atomicAdd(&h[2 * nbins + bins[2]], w2);
         }
     }
+    double mxd[NP + 1];
 #pragma unroll
-    for (int i = 0; i <= NP; ++i) v[kRM + i] = max_value<T>(mx[i]);
-    v[kRNz] = (double)nnz;
-    block_record(v, partials + (size_t)blockIdx.x * kRec);
+    for (int i = 0; i <= NP; ++i) mxd[i] = max_value<T>(mx[i]);
+    block_record_sums<NT, NP>(sw, sw2, mxd, (double)nnz, partials + (size_t)blockIdx.x * kRec);
     if (HIST) {
         __syncthreads();
-        for (int c = threadIdx.x; c < 3 * nbins; c += blockDim.x) {
+        for (int c = threadIdx.x; c < 3 * nbins; c += NT) {
             double s = shist[c];
-            for (int w = 1; w < kBlock / 64; ++w) s += shist[w * 3 * nbins + c];
+            for (int w = 1; w < NT / 64; ++w) s += shist[w * 3 * nbins + c];
             hist[(size_t)blockIdx.x * 3 * nbins + c] = s;
         }
     }
@@ -224,10 +236,32 @@ static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadCons
     return hipGetLastError();
 }
 
+// FP32 grid integration with the LDS window: one 1024-thread block per CU.
+template <bool VEGAS, bool HEAVY>
+static hipError_t launch_had_win(hipStream_t s, const HadLaunch &L, const float *tab)
+{
+    auto kern = k_integrate_had<float, VEGAS, false, true, HEAVY, kWinBlock, true>;
+    // allows dynamic LDS above the default limit where the runtime asks for it; a runtime that
+    // has no such limit may refuse the call, which the launch below then does not depend on
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)L.win_smem);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(kern, dim3(L.g.blocks), dim3(kWinBlock), L.win_smem, s, L.kf, L.mask, L.key, L.ctr0, L.n,
+                       L.g.ppt, tab, VEGAS ? L.grid->nbins : 0, L.partials, L.hist, L.gf);
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                const GridPdf<T> &gp)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (L.win && gp.active && !L.hist) {
+            const bool hv = has_heavy(L.mask);
+            if (L.vegas) return hv ? launch_had_win<true, true>(s, L, tab) : launch_had_win<true, false>(s, L, tab);
+            return hv ? launch_had_win<false, true>(s, L, tab) : launch_had_win<false, false>(s, L, tab);
+        }
+    }
     if (has_heavy(L.mask))
         return gp.active ? launch_had_tg<T, true, true>(s, L, k, tab, gp) : launch_had_tg<T, false, true>(s, L, k, tab, gp);
     return gp.active ? launch_had_tg<T, true, false>(s, L, k, tab, gp) : launch_had_tg<T, false, false>(s, L, k, tab, gp);

@@ -11,6 +11,7 @@
 // Interpolation: GridPdf / grid_qw / grid_xf in mi_device.h (LHAGrid1::xfxevolve, :648-745).
 #include "mi_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -27,6 +28,7 @@ struct PdfState {
     double *d64 = nullptr;    // xk | lnx | qk | lnq | val
     float *d32 = nullptr;     // val | ixinv | iqinv | xk | lnx | qk | lnq (val and the reciprocal
                               // tables 16-byte aligned: 12 floats per node / stencil start)
+    unsigned short *dlut = nullptr;   // FP32 node guesses: x table | q table
     GridPdf<double> g64;      // views into d64
     GridPdf<float> g32;       // views into d32
     GridPdf<double> host64;   // views into the host copies
@@ -152,6 +154,23 @@ void lagrange_inv(const std::vector<double> &t, std::vector<float> &inv)
     }
 }
 
+// Node guesses for a uniform binning of [lnt[0], lnt[n-1]] into nb bins: lut[j] = the last node
+// (at most n - 2) whose log is at or below bin j's left edge.
+constexpr int kLutX = 4096, kLutQ = 1024;
+void node_lut(const std::vector<double> &lnt, int nb, std::vector<unsigned short> &lut, float &l0, float &ih)
+{
+    const int n = (int)lnt.size();
+    const double a = lnt.front(), h = (lnt.back() - a) / nb;
+    l0 = (float)a;
+    ih = (float)(1.0 / h);
+    int i = 0;
+    for (int j = 0; j < nb; ++j) {
+        const double e = a + j * h;
+        while (i + 1 <= n - 2 && lnt[i + 1] <= e) ++i;
+        lut.push_back((unsigned short)i);
+    }
+}
+
 template <typename T>
 void fill_view(GridPdf<T> &g, const PdfState &st)
 {
@@ -184,6 +203,7 @@ void release_state(PdfState *&st)
     if (!st) return;
     (void)hipFree(st->d64);
     (void)hipFree(st->d32);
+    (void)hipFree(st->dlut);
     delete st;
     st = nullptr;
 }
@@ -215,6 +235,35 @@ int pdf_views(mi_ctx *ctx, const mi_physics &p, GridPdf<float> *g32, GridPdf<dou
     if (g32) { *g32 = ctx->pdf->g32; g32->active = 1; g32->factFac2 = (float)ff2; }
     if (g64) { *g64 = ctx->pdf->g64; g64->active = 1; g64->factFac2 = ff2; }
     return MI_OK;
+}
+
+bool pdf_window(const mi_ctx *ctx, const mi_physics &p, size_t off, size_t max_bytes, GridPdf<float> *g,
+                size_t *bytes)
+{
+    if (p.pdf_source != MI_PDF_GRID || !ctx->pdf || !(p.eCM > 0.0) || !(p.pTHatMin > 0.0)) return false;
+    const PdfState &st = *ctx->pdf;
+    const int nx = st.nx, nq = st.nq;
+    // last node (FP32, as the device compares) at or below v, -1 if none
+    auto last_le = [](const std::vector<double> &t, float v) {
+        int i = -1;
+        while (i + 1 < (int)t.size() && (float)t[i + 1] <= v) ++i;
+        return i;
+    };
+    const float xlo = (float)(0.999 * 4.0 * p.pTHatMin * p.pTHatMin / (p.eCM * p.eCM));
+    int x0 = 0;
+    if (xlo > (float)st.x.front()) x0 = std::max(0, std::min(last_le(st.x, xlo), nx - 2) - 1);
+    const float qlo = (float)(0.999 * p.fact_fac * p.pTHatMin), qhi = (float)(1.001 * p.fact_fac * 0.5 * p.eCM);
+    const int r0 = qlo <= (float)st.qk.front() ? 0 : std::max(0, last_le(st.qk, qlo) - 3);
+    const int r1 = qhi >= (float)st.qk.back() ? nq - 1 : std::min(nq - 1, last_le(st.qk, qhi) + 4);
+    const size_t b = (size_t)(r1 - r0 + 1) * (size_t)(nx - x0) * kGridNF * sizeof(float);
+    if (r1 < r0 || off % 16 != 0 || off + b > max_bytes) return false;
+    g->woff = (int)off;
+    g->wq0 = r0;
+    g->wq1 = r1;
+    g->wx0 = x0;
+    g->wstride = nx - x0;
+    *bytes = b;
+    return true;
 }
 
 bool pdf_host_view(const mi_ctx *ctx, const mi_physics &p, GridPdf<double> *g)
@@ -271,12 +320,20 @@ int mi_pdf_load_slot(mi_ctx *ctx, int slot, const char *path)
     for (double v : st->lnx) h32.push_back((float)v);
     for (double v : st->qk) h32.push_back((float)v);
     for (double v : st->lnq) h32.push_back((float)v);
+    if (nx > 65535 || nq > 65535) { delete st; return pfail(ctx, MI_ERR_RANGE, "PDF grid above 65535 nodes"); }
+    std::vector<unsigned short> lut;
+    float xl0, xih, ql0, qih;
+    node_lut(st->lnx, kLutX, lut, xl0, xih);
+    node_lut(st->lnq, kLutQ, lut, ql0, qih);
     if (e == hipSuccess) e = hipMalloc(&st->d64, sizeof(double) * h64.size());
     if (e == hipSuccess) e = hipMalloc(&st->d32, sizeof(float) * h32.size());
+    if (e == hipSuccess) e = hipMalloc(&st->dlut, sizeof(unsigned short) * lut.size());
     if (e == hipSuccess) e = hipMemcpy(st->d64, h64.data(), sizeof(double) * h64.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(st->d32, h32.data(), sizeof(float) * h32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(st->dlut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(st->d64); (void)hipFree(st->d32);
+        (void)hipFree(st->d64); (void)hipFree(st->d32); (void)hipFree(st->dlut);
         delete st;
         return pfail(ctx, MI_ERR_HIP, std::string("PDF grid upload: ") + hipGetErrorString(e));
     }
@@ -291,6 +348,10 @@ int mi_pdf_load_slot(mi_ctx *ctx, int slot, const char *path)
     st->g32.lnx = st->g32.xk + nx;
     st->g32.qk = st->g32.lnx + nx;
     st->g32.lnq = st->g32.qk + nq;
+    st->g32.xlut = st->dlut;
+    st->g32.qlut = st->dlut + kLutX;
+    st->g32.xl0 = xl0; st->g32.xih = xih; st->g32.nbx = kLutX;
+    st->g32.ql0 = ql0; st->g32.qih = qih; st->g32.nbq = kLutQ;
     fill_view(st->host64, *st);
     st->host64.xk = st->x.data(); st->host64.lnx = st->lnx.data(); st->host64.qk = st->qk.data();
     st->host64.lnq = st->lnq.data(); st->host64.val = st->val.data();
