@@ -196,7 +196,11 @@ int mi_rng_philox_fill(mi_ctx *ctx, uint64_t key, uint64_t counter0, uint32_t st
 /* Batched Kfactor * sigmaHatWrap(id1, id2) (mb/GeV^2) for code 111..116 (massless) and the
  * heavy-quark pairs 121..124, FP64.  uH_dev may be NULL for 111..116 (then uH = -(sH + tH) as in
  * Sigma2Process::store2Kin); 121..124 need it and take m3^2 = m4^2 = (sH + tH + uH) / 2.
- * id1/id2 may be NULL (only 114 depends on them; NULL means q q' with different flavours). */
+ * id1/id2 may be NULL (only 114 depends on them; NULL means q q' with different flavours).
+ * With alpS given per point this is also Pythia's multiparton-interaction entry,
+ * Sigma2Process::set2KinMPI(x1, x2, sH, tH, uH, alpS, alpEM, needMasses, m3, m4) then sigmaHatWrap
+ * (pythia8317/include/Pythia8/SigmaProcess.h:504-508, SigmaProcess.cc:931-983): x1, x2 and alpEM
+ * do not enter these codes' sigmaHat, and needMasses with m3 = m4 is the heavy pairs' uH. */
 int mi_sigma_hat(mi_ctx *ctx, int code, uint64_t n, const double *sH_dev, const double *tH_dev,
                  const double *uH_dev, const double *alpS_dev, const int32_t *id1_dev,
                  const int32_t *id2_dev, int nQuarkNew, double kfactor, double *out_dev);
