@@ -251,7 +251,8 @@ static_assert(sizeof(HadConst<float>) == 42 * sizeof(float), "HadConst layout");
 // ------------------------------------------------------------------ PDF grid (LHAGrid1)
 // A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
 // LHAGrid1::xfxevolve does (pythia8317/src/PartonDistributions.cc:648-745, PDF:extrapolate =
-// off): bisection for the neighbouring nodes in x and in q (within the q subgrid holding q),
+// off): bisection for the neighbouring nodes in x and in q (within the q subgrid holding q; the
+// FP32 device path starts from node-guess tables and steps to the same interval),
 // 4-point Lagrange weights in ln x and ln q (linear in ln q on subgrids of 2-3 points), frozen
 // outside the q range and below xMin, 0 at x >= xMax.  Slots: 0 = g, 1..5 = d u s c b,
 // 6..10 = dbar ubar sbar cbar bbar, 11 = photon; values flavour-interleaved
