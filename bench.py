@@ -221,6 +221,11 @@ def workloads(ctx, mi):
     out["pdf_grid"] = {"points": 1 << 26, "kernel_ms": ge.kernel_ms, "points_per_s": (1 << 26) / (ge.kernel_ms / 1e3),
                        "sigma_mb": ge.sigmaGen(), "sigma_err_mb": ge.sigmaErr(),
                        "grid": "100 x-nodes (NNPDF2.3 layout) x 50 Q-nodes, 11 flavours, FP32 Lagrange interpolation"}
+    # unweighting with the grid PDFs: 2^26 candidates of the flat map, exact maximum
+    gu = ctx.unweight_compact(mi.make_run(gphys, 1 << 26, key=50))
+    out["pdf_grid_unweight"] = {"candidates": 1 << 26, "accepted": gu.n, "unweight_ms": gu.estimate.kernel_ms,
+                                "candidates_per_s": (1 << 26) / (gu.estimate.kernel_ms / 1e3)}
+    del gu
     # the ten-process library with the heavy-quark pairs (codes 121..124: own massive map per
     # family at the same uniforms), flat map, toy PDFs
     hphys = mi.physics_default(1, proc_mask=0x3FF)

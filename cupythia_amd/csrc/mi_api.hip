@@ -638,6 +638,18 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
         rc = pdf_views(ctx, p, &L.h.gf, &L.h.gd);
         if (rc) return rc;
         L.h.grid = &ctx->grid; L.h.partials = ctx->partials;
+        // FP32 grid candidates: the eval kernel reads the reachable part of the grid from LDS
+        if (run->precision == MI_PREC_FP32 && L.h.gf.active && ctx->lds_max > 0) {
+            const size_t tab = L.h.vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 15) & ~(size_t)15 : 0;
+            const size_t stat = (size_t)(kWinBlock / 64) * kRec * sizeof(double) + 1024;
+            GridPdf<float> gw = L.h.gf;
+            size_t bytes = 0;
+            if ((size_t)ctx->lds_max > stat && pdf_window(ctx, p, tab, (size_t)ctx->lds_max - stat, &gw, &bytes)) {
+                L.win = true;
+                L.win_smem = tab + bytes;
+                L.wgrid = gw;
+            }
+        }
     }
     // one wave of equal-work blocks of the eval kernel; flags and compact share its layout
     const uint32_t maxb = ncu * (uint32_t)occ_unweight_eval(L);

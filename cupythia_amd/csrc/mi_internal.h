@@ -164,6 +164,9 @@ struct UnwLaunch {
     uint64_t base;        // rank of the chunk's first accepted event
     uint32_t mat_blocks;  // grid of the materialize launch (grid-stride over the accepted events)
     const mi_events *out;
+    bool win = false;     // FP32 grid: the eval kernel keeps the window of wgrid in LDS
+    size_t win_smem = 0;  // its dynamic LDS bytes (tables + window)
+    GridPdf<float> wgrid; // the window (woff, wq0, wq1, wx0, wstride) for the eval kernel only
 };
 hipError_t launch_unweight_eval(hipStream_t s, const UnwLaunch &L);
 int occ_unweight_eval(const UnwLaunch &L);   // resident blocks per CU of the eval kernel

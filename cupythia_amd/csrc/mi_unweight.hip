@@ -151,15 +151,56 @@ __device__ __forceinline__ void stage_tables(const Params<KIND, T> &P, T *stab)
     }
 }
 
-// eval: the integrand of every candidate, the stored total (wout may be null: estimate and
-// maximum only) and one kRec record per block.
-template <int KIND, typename T>
-__global__ __launch_bounds__(kBlock) void k_unw_eval(Params<KIND, T> P, T *__restrict__ wout,
-                                                     double *__restrict__ partials)
+// Per-256-thread-group record of a block of NT threads: group g's four waves fold into
+// partials[vb] (vb = blockIdx.x * NT / 256 + g) when live, in the order block_record uses.
+template <int NT>
+__device__ __forceinline__ void group_record(const double *v, double *partials, uint32_t vb, bool live)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[NT / 64][kRec];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kRec; ++i) {
+        const double r = rec_is_max(i) ? wave_max(v[i]) : wave_sum(v[i]);
+        if (lane == 0) red[wave][i] = r;
+    }
+    __syncthreads();
+    const int t = threadIdx.x & (kBlock - 1), w0 = (threadIdx.x / kBlock) * (kBlock / 64);
+    if (t < kRec && live) {
+        const bool isMax = rec_is_max(t);
+        double r = red[w0][t];
+        for (int w = 1; w < kBlock / 64; ++w) r = isMax ? fmax(r, red[w0 + w][t]) : r + red[w0 + w][t];
+        partials[(size_t)vb * kRec + t] = r;
+    }
+}
+
+// eval: the integrand of every candidate, the stored total (wout may be null: estimate and
+// maximum only) and one kRec record per 256-candidate-wide block of the layout ("virtual block"
+// vb < nvb: candidates [vb*256*ppt, (vb+1)*256*ppt)).  NT = 256: one virtual block per block.
+// WIN (FP32 grid): NT = 512 threads run two virtual blocks and first copy the window of the PDF
+// grid's node values that P.gpdf describes into their dynamic LDS at P.gpdf.woff.
+template <int KIND, typename T, int NT = kBlock, bool WIN = false>
+__global__ __launch_bounds__(NT) void k_unw_eval(Params<KIND, T> P, T *__restrict__ wout,
+                                                 double *__restrict__ partials, uint32_t nvb)
+{
+    static_assert(!WIN || (kind_grid(KIND) && sizeof(T) == 4), "the LDS window is for FP32 grid candidates");
+    unsigned char *smem = mi_lds;
     T *stab = reinterpret_cast<T *>(smem);
-    stage_tables<KIND, T>(P, stab);
+    if (kind_vegas(KIND))
+        for (int i = threadIdx.x; i < 9 * P.nbins; i += NT) stab[i] = P.gtab[i];
+    if (WIN) {
+        float4 *dst = reinterpret_cast<float4 *>(smem + P.gpdf.woff);
+        const int rowq = P.gpdf.wstride * (kGridNF / 4);
+        const int total = (P.gpdf.wq1 - P.gpdf.wq0 + 1) * rowq;
+        for (int t = threadIdx.x; t < total; t += NT) {
+            const int r = t / rowq, c = t - r * rowq;
+            dst[t] = reinterpret_cast<const float4 *>(
+                P.gpdf.val + ((size_t)(P.gpdf.wq0 + r) * P.gpdf.nx + P.gpdf.wx0) * kGridNF)[c];
+        }
+    }
+    if (kind_vegas(KIND) || WIN) __syncthreads();
+    const uint32_t vb = blockIdx.x * (NT / kBlock) + threadIdx.x / kBlock;
+    const bool live = vb < nvb;
+    const int tid = threadIdx.x & (kBlock - 1);
     constexpr int NP = (KIND == kPartonic) ? 1 : (kind_heavy(KIND) ? kMaxProc : kLightProc);
     double v[kRec];
 #pragma unroll
@@ -168,9 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_unw_eval(Params<KIND, T> P, T *__res
 #pragma unroll
     for (int i = 0; i <= NP; ++i) mx[i] = 0;
     uint32_t nnz = 0;
-    const uint64_t base = (uint64_t)blockIdx.x * kBlock * P.ppt;
-    for (uint32_t j = 0; j < P.ppt; ++j) {
-        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    const uint64_t base = (uint64_t)vb * kBlock * P.ppt;
+    for (uint32_t j = 0; live && j < P.ppt; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + tid;
         if (i >= P.n) break;
         T ws[kMaxProc];
         const T tot = eval_cand<KIND, T>(P, stab, P.ctr0 + i, ws);
@@ -191,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_unw_eval(Params<KIND, T> P, T *__res
 #pragma unroll
     for (int i = 0; i <= NP; ++i) v[kRM + i] = max_value<T>(mx[i]);
     v[kRNz] = (double)nnz;
-    block_record(v, partials + (size_t)blockIdx.x * kRec);
+    group_record<NT>(v, partials, vb, live);
 }
 
 // flags: accept decisions from the stored totals over the padded range of nwords x 64 candidates
@@ -372,7 +413,22 @@ hipError_t stage_t(hipStream_t s, Stage st, const UnwLaunch &L, const HadConst<T
     switch (st) {
     case Stage::Eval: {
         double *parts = L.partonic ? L.c.partials : L.h.partials;
-        hipLaunchKernelGGL((k_unw_eval<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, (T *)L.w, parts);
+        if constexpr (kind_grid(KIND) && sizeof(T) == 4) {
+            if (L.win) {
+                Params<KIND, T> PW = P;
+                PW.gpdf.woff = L.wgrid.woff; PW.gpdf.wq0 = L.wgrid.wq0; PW.gpdf.wq1 = L.wgrid.wq1;
+                PW.gpdf.wx0 = L.wgrid.wx0; PW.gpdf.wstride = L.wgrid.wstride;
+                auto kern = k_unw_eval<KIND, T, kWinBlock, true>;
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.win_smem);
+                (void)hipGetLastError();
+                constexpr uint32_t per = kWinBlock / kBlock;
+                hipLaunchKernelGGL(kern, dim3((g.blocks + per - 1) / per), dim3(kWinBlock), L.win_smem, s, PW,
+                                   (T *)L.w, parts, g.blocks);
+                break;
+            }
+        }
+        hipLaunchKernelGGL((k_unw_eval<KIND, T>), dim3(g.blocks), dim3(kBlock), sm, s, P, (T *)L.w, parts, g.blocks);
         break;
     }
     case Stage::Flags: {
@@ -450,10 +506,11 @@ int occ_eval_h(const UnwLaunch &L)
 
 }  // namespace
 
-// Resident blocks per CU of the eval kernel the launch will use.
+// Resident blocks per CU of the eval kernel the launch will use (virtual 256-thread blocks).
 int occ_unweight_eval(const UnwLaunch &L)
 {
     if (L.partonic) return occ_eval_t<kPartonic, double>(0);
+    if (L.win) return kWinBlock / kBlock;                // one window block per CU
     return L.h.prec == MI_PREC_FP32 ? occ_eval_h<float>(L) : occ_eval_h<double>(L);
 }
 

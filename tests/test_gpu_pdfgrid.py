@@ -96,19 +96,33 @@ def test_grid_integral_vs_quadrature(ctx, O, grid_files):
     assert abs(et.sigmaGen() - e.sigmaGen()) > 5 * e.sigmaErr()
 
 
-def test_grid_unweight_bit_exact(ctx, O, grid_files):
-    main, _ = grid_files
-    ctx.pdf_load(main)
-    g = O.LhaGrid(main)
-    phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
-    cfg = O.hadcfg_default(grid=g)
-    n = 1 << 19
-    ev = ctx.unweight_compact(mi.make_run(phys, n, key=42, counter0=1000))
-    cnt, flags, pid, w = O.unweight32(cfg, 42, 1000, n, ev.wmax, None)
+@pytest.mark.parametrize("which,n,sampler,mask", [
+    (0, 1 << 19, mi.MI_SAMPLER_FLAT, 0x1F),
+    (1, (1 << 19) + 4096, mi.MI_SAMPLER_FLAT, 0x3F),        # an odd count of 256-wide eval blocks
+    (0, 1 << 19, mi.MI_SAMPLER_VEGAS, 0x1F),
+    (1, 1 << 18, mi.MI_SAMPLER_VEGAS, 0x3FF),
+])
+def test_grid_unweight_bit_exact(ctx, O, grid_files, which, n, sampler, mask):
+    """Compacted event indices and codes of FP32 grid-PDF unweighting (the evaluation reads the
+    grid's reachable window from LDS) equal the oracle's FP32 mirror bit for bit."""
+    path = grid_files[which]
+    ctx.pdf_load(path)
+    g = O.LhaGrid(path)
+    phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID, proc_mask=mask)
+    cfg = O.hadcfg_default(grid=g, procMask=mask)
+    edges = None
+    if sampler == mi.MI_SAMPLER_VEGAS:
+        ctx.vegas_init(3, 64)
+        ctx.vegas_adapt(mi.make_run(phys, 1 << 18, key=7), 4, 1.5)
+        edges = ctx.vegas_get_grid()
+    ev = ctx.unweight_compact(mi.make_run(phys, n, key=42, counter0=1000, sampler=sampler))
+    cnt, flags, pid, w = O.unweight32(cfg, 42, 1000, n, ev.wmax, edges)
     assert ev.wmax == float(w.max()) and ev.n == cnt and cnt > 0
     idx = np.nonzero(flags)[0]
     assert np.array_equal(ev.index.cpu().numpy(), idx + 1000)
     assert np.array_equal(ev.pid.cpu().numpy(), pid[idx])
+    est = ev.estimate.raw
+    assert est.wMax[0] == float(w.max()) and est.nNonZero == int(np.count_nonzero(w > 0))
 
 
 @pytest.mark.parametrize("sampler", [mi.MI_SAMPLER_FLAT, mi.MI_SAMPLER_VEGAS])
