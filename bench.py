@@ -221,6 +221,14 @@ def workloads(ctx, mi):
     out["pdf_grid"] = {"points": 1 << 26, "kernel_ms": ge.kernel_ms, "points_per_s": (1 << 26) / (ge.kernel_ms / 1e3),
                        "sigma_mb": ge.sigmaGen(), "sigma_err_mb": ge.sigmaErr(),
                        "grid": "100 x-nodes (NNPDF2.3 layout) x 50 Q-nodes, 11 flavours, FP32 Lagrange interpolation"}
+    # VEGAS adaptation with the grid PDFs (3-D, 64 bins, 12 x 2^24 points, alpha 1.5); the toy
+    # run's adapted grid is put back afterwards for the flow below
+    toy_grid = ctx.vegas_get_grid()
+    ctx.vegas_init(3, 64)
+    gv = ctx.vegas_adapt(mi.make_run(gphys, 1 << 24, key=51), 12, 1.5)
+    ctx.vegas_set_grid(toy_grid)
+    out["pdf_grid"]["vegas_ms"] = gv.kernel_ms
+    out["pdf_grid"]["vegas_points_per_s"] = (12 << 24) / (gv.kernel_ms / 1e3)
     # unweighting with the grid PDFs: 2^26 candidates of the flat map, exact maximum
     gu = ctx.unweight_compact(mi.make_run(gphys, 1 << 26, key=50))
     out["pdf_grid_unweight"] = {"candidates": 1 << 26, "accepted": gu.n, "unweight_ms": gu.estimate.kernel_ms,

@@ -226,16 +226,17 @@ Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks, uint32_t nt)
     return Geom{(uint32_t)blocks, (uint32_t)ppt};
 }
 
-// The LDS window of an FP32 grid-PDF integration (no VEGAS histogram): fills g's window and the
-// launch's dynamic LDS size; false when the run has none or the window does not fit one block's
-// LDS next to the VEGAS table and the reduction's static LDS.
+// The LDS window of an FP32 grid-PDF integration: fills g's window and the launch's dynamic LDS
+// size; false when the run has none or the window does not fit one block's LDS after the VEGAS
+// table and histograms (one copy per wave) and the reduction's static LDS.
 bool had_window(const mi_ctx *ctx, const mi_run *run, bool vegas, bool hist, GridPdf<float> *g, size_t *smem)
 {
-    if (hist || run->precision != MI_PREC_FP32 || run->phys.mode != MI_MODE_HADRONIC_3D ||
+    if (run->precision != MI_PREC_FP32 || run->phys.mode != MI_MODE_HADRONIC_3D ||
         run->phys.pdf_source != MI_PDF_GRID || ctx->lds_max <= 0)
         return false;
     const size_t tab = vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 7) & ~(size_t)7 : 0;
-    const size_t off = (tab + 15) & ~(size_t)15;
+    const size_t hb = (vegas && hist) ? sizeof(double) * (kWinBlock / 64) * 3 * ctx->grid.nbins : 0;
+    const size_t off = (tab + hb + 15) & ~(size_t)15;
     const size_t stat = (size_t)(kWinBlock / 64) * kRec * sizeof(double) + 1024;   // block_record + slack
     if ((size_t)ctx->lds_max <= stat) return false;
     size_t bytes = 0;

@@ -122,10 +122,10 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
 // Block b owns points [b*256*ppt, (b+1)*256*ppt); in step j thread t takes b*256*ppt + j*256 + t.
 // GRID: parton densities from the PDF grid gpdf instead of the toy PDFs.  HEAVY: mask may
 // select the heavy-quark codes 121..124 (their families are evaluated); otherwise only the
-// massless codes are computed and accumulated.  NT threads per block.  WIN (FP32 grid, no
-// histogram): the block first copies the window of the grid's node values that gpdf describes
-// (rows wq0..wq1, columns wx0..nx-1) into its dynamic LDS at gpdf.woff, so the interpolation
-// stencils read LDS instead of gathering from L2.
+// massless codes are computed and accumulated.  NT threads per block.  WIN (FP32 grid): the
+// block first copies the window of the grid's node values that gpdf describes (rows wq0..wq1,
+// columns wx0..nx-1) into its dynamic LDS at gpdf.woff (after the VEGAS table and histograms),
+// so the interpolation stencils read LDS instead of gathering from L2.
 template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, int NT = kBlock, bool WIN = false>
 __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
                                                       uint64_t ctr0, uint64_t n, uint32_t ppt,
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
                                                       double *__restrict__ partials,
                                                       double *__restrict__ hist, GridPdf<T> gpdf)
 {
-    static_assert(!WIN || (GRID && !HIST && sizeof(T) == 4), "the LDS window is for FP32 grid integration");
+    static_assert(!WIN || (GRID && sizeof(T) == 4), "the LDS window is for FP32 grid integration");
     unsigned char *smem = mi_lds;
     T *stab = reinterpret_cast<T *>(smem);                                   // 3 dims x nbins x {lo, wd, 1-lo}
     const size_t tabBytes = (sizeof(T) * 9 * (VEGAS ? nbins : 0) + 7) & ~(size_t)7;
@@ -236,11 +236,11 @@ static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadCons
     return hipGetLastError();
 }
 
-// FP32 grid integration with the LDS window: one 1024-thread block per CU.
-template <bool VEGAS, bool HEAVY>
+// FP32 grid integration with the LDS window: one kWinBlock-thread block per CU.
+template <bool VEGAS, bool HIST, bool HEAVY>
 static hipError_t launch_had_win(hipStream_t s, const HadLaunch &L, const float *tab)
 {
-    auto kern = k_integrate_had<float, VEGAS, false, true, HEAVY, kWinBlock, true>;
+    auto kern = k_integrate_had<float, VEGAS, HIST, true, HEAVY, kWinBlock, true>;
     // allows dynamic LDS above the default limit where the runtime asks for it; a runtime that
     // has no such limit may refuse the call, which the launch below then does not depend on
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -256,10 +256,13 @@ static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst
                                const GridPdf<T> &gp)
 {
     if constexpr (sizeof(T) == 4) {
-        if (L.win && gp.active && !L.hist) {
+        if (L.win && gp.active) {
             const bool hv = has_heavy(L.mask);
-            if (L.vegas) return hv ? launch_had_win<true, true>(s, L, tab) : launch_had_win<true, false>(s, L, tab);
-            return hv ? launch_had_win<false, true>(s, L, tab) : launch_had_win<false, false>(s, L, tab);
+            if (L.vegas && L.hist)
+                return hv ? launch_had_win<true, true, true>(s, L, tab) : launch_had_win<true, true, false>(s, L, tab);
+            if (L.vegas)
+                return hv ? launch_had_win<true, false, true>(s, L, tab) : launch_had_win<true, false, false>(s, L, tab);
+            return hv ? launch_had_win<false, false, true>(s, L, tab) : launch_had_win<false, false, false>(s, L, tab);
         }
     }
     if (has_heavy(L.mask))
