@@ -150,6 +150,16 @@ def test_stride_and_errors(ctx):
     assert int(r8.pdg[:, 4:].abs().max()) == 0 and float(r8.e[:, 4:].abs().max()) == 0.0
     with pytest.raises(mi.MIError):
         ctx.build_events(phys, ev, key=1, max_part=3)
+    # an event with a code outside the library gets no partons; the others are unchanged
+    bad = mi.Events(ev.sHat, ev.tHat, ev.x1, ev.x2, ev.weight, ev.pid.clone(), ev.index, ev.n, ev.wmax)
+    bad.pid[3] = 999
+    rb = ctx.build_events(phys, bad, key=1)
+    nb = rb.n_part.cpu().numpy()
+    assert nb[3] == 0 and np.all(np.delete(nb, 3) == 4)
+    keep = torch.ones(ev.n, dtype=torch.bool, device=rb.px.device)
+    keep[3] = False
+    assert torch.equal(rb.px[keep], r4.px[keep]) and torch.equal(rb.pdg[keep], r4.pdg[keep])
+    assert int(rb.pdg[3].abs().max()) == 0 and float(rb.e[3].abs().max()) == 0.0
     with mi.Context(0) as fresh:
         with pytest.raises(mi.MIError) as e:
             fresh.build_events(mi.physics_default(1, pdf_source=mi.MI_PDF_GRID), ev, key=1)
