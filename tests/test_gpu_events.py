@@ -80,6 +80,12 @@ def test_record_vs_oracle_partonic(ctx, O):
     compare(O, O.hadcfg_default(), rec, ev, 42, partonic=True, sHat=phys.sHat)
     e = rec.e.cpu().numpy()
     assert np.all(e == 0.5 * np.sqrt(phys.sHat))                  # 50 GeV per parton at sqrt(s) = 100 GeV
+    # the reference's 2 -> 2 four-momenta at sqrt(s) = 100 GeV (cuPythia/12_phase_space/README.md:8-16):
+    # max |p_cons| 3.6e-15, max |p^2 - m^2| 1.7e-12 GeV^2; the record does at least as well
+    p = rec.p4().cpu().numpy()
+    cons = np.abs(p[:, 2] + p[:, 3] - p[:, 0] - p[:, 1]).max()
+    msq = np.abs(p[:, :, 3] ** 2 - (p[:, :, :3] ** 2).sum(axis=2)).max()
+    assert cons <= 3.6e-15 and msq <= 1.7e-12, (cons, msq)
 
 
 def test_invariants_at_2p26(ctx):
