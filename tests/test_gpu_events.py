@@ -154,6 +154,11 @@ def test_stride_and_errors(ctx):
     r8 = ctx.build_events(phys, ev, key=1, max_part=8)
     assert torch.equal(r8.px[:, :4], r4.px) and torch.equal(r8.col[:, :4], r4.col)
     assert int(r8.pdg[:, 4:].abs().max()) == 0 and float(r8.e[:, 4:].abs().max()) == 0.0
+    # a stride that is not a multiple of 4: the per-element stores, the same record
+    r5 = ctx.build_events(phys, ev, key=1, max_part=5)
+    for f in ("px", "py", "pz", "e", "m", "pdg", "status", "col", "acol", "mo1", "mo2", "d1", "d2"):
+        assert torch.equal(getattr(r5, f)[:, :4], getattr(r4, f)), f
+    assert int(r5.pdg[:, 4].abs().max()) == 0 and torch.equal(r5.scale, r4.scale)
     with pytest.raises(mi.MIError):
         ctx.build_events(phys, ev, key=1, max_part=3)
     # an event with a code outside the library gets no partons; the others are unchanged
