@@ -2,15 +2,19 @@
 //
 // The partons of each event (flavours, colour flow, lab momenta, the hard scale) come from the
 // device event record (mi_events.hip), built in chunks and copied back; this file only formats
-// them.  Colour tags as in the record (501, ...); LHEF status -1 / 1, mothers 1 2 for the outgoing
+// them (host threads format contiguous slices of a chunk, written in event order, so the file is
+// the same whatever the thread count).  Colour tags as in the record (501, ...); LHEF status -1 / 1, mothers 1 2 for the outgoing
 // pair; weights and cross sections in pb (IDWTUP = 3), LPRUP = the Pythia process code.
 // Reference counterpart: cuPythia/pipeline/lhe_writer.cu:46-70, cuPythia/07_unweighting_lhe/
 // unweight_lhe.cu:74-102 (gg -> gg only, fixed colour flow).
 #include "mi_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 using namespace mi;
@@ -101,20 +105,39 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
             if (e == hipSuccess) e = hipMemcpyAsync(q.dst, q.src, q.b, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) break;
-        for (uint64_t i = 0; i < m; ++i) {
-            if (npart[i] != 4) continue;                     // unknown code: no partons
-            const size_t b = i * 4;
-            std::fprintf(f, "<event>\n4 %d %.10E %.10E -1.0E+00 -1.0E+00\n", pid[i], w[i] * mb2pb, scale[i]);
-            for (int k = 0; k < 4; ++k) {
-                const size_t j = b + k;
-                if (k < 2)
-                    std::fprintf(f, "%d -1 0 0 %d %d 0 0 %.10E %.10E 0 0 9\n", pdg[j], col[j], acol[j], pz[j], pe[j]);
-                else
-                    std::fprintf(f, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E %.10E 0 9\n", pdg[j], col[j], acol[j],
-                                 px[j], py[j], pz[j], pe[j], pm[j]);
+        // text of events [a, b) of the chunk
+        auto format = [&](uint64_t a, uint64_t b, std::string &out) {
+            char buf[512];
+            out.clear();
+            out.reserve((size_t)(b - a) * 600);
+            for (uint64_t i = a; i < b; ++i) {
+                if (npart[i] != 4) continue;                 // unknown code: no partons
+                const size_t j0 = i * 4;
+                int len = std::snprintf(buf, sizeof buf, "<event>\n4 %d %.10E %.10E -1.0E+00 -1.0E+00\n", pid[i],
+                                        w[i] * mb2pb, scale[i]);
+                out.append(buf, (size_t)len);
+                for (int k = 0; k < 4; ++k) {
+                    const size_t j = j0 + k;
+                    if (k < 2)
+                        len = std::snprintf(buf, sizeof buf, "%d -1 0 0 %d %d 0 0 %.10E %.10E 0 0 9\n", pdg[j], col[j],
+                                            acol[j], pz[j], pe[j]);
+                    else
+                        len = std::snprintf(buf, sizeof buf, "%d 1 1 2 %d %d %.10E %.10E %.10E %.10E %.10E 0 9\n",
+                                            pdg[j], col[j], acol[j], px[j], py[j], pz[j], pe[j], pm[j]);
+                    out.append(buf, (size_t)len);
+                }
+                out.append("</event>\n");
             }
-            std::fprintf(f, "</event>\n");
-        }
+        };
+        // formatted by host threads over contiguous slices, written in event order
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>(hw, m / 4096));
+        std::vector<std::string> text(nt);
+        std::vector<std::thread> pool;
+        for (uint64_t t = 1; t < nt; ++t) pool.emplace_back(format, m * t / nt, m * (t + 1) / nt, std::ref(text[t]));
+        format(0, m / nt, text[0]);
+        for (std::thread &th : pool) th.join();
+        for (const std::string &t : text) std::fwrite(t.data(), 1, t.size(), f);
     }
     std::fprintf(f, "</LesHouchesEvents>\n");
     const bool ok = std::fclose(f) == 0;
