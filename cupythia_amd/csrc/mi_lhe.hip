@@ -3,8 +3,11 @@
 // The partons of each event (flavours, colour flow, lab momenta, the hard scale) come from the
 // device event record (mi_events.hip), built in chunks and copied back; this file only formats
 // them (host threads format contiguous slices of a chunk, written in event order, so the file is
-// the same whatever the thread count).  Colour tags as in the record (501, ...); LHEF status -1 / 1, mothers 1 2 for the outgoing
-// pair; weights and cross sections in pb (IDWTUP = 3), LPRUP = the Pythia process code.
+// the same whatever the thread count).  Colour tags as in the record (501, ...); LHEF status -1 / 1,
+// mothers 1 2 for the outgoing pair; cross sections in pb, LPRUP = the Pythia process code.
+// Unweighted events (IDWTUP = 3): every event carries XWGTUP = sigma_tot / N in pb (so the weights
+// sum to sigma; Pythia's strategy 3 reads the sign only and sigma from XSECUP), SCALUP the hard
+// scale, AQCDUP alphaS at it (the run's running coupling), AQEDUP alphaEM(MZ).
 // Reference counterpart: cuPythia/pipeline/lhe_writer.cu:46-70, cuPythia/07_unweighting_lhe/
 // unweight_lhe.cu:74-102 (gg -> gg only, fixed colour flow).
 #include "mi_internal.h"
@@ -87,8 +90,14 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
                          est->wMax[k] * mb2pb, MI_PROC_CODE(k - 1));
     std::fprintf(f, "</init>\n");
 
+    // per-event weight of unweighted events, alphaS at the event's scale, alphaEM(MZ) (Pythia's
+    // StandardModel:alphaEMmZ)
+    const double xw = (est->sigmaGen[0] > 0.0 && n > 0) ? est->sigmaGen[0] * mb2pb / (double)n : 1.0;
+    const double aqed = 0.00781751;
+    HadConst<double> kd{};
+    if (!partonic) fill_had_consts_f64(*phys, kd);
     const size_t np = (size_t)cap * 4;
-    std::vector<double> px(np), py(np), pz(np), pe(np), pm(np), scale(cap), w(cap);
+    std::vector<double> px(np), py(np), pz(np), pe(np), pm(np), scale(cap);
     std::vector<int32_t> pdg(np), col(np), acol(np), npart(cap), pid(cap);
     hipError_t e = hipSuccess;
     for (uint64_t first = 0; first < n && e == hipSuccess; first += cap) {
@@ -99,7 +108,7 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
             {px.data(), rec.r.px, dp}, {py.data(), rec.r.py, dp}, {pz.data(), rec.r.pz, dp}, {pe.data(), rec.r.e, dp},
             {pm.data(), rec.r.m, dp}, {pdg.data(), rec.r.pdg, ip}, {col.data(), rec.r.col, ip},
             {acol.data(), rec.r.acol, ip}, {npart.data(), rec.r.n_part, sizeof(int32_t) * m},
-            {scale.data(), rec.r.scale, sizeof(double) * m}, {w.data(), ev->weight + first, sizeof(double) * m},
+            {scale.data(), rec.r.scale, sizeof(double) * m},
             {pid.data(), ev->pid + first, sizeof(int32_t) * m}};
         for (auto &q : cp)
             if (e == hipSuccess) e = hipMemcpyAsync(q.dst, q.src, q.b, hipMemcpyDeviceToHost, ctx->stream);
@@ -113,8 +122,9 @@ extern "C" int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phy
             for (uint64_t i = a; i < b; ++i) {
                 if (npart[i] != 4) continue;                 // unknown code: no partons
                 const size_t j0 = i * 4;
-                int len = std::snprintf(buf, sizeof buf, "<event>\n4 %d %.10E %.10E -1.0E+00 -1.0E+00\n", pid[i],
-                                        w[i] * mb2pb, scale[i]);
+                const double aqcd = partonic ? phys->alphaS_value : alphas_f64(kd, scale[i] * scale[i]);
+                int len = std::snprintf(buf, sizeof buf, "<event>\n4 %d %.10E %.10E %.10E %.10E\n", pid[i], xw,
+                                        scale[i], aqed, aqcd);
                 out.append(buf, (size_t)len);
                 for (int k = 0; k < 4; ++k) {
                     const size_t j = j0 + k;

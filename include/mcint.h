@@ -274,8 +274,9 @@ int mi_build_events(mi_ctx *ctx, const mi_physics *phys, const mi_events *ev, ui
 
 /* -------------------------------------------------------------- LHE output ---- */
 /* LHEF 3.0 file of n compacted events (device SoA, e.g. from mi_unweight_compact): the partons
- * of mi_build_events (same key: flavours, colour flow, lab momenta), weights and cross sections
- * (est->sigmaGen per process code) in pb.  Replaces cuPythia/pipeline/lhe_writer.cu:46-70 and
+ * of mi_build_events (same key: flavours, colour flow, lab momenta), cross sections
+ * (est->sigmaGen per process code) in pb; unweighted events (IDWTUP = 3) each with XWGTUP =
+ * est->sigmaGen[0] / n in pb, SCALUP the hard scale, AQCDUP alphaS at it, AQEDUP alphaEM(MZ).  Replaces cuPythia/pipeline/lhe_writer.cu:46-70 and
  * cuPythia/07_unweighting_lhe/unweight_lhe.cu:74-102. */
 int mi_write_lhe(mi_ctx *ctx, const char *path, const mi_physics *phys, const mi_events *ev, uint64_t n,
                  uint64_t key, const mi_sigma_est *est);

@@ -1,6 +1,7 @@
 """LHEF output of compacted events: structure, four-momentum conservation, masslessness,
 consistency with the compacted record, flavour content of each process's flux list and colour-flow
 validity (every colour tag connects exactly one colour end and one anticolour end)."""
+import ctypes
 import math
 from collections import Counter
 
@@ -44,7 +45,7 @@ def colour_ok(parts):
     return all(ends[(t, "c")] == 1 and ends[(t, "a")] == 1 for t in tags)
 
 
-def test_lhe_hadronic(ctx, tmp_path):
+def test_lhe_hadronic(ctx, O, tmp_path):
     phys = mi.physics_default(1, proc_mask=0x3F)
     run = mi.make_run(phys, 1 << 18, key=21, counter0=99)
     ev = ctx.unweight_compact(run)
@@ -58,8 +59,14 @@ def test_lhe_hadronic(ctx, tmp_path):
     sH = ev.sHat.cpu().numpy()
     pid = ev.pid.cpu().numpy()
     flux = {111: "gg", 112: "gg", 113: "qg", 114: "qq", 115: "qqbar", 116: "qqbar"}
+    # unweighted events: one weight, sigma_tot / N in pb; alphaEM(MZ); alphaS at the event's scale
+    xw = ev.estimate.sigmaGen() * 1e9 / ev.n
+    cfg = O.hadcfg_default(procMask=0x3F)
     for n, (head, parts) in enumerate(events):
         assert int(head[0]) == 4 and int(head[1]) == pid[n]
+        assert float(head[2]) == pytest.approx(xw, rel=1e-10) and float(head[4]) == pytest.approx(0.00781751, rel=1e-10)
+        scale = float(head[3])
+        assert float(head[5]) == pytest.approx(O.lib().orc_alphas_eval(ctypes.byref(cfg.as_), scale * scale), rel=1e-9)
         p = [[float(x) for x in q[6:10]] for q in parts]
         tot = [p[2][j] + p[3][j] - p[0][j] - p[1][j] for j in range(4)]
         scale = max(abs(x) for x in p[0] + p[1])
