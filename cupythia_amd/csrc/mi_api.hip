@@ -246,8 +246,10 @@ bool had_window(const mi_ctx *ctx, const mi_run *run, bool vegas, bool hist, Gri
 }
 
 // One wave of equal-work blocks over all CUs: every block takes ceil(n / (blocks * 256))
-// points per thread, so the grid drains together and the block records stay few.  With the LDS
-// window: one 1024-thread block per CU.
+// points per thread, so the grid drains together and the block records stay few.  Small runs
+// (VEGAS iterations, pilots) still fill the resident-block slots down to 4 points per thread:
+// one 256-thread block per CU would leave each SIMD one wave and no latency hiding.  With the
+// LDS window: one kWinBlock-thread block per CU.
 Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas, bool hist)
 {
     {
@@ -262,13 +264,14 @@ Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas
                                             has_heavy(run->phys.proc_mask))
                         : occ_integrate_c1(vegas, hist, nb);
     const uint32_t cap = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * (uint32_t)occ;
-    return geom_for(n, 16, cap);
+    return geom_for(n, 4, cap);
 }
 
 // Integration driver shared by mi_integrate, VEGAS and unweighting: launches the integrand
-// over [ctr0, ctr0 + n) and the reduction into out_dev (kRec doubles).
+// over [ctr0, ctr0 + n) and the reduction into out_dev (kRec doubles); reduce = false leaves the
+// block records in ctx->partials for the caller's own reduction.
 int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, bool vegas, double *hist_parts,
-                  Geom g, double *out_dev, hipEvent_t after_kernel = nullptr)
+                  Geom g, double *out_dev, hipEvent_t after_kernel = nullptr, bool reduce = true)
 {
     int rc = ensure(ctx, &ctx->partials, &ctx->partials_cap, (size_t)g.blocks * kRec);
     if (rc) return rc;
@@ -291,7 +294,7 @@ int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, boo
         HIPCHK(ctx, launch_integrate_c1(ctx->stream, L));
     }
     if (after_kernel) HIPCHK(ctx, hipEventRecord(after_kernel, ctx->stream));
-    HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, out_dev));
+    if (reduce) HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, out_dev));
     return MI_OK;
 }
 
@@ -555,10 +558,10 @@ int mi_vegas_adapt(mi_ctx *ctx, const mi_run *run, int n_iter, double alpha, mi_
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int it = 0; it < n_iter; ++it) {
         rc = run_integrand(ctx, &r, r.counter0 + (uint64_t)it * r.n_points, r.n_points, true, ctx->hist, g,
-                           ctx->red + (size_t)it * kRec);
+                           nullptr, nullptr, false);
         if (rc) return rc;
-        HIPCHK(ctx, launch_reduce_columns(ctx->stream, ctx->hist, g.blocks, ncol, hsum));
-        HIPCHK(ctx, launch_vegas_refine(ctx->stream, gr, hsum, alpha));
+        HIPCHK(ctx, launch_vegas_step(ctx->stream, ctx->partials, g.blocks, ctx->red + (size_t)it * kRec, ctx->hist,
+                                      gr, hsum, alpha));
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<double> rec((size_t)n_iter * kRec);

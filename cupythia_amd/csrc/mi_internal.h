@@ -142,11 +142,13 @@ hipError_t launch_points_c1(hipStream_t s, const C1Launch &L, double sHat, doubl
 
 // Deterministic reductions.
 hipError_t launch_reduce_records(hipStream_t s, const double *partials, uint32_t nblk, double *out);
-hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nblk, uint32_t ncol,
-                                 double *out);
 
 // VEGAS refinement on the device (one block).
-hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, double alpha);
+// One VEGAS iteration's reductions and refinement in two launches: the nblk block records into
+// rec_out (kRec doubles) and the nblk x (ndim * nbins) histogram rows into hsum, then the grid
+// edges and tables refined from hsum.
+hipError_t launch_vegas_step(hipStream_t s, const double *partials, uint32_t nblk, double *rec_out,
+                             const double *hist_parts, Grid &g, double *hsum, double alpha);
 hipError_t launch_vegas_tables(hipStream_t s, Grid &g);
 
 // Unweighting (mi_unweight.hip).

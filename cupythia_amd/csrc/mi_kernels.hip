@@ -467,37 +467,39 @@ hipError_t launch_reduce_records(hipStream_t s, const double *partials, uint32_t
     return hipGetLastError();
 }
 
-// Column sums of an nblk x ncol matrix (the VEGAS histograms of the blocks): block c folds column c.
-__global__ __launch_bounds__(kBlock) void k_reduce_columns(const double *__restrict__ parts, uint32_t nblk,
-                                                           uint32_t ncol, double *__restrict__ out)
-{
-    fold_column(parts, nblk, ncol, blockIdx.x, false, out + blockIdx.x);
-}
-
-hipError_t launch_reduce_columns(hipStream_t s, const double *parts, uint32_t nblk, uint32_t ncol, double *out)
-{
-    hipLaunchKernelGGL(k_reduce_columns, dim3(ncol), dim3(kBlock), 0, s, parts, nblk, ncol, out);
-    return hipGetLastError();
-}
-
 // ------------------------------------------------------------------------------------ VEGAS refinement
 // Lepage, J. Comput. Phys. 27 (1978) 192: per dimension, d_i = sum of w^2 in bin i, smoothed
 // over neighbours (ends averaged with one neighbour), normalised, damped
 // m_i = ((1 - d_i) / -ln d_i)^alpha, then the edges are moved so every new bin holds an equal
-// share of sum m_i, interpolating linearly inside the old bins.  One block per dimension: the
-// histogram and edges are staged in LDS, the per-bin smoothing and damping run one thread per
-// bin, the two sums and the edge walk run in one thread in bin order.
-__global__ __launch_bounds__(kMaxBins) void k_vegas_refine(double *__restrict__ edges, const double *__restrict__ hist,
-                                                           int ndim, int nbins, double alpha)
+// share of sum m_i, interpolating linearly inside the old bins.  One block, one thread per bin:
+// the smoothed total is a fixed-shape block sum, sum m an inclusive block scan C_i (fixed shape,
+// so the result does not depend on thread timing), and new edge j takes the first old bin
+// k < nbins - 1 with C_k >= j * C_last / nbins (else the last bin) by binary search over C: the
+// bin that the sequential walk of the old edges stops in.  The per-bin tables {lo, width, 1 - lo}
+// of the new edges (FP64 and FP32) are written as well.
+static_assert(kBlock == kMaxBins, "the refinement runs one thread per bin in a kBlock-thread block");
+
+__device__ __forceinline__ double block_sum_bins(double v, double *wpart)
 {
-    __shared__ double h[kMaxBins], m[kMaxBins], x[kMaxBins + 1], ne[kMaxBins + 1];
-    __shared__ int go;
-    const int d = blockIdx.x, i = threadIdx.x;
-    if (d >= ndim) return;
-    const double *hd = hist + (size_t)d * nbins;
-    double *xd = edges + (size_t)d * (nbins + 1);
+    const double s = wave_sum(v);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wpart[wave] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kMaxBins / 64; ++w) t += wpart[w];
+    __syncthreads();
+    return t;
+}
+
+// Refines one dimension: hd = its nbins histogram sums, xd = its nbins + 1 edges (updated in
+// place), t64d / t32d = its 3 * nbins table entries.  Called by every thread of the block.
+__device__ void refine_dim(const double *__restrict__ hd, double *__restrict__ xd, int nbins, double alpha,
+                           double *__restrict__ t64d, float *__restrict__ t32d)
+{
+    __shared__ double h[kMaxBins], c[kMaxBins], x[kMaxBins + 1], ne[kMaxBins + 1], wpart[kMaxBins / 64];
+    const int i = threadIdx.x, wave = i >> 6, lane = i & 63;
     if (i < nbins) h[i] = hd[i];
-    for (int j = i; j <= nbins; j += blockDim.x) x[j] = xd[j];
+    for (int j = i; j <= nbins; j += kMaxBins) { x[j] = xd[j]; ne[j] = x[j]; }
     __syncthreads();
     double s = 0.0;
     if (i < nbins) {
@@ -505,50 +507,89 @@ __global__ __launch_bounds__(kMaxBins) void k_vegas_refine(double *__restrict__ 
         else if (i == 0) s = 0.5 * (h[0] + h[1]);
         else if (i == nbins - 1) s = 0.5 * (h[nbins - 2] + h[nbins - 1]);
         else s = (h[i - 1] + h[i] + h[i + 1]) / 3.0;
-        m[i] = s;
     }
-    __syncthreads();
-    if (i == 0) {
-        double tot = 0.0;
-        for (int j = 0; j < nbins; ++j) tot += m[j];
-        go = tot > 0.0;
-        h[0] = tot;                                                // the total, for every thread
-    }
-    __syncthreads();
-    if (!go) return;                                               // nothing to adapt to
-    const double tot = h[0];
-    if (i < nbins) {
+    const double tot = block_sum_bins(s, wpart);
+    double mi = 0.0;
+    if (i < nbins && tot > 0.0) {
         const double r = s / tot;
-        double mi = 0.0;
         if (r > 0.0) mi = (r < 1.0) ? pow((1.0 - r) / -log(r), alpha) : 1.0;
-        m[i] = mi;
     }
+    if (i < nbins) h[i] = mi;                                        // the smoothing reads are done
+    // inclusive scan: within the wave (shifts 1..32), then the totals of the lower waves
+    double sc = mi;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double up = __shfl_up(sc, off, 64);
+        if (lane >= off) sc += up;
+    }
+    if (lane == 63) wpart[wave] = sc;
     __syncthreads();
-    if (i == 0) {
-        double msum = 0.0;
-        for (int j = 0; j < nbins; ++j) msum += m[j];
-        go = msum > 0.0;
-        if (go) {
-            const double step = msum / nbins;
-            ne[0] = x[0];
-            ne[nbins] = x[nbins];
-            int old = 0;
-            double acc = 0.0;                                      // sum of m over bins < old
-            for (int j = 1; j < nbins; ++j) {
-                const double target = step * j;
-                while (old < nbins - 1 && acc + m[old] < target) { acc += m[old]; ++old; }
-                double frac = (m[old] > 0.0) ? (target - acc) / m[old] : 0.0;
-                frac = fmin(fmax(frac, 0.0), 1.0);
-                ne[j] = x[old] + frac * (x[old + 1] - x[old]);
-            }
+    for (int w = 0; w < wave; ++w) sc += wpart[w];
+    c[i] = sc;
+    __syncthreads();
+    const double msum = c[nbins - 1];
+    if (tot > 0.0 && msum > 0.0 && i >= 1 && i < nbins) {
+        const double target = (msum / nbins) * i;
+        int lo = 0, hi = nbins - 1;                                  // first k < nbins - 1 with c[k] >= target
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (c[mid] >= target) hi = mid; else lo = mid + 1;
         }
+        const double acc = lo > 0 ? c[lo - 1] : 0.0;
+        double frac = (h[lo] > 0.0) ? (target - acc) / h[lo] : 0.0;
+        frac = fmin(fmax(frac, 0.0), 1.0);
+        ne[i] = x[lo] + frac * (x[lo + 1] - x[lo]);
     }
     __syncthreads();
-    if (!go) return;
-    for (int j = i; j <= nbins; j += blockDim.x) xd[j] = ne[j];
+    for (int j = i; j <= nbins; j += kMaxBins) xd[j] = ne[j];
+    if (i < nbins) {
+        const double lo = ne[i], hi = ne[i + 1];
+        t64d[3 * i + 0] = lo; t64d[3 * i + 1] = hi - lo; t64d[3 * i + 2] = 1.0 - lo;
+        t32d[3 * i + 0] = (float)lo; t32d[3 * i + 1] = (float)(hi - lo); t32d[3 * i + 2] = (float)(1.0 - lo);
+    }
+    __syncthreads();                                                 // LDS reused by the next dimension
 }
 
-// Per-bin tables {lo, width, 1 - lo} in FP64 and FP32 from the FP64 edges.
+// The reductions of one VEGAS iteration in one launch: blocks 0 .. kRec - 1 fold the block records
+// into rec_out (as k_reduce_records), blocks kRec .. kRec + ncol - 1 fold the histogram columns
+// into hsum (one block per column).
+__global__ __launch_bounds__(kBlock) void k_vegas_fold(const double *__restrict__ parts, uint32_t nblk,
+                                                       double *__restrict__ rec_out,
+                                                       const double *__restrict__ hparts, uint32_t ncol,
+                                                       double *__restrict__ hsum)
+{
+    const uint32_t b = blockIdx.x;
+    if (b < (uint32_t)kRec) fold_column(parts, nblk, kRec, b, rec_is_max((int)b), rec_out + b);
+    else fold_column(hparts, nblk, ncol, b - kRec, false, hsum + (b - kRec));
+}
+
+// Block d refines dimension d of the grid from hsum (the next launch on the stream, so hsum is
+// complete).
+__global__ __launch_bounds__(kBlock) void k_vegas_refine(const double *__restrict__ hsum, double *__restrict__ edges,
+                                                         int nbins, double alpha, double *__restrict__ t64,
+                                                         float *__restrict__ t32)
+{
+    const int d = blockIdx.x;
+    refine_dim(hsum + (size_t)d * nbins, edges + (size_t)d * (nbins + 1), nbins, alpha, t64 + (size_t)3 * d * nbins,
+               t32 + (size_t)3 * d * nbins);
+}
+
+hipError_t launch_vegas_step(hipStream_t s, const double *partials, uint32_t nblk, double *rec_out,
+                             const double *hist_parts, Grid &g, double *hsum, double alpha)
+{
+    if (g.ndim < 1 || g.nbins < 1 || g.nbins > kMaxBins) return hipErrorInvalidValue;
+    const uint32_t ncol = (uint32_t)(g.ndim * g.nbins);
+    hipLaunchKernelGGL(k_vegas_fold, dim3(kRec + ncol), dim3(kBlock), 0, s, partials, nblk, rec_out, hist_parts,
+                       ncol, hsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vegas_refine, dim3(g.ndim), dim3(kBlock), 0, s, hsum, g.edges, g.nbins, alpha, g.tab64,
+                       g.tab32);
+    return hipGetLastError();
+}
+
+// Per-bin tables {lo, width, 1 - lo} in FP64 and FP32 from the FP64 edges (after an explicit
+// grid set or reset; the refinement writes them itself).
 __global__ void k_vegas_tables(const double *__restrict__ edges, int ndim, int nbins, double *__restrict__ t64,
                                float *__restrict__ t32)
 {
@@ -558,14 +599,6 @@ __global__ void k_vegas_tables(const double *__restrict__ edges, int ndim, int n
     const double lo = edges[d * (nbins + 1) + b], hi = edges[d * (nbins + 1) + b + 1];
     t64[3 * i + 0] = lo; t64[3 * i + 1] = hi - lo; t64[3 * i + 2] = 1.0 - lo;
     t32[3 * i + 0] = (float)lo; t32[3 * i + 1] = (float)(hi - lo); t32[3 * i + 2] = (float)(1.0 - lo);
-}
-
-hipError_t launch_vegas_refine(hipStream_t s, Grid &g, const double *hist_sum, double alpha)
-{
-    hipLaunchKernelGGL(k_vegas_refine, dim3(g.ndim), dim3(kMaxBins), 0, s, g.edges, hist_sum, g.ndim, g.nbins, alpha);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_vegas_tables(s, g);
 }
 
 hipError_t launch_vegas_tables(hipStream_t s, Grid &g)
