@@ -38,7 +38,7 @@ constexpr int HS = 72;      // LDS row stride (bf16) of 64-wide hidden activatio
 constexpr int OS = 52;      // LDS row stride (fp32) of the 49 MLP outputs (208 B rows, 16 B aligned)
 constexpr int GS = 68;      // LDS row stride (fp32) of 64-wide gradient rows
 constexpr int FWF = 4;      // waves per block of the forward / inverse layer kernel
-constexpr int FWB = 3;      // waves per block of the backward layer kernel
+constexpr int FWB = 4;      // waves per block of the backward layer kernel (one per SIMD)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -53,6 +53,14 @@ struct PackedLayer {
     uint16_t W3[64 * FH];      // BF16 [o][h], rows 49..63 zero
 };
 static_assert(sizeof(PackedLayer) % 16 == 0, "PackedLayer copies as uint4");
+// LDS of the backward layer kernel: the staged layer and per wave H1, H2 (bf16, 64 x HS), G (fp32,
+// 64 x GS) and the conditioning coordinates (fp32, 64 x 2); one block of FWB = 4 waves per CU
+// (159 KB of the 160 KB).  After the tiles the block's four gradient rows reuse it: two rows in the
+// G buffers, one in the H1 buffers, one in the H2 buffers.
+constexpr size_t kBwdLds = sizeof(PackedLayer) + FWB * (sizeof(uint16_t) * 2 * 64 * HS + sizeof(float) * (64 * GS + 128));
+static_assert(FWB == 4 && kBwdLds <= 160 * 1024, "backward LDS layout");
+static_assert(2 * FPL * sizeof(float) <= sizeof(float) * FWB * 64 * GS &&
+              FPL * sizeof(float) <= sizeof(uint16_t) * FWB * 64 * HS, "gradient rows in the backward LDS");
 
 // FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32 converts two at a time).
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -231,6 +239,14 @@ __device__ __forceinline__ void stage_layer(PackedLayer &dst, const PackedLayer 
     __syncthreads();
 }
 
+// Element i (0..2) of a 3-vector by selects, so that a per-launch index keeps the vector in registers.
+__device__ __forceinline__ float pick3(const float *v, int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
+__device__ __forceinline__ void set3(float *v, int i, float x)
+{
+    v[0] = i == 0 ? x : v[0];
+    v[1] = i == 1 ? x : v[1];
+    v[2] = i == 2 ? x : v[2];
+}
 __device__ __forceinline__ int cond0(int d) { return d == 0 ? 1 : 0; }
 __device__ __forceinline__ int cond1(int d) { return d == 2 ? 1 : 2; }
 
@@ -399,8 +415,9 @@ __global__ __launch_bounds__(64 * FWF) void k_flow_layer(const PackedLayer *__re
 // Adjoint of one layer for L = sum_i (abar_out_i . x_out_i + g_i log S'_i) summed with the
 // later layers' contributions: abar (n x 3) holds dL/d(output coords) on entry and dL/d(input
 // coords) on exit.  FWB waves per block, each with its own tiles; the weight gradients dW2, dW3
-// accumulate in the matrix-core accumulators of the wave across all its tiles, and each wave
-// writes its parameter gradient to row (blockIdx.x * FWB + wave) of gp (rows x FPL).
+// accumulate in the matrix-core accumulators of the wave across all its tiles.  At the end the
+// waves write their parameter gradients into the block's LDS and the block writes their sum
+// (in wave order) to row blockIdx.x of gp (rows x FPL).
 __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *__restrict__ layers, int l, uint64_t n,
                                                              const float *__restrict__ xin, const float *__restrict__ g,
                                                              float *__restrict__ abar, float *__restrict__ gp)
@@ -433,24 +450,25 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
             ab[0] = abar[3 * i]; ab[1] = abar[3 * i + 1]; ab[2] = abar[3 * i + 2];
             gi = g[i];
         }
-        Cc[2 * lane] = x[ca];
-        Cc[2 * lane + 1] = x[cb];
-        mlp_tile(P, x[ca], x[cb], H1, H2, G, GS, 64);
+        Cc[2 * lane] = pick3(x, ca);
+        Cc[2 * lane + 1] = pick3(x, cb);
+        mlp_tile(P, pick3(x, ca), pick3(x, cb), H1, H2, G, GS, 64);
         // ---- spline backward (lane = point), dO written over the outputs
         {
             float *o = G + lane * GS;
             float pw[FK], ph[FK];
             softmax16(o, pw);
             softmax16(o + FK, ph);
-            const Bin b = locate(o, pw, ph, x[d], false);
+            const float xd = pick3(x, d);
+            const Bin b = locate(o, pw, ph, xd, false);
             const float iw = 1.0f / b.wk;
-            const float xi = (x[d] - b.xk) * iw, s = b.hk * iw, t = xi * (1.0f - xi);
+            const float xi = (xd - b.xk) * iw, s = b.hk * iw, t = xi * (1.0f - xi);
             const float A = b.dk1 + b.dk - 2.0f * s, den = s + A * t;
             const float iden = 1.0f / den, iden2 = iden * iden;
             const float P2 = s * xi * xi + b.dk * t, R = P2 * iden;
             const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
             const float in2 = 1.0f / n2;
-            const float aS = live ? ab[d] : 0.0f, gL = live ? gi : 0.0f;
+            const float aS = live ? pick3(ab, d) : 0.0f, gL = live ? gi : 0.0f;
             const float omx = 1.0f - xi, om2t = 1.0f - 2.0f * t;
             // d R / d (xi, s, dk, dk1)
             const float Pxi = 2.0f * s * xi + b.dk * (1.0f - 2.0f * xi), denxi = A * (1.0f - 2.0f * xi);
@@ -497,7 +515,7 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
                 o[2 * FK + b.k] = Gdk * sgk;
                 o[2 * FK + b.k + 1] = Gdk1 * sgk1;
             }
-            ab[d] = live ? dx : 0.0f;
+            set3(ab, d, live ? dx : 0.0f);
         }
         wave_sync();
         // ---- MLP backward
@@ -534,8 +552,8 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
             dc1 += v * P.W1[2 * j + 1];
         }
         if (live) {
-            ab[ca] += dc0;
-            ab[cb] += dc1;
+            set3(ab, ca, pick3(ab, ca) + dc0);
+            set3(ab, cb, pick3(ab, cb) + dc1);
             abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
         }
         wave_sync();                                   // row reads done before the next tile
@@ -548,7 +566,12 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
         gw11[h] += __shfl_xor(gw11[h], 32, 64);
         gb2[h] += __shfl_xor(gb2[h], 32, 64);
     }
-    float *out = gp + ((size_t)blockIdx.x * FWB + wave) * FPL;
+    __syncthreads();                                   // every wave is past its tiles: LDS holds the rows now
+    // the FWB = 4 gradient rows of the block: two in Gs, one each in H1s and H2s
+    auto rowp = [&](int w) -> float * {
+        return w < 2 ? &Gs[0][0] + (size_t)w * FPL : reinterpret_cast<float *>(w == 2 ? &H1s[0][0] : &H2s[0][0]);
+    };
+    float *out = rowp(wave);
     if (lane < 32) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -562,6 +585,14 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
     if (lane < FO) out[PB3 + lane] = gb3;
     acc_each(gW2, [&](int m, int nn, float v) { out[PW2 + m * FH + nn] = v; });
     acc_each(gW3, [&](int m, int nn, float v) { if (m < FO) out[PW3 + m * FH + nn] = v; });
+    __syncthreads();
+    float *dst = gp + (size_t)blockIdx.x * FPL;
+    for (int j = threadIdx.x; j < FPL; j += 64 * FWB) {
+        float t = rowp(0)[j];
+#pragma unroll
+        for (int w = 1; w < FWB; ++w) t += rowp(w)[j];
+        dst[j] = t;
+    }
 }
 
 // ------------------------------------------------------------------------------------ small kernels
@@ -710,7 +741,7 @@ struct FlowState {
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
     float *ld = nullptr, *w = nullptr, *g = nullptr, *abar = nullptr;
-    float *gp = nullptr;        // per-wave layer gradients, FL x gp_rows x FPL
+    float *gp = nullptr;        // per-block layer gradients, FL x gp_rows x FPL
     uint32_t gp_rows = 0;
     double *hist = nullptr;     // training history (sum w, sum w^2 per step)
     int hist_cap = 0;
@@ -755,7 +786,7 @@ int num_cu(mi_ctx *ctx) { return ctx->num_cu > 0 ? ctx->num_cu : 256; }
 // Grids: the forward / inverse kernel keeps as many blocks of FWF waves per CU resident as its
 // LDS allows (the staged layer plus FWF output buffers: two blocks of 4 waves, 70 KB each; 10-wave
 // blocks measured slower, 123 vs 115 us per layer over 2^22 points) and walks
-// the tiles with a grid stride; the backward kernel (124 KB of LDS) one block of FWB waves per CU.
+// the tiles with a grid stride; the backward kernel (159 KB of LDS) one block of FWB waves per CU.
 constexpr uint64_t kFwdLds = sizeof(PackedLayer) + sizeof(float) * FWF * 64 * OS;
 constexpr uint64_t kFwdPerCu = (160 * 1024) / kFwdLds > 0 ? (160 * 1024) / kFwdLds : 1;
 uint32_t fwd_blocks(mi_ctx *ctx, uint64_t n)
@@ -783,7 +814,7 @@ int flow_alloc(mi_ctx *ctx, uint64_t npts)
         FCHK(hipMalloc(&F.abar, sizeof(float) * 3 * npts));
         F.cap = npts;
     }
-    const uint32_t rows = (uint32_t)num_cu(ctx) * FWB;
+    const uint32_t rows = (uint32_t)num_cu(ctx);                  // one gradient row per backward block
     if (rows > F.gp_rows) {
         (void)hipFree(F.gp);
         F.gp = nullptr;
@@ -944,7 +975,7 @@ static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev)
 {
     FlowState &F = *ctx->flow;
     const uint32_t blocks = bwd_blocks(ctx, n);
-    const uint32_t rows = blocks * FWB;
+    const uint32_t rows = blocks;
     if (rows > F.gp_rows) return ferr(ctx, MI_ERR_STATE, "gradient partials buffer too small");
     for (int l = FL - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64 * FWB), 0, ctx->stream, F.packed, l, n,
