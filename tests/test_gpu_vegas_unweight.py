@@ -37,12 +37,14 @@ def test_vegas_deterministic_and_alpha0(ctx):
     assert np.allclose(ctx.vegas_get_grid(), np.tile(np.linspace(0, 1, 65), (3, 1)), atol=1e-14)
 
 
-def test_vegas_refine_kernel_matches_oracle_refine(ctx, O):
+@pytest.mark.parametrize("nb", [64, 50, 256])
+def test_vegas_refine_kernel_matches_oracle_refine(ctx, O, nb):
     """One adaptation step from a non-uniform grid equals the oracle's refinement of the oracle's
-    histogram of the same events."""
+    histogram of the same events: 64 bins (one wave per dimension), 50 (a partial wave) and 256
+    (the maximum, four waves: the block scan crosses waves)."""
     cfg = O.hadcfg_default()
     rng = np.random.default_rng(2)
-    e0 = np.sort(rng.random((3, 63)), axis=1)
+    e0 = np.sort(rng.random((3, nb - 1)), axis=1)
     e0 = np.concatenate([np.zeros((3, 1)), e0, np.ones((3, 1))], axis=1)
     ctx.vegas_set_grid(e0)
     ctx.vegas_adapt(mi.make_run(mi.physics_default(1), 1 << 16, key=11), 1, 1.5)
@@ -50,8 +52,9 @@ def test_vegas_refine_kernel_matches_oracle_refine(ctx, O):
     import ctypes as C
     e = e0.copy()
     sums = np.zeros(2)
-    O.lib().orc32_vegas_adapt(C.byref(cfg), 64, 1.5, 1, 11, 0, 1 << 16, O._dp(e), O._dp(sums))
+    O.lib().orc32_vegas_adapt(C.byref(cfg), nb, 1.5, 1, 11, 0, 1 << 16, O._dp(e), O._dp(sums))
     assert np.max(np.abs(g - e)) < 1e-10
+    assert np.all(np.diff(g, axis=1) > 0) and np.all(g[:, 0] == 0) and np.all(g[:, -1] == 1)
 
 
 def test_vegas_c3_workload(ctx, O):
