@@ -160,7 +160,8 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
     // maximum is an integer max of the bit patterns.  Converted to FP64 once at the end, which is
     // exact and order-preserving.  The nonzero count is an integer.
     using MaxT = typename std::conditional<sizeof(T) == 4, uint32_t, double>::type;
-    constexpr int NP = HEAVY ? kMaxProc : kLightProc;
+    // VEGAS adaptation (HIST) reads only the totals of its records: the per-process slots stay 0
+    constexpr int NP = HIST ? 0 : (HEAVY ? kMaxProc : kLightProc);
     MaxT mx[NP + 1];
     double sw[NP + 1], sw2[NP + 1];                                          // sum w, sum w^2 per slot
 #pragma unroll
