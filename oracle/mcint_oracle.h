@@ -226,8 +226,11 @@ void orc32_had_points(const orc_hadcfg *c, int nbins, const double *edges, uint6
    code and total weight per candidate; returns the accepted count. */
 uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uint64_t key, uint64_t ctr0,
                         uint64_t n, double wmax, uint8_t *flags, int32_t *pid, float *w_out);
-/* Lepage refinement of ndim x (nbins + 1) edges from per-bin sums of w^2.
-   Pinned: alpha = 0 and a flat histogram leave the grid unchanged; separable test functions. */
+/* Lepage refinement of ndim x (nbins + 1) edges from per-bin sums of w^2 (oracle_vegas_refine.c).
+   Pinned (tests/test_oracle_vegas.py): alpha = 0 and a flat histogram leave a random grid
+   unchanged; a 4-bin example worked by hand; the growth factor of a small alternating
+   perturbation of the bin widths against its linearisation; convergence toward the closed-form
+   grid of separable integrands in each of 3 dimensions; planted mistakes fail these pins. */
 void orc_vegas_refine(int ndim, int nbins, double alpha, const double *hist, double *edges);
 void orc32_vegas_adapt(const orc_hadcfg *c, int nbins, double alpha, int n_iter, uint64_t key, uint64_t ctr0,
                        uint64_t n, double *edges, double *sums_out);

@@ -1,12 +1,12 @@
 /*
  * oracle_vegas.c -- CPU oracle: batched per-point evaluation through the flat or VEGAS map,
- * VEGAS refinement, and the FP32 unweighting decisions.  TEST INFRASTRUCTURE ONLY (see
+ * VEGAS adaptation loops, and the FP32 unweighting decisions.  TEST INFRASTRUCTURE ONLY (see
  * mcint_oracle.h).
  *
- * VEGAS: G. P. Lepage, J. Comput. Phys. 27 (1978) 192 -- per-bin sum of (f J)^2, neighbour
- * smoothing, damping ((1 - r)/(-ln r))^alpha, equal-share re-partition with linear
- * interpolation inside the old bins.  The reference's 1-D variant (cuPythia/11_vegas_importance/
- * vegas.cu:69-85: first moment, 3-point smoothing, no damping) is restated too, for its KAT.
+ * VEGAS: the adaptation loops sample through the grid, accumulate per-bin sums of (f J)^2 and
+ * call the Lepage refinement of oracle_vegas_refine.c.  The reference's 1-D variant
+ * (cuPythia/11_vegas_importance/vegas.cu:40-124: first moment, 3-point smoothing, no damping) is
+ * restated here too, for its KAT.
  * Unweighting: accept iff u * wMax < w with u from a disjoint stream
  * (cuPythia/pipeline/generate.cu:56-62; Pythia ProcessContainer.cc:437).
  */
@@ -179,50 +179,6 @@ uint64_t orc32_unweight(const orc_hadcfg *c, int nbins, const double *edges, uin
     free(tab);
     if (gp) orc32_grid_free(&g32);
     return acc;
-}
-
-/* ---------------------------------------------------------------- VEGAS refinement */
-void orc_vegas_refine(int ndim, int nbins, double alpha, const double *hist, double *edges)
-{
-    double *m = (double *)malloc(sizeof(double) * nbins);
-    double *ne = (double *)malloc(sizeof(double) * (nbins + 1));
-    for (int d = 0; d < ndim; ++d) {
-        const double *h = hist + (size_t)d * nbins;
-        double *x = edges + (size_t)d * (nbins + 1);
-        double tot = 0.0;
-        for (int i = 0; i < nbins; ++i) {
-            double s;
-            if (nbins == 1) s = h[0];
-            else if (i == 0) s = 0.5 * (h[0] + h[1]);
-            else if (i == nbins - 1) s = 0.5 * (h[nbins - 2] + h[nbins - 1]);
-            else s = (h[i - 1] + h[i] + h[i + 1]) / 3.0;
-            m[i] = s;
-            tot += s;
-        }
-        if (!(tot > 0.0)) continue;
-        double msum = 0.0;
-        for (int i = 0; i < nbins; ++i) {
-            double r = m[i] / tot, mi = 0.0;
-            if (r > 0.0) mi = (r < 1.0) ? pow((1.0 - r) / -log(r), alpha) : 1.0;
-            m[i] = mi;
-            msum += mi;
-        }
-        if (!(msum > 0.0)) continue;
-        double step = msum / nbins, acc = 0.0;
-        int old = 0;
-        ne[0] = x[0];
-        ne[nbins] = x[nbins];
-        for (int j = 1; j < nbins; ++j) {
-            double target = step * j;
-            while (old < nbins - 1 && acc + m[old] < target) { acc += m[old]; ++old; }
-            double frac = (m[old] > 0.0) ? (target - acc) / m[old] : 0.0;
-            frac = fmin(fmax(frac, 0.0), 1.0);
-            ne[j] = x[old] + frac * (x[old + 1] - x[old]);
-        }
-        for (int j = 0; j <= nbins; ++j) x[j] = ne[j];
-    }
-    free(m);
-    free(ne);
 }
 
 /* FP32-mirror VEGAS adaptation of the hadronic integrand: iteration it samples events
