@@ -48,6 +48,30 @@ def traffic_from_profile():
     return None
 
 
+def host_cpu():
+    """The host's CPU model (lscpu's "Model name", read from /proc/cpuinfo), its logical CPU count
+    (nproc), the CPUs this process may run on, and the threads the baseline uses: every CPU the
+    process may use, capped by OMP_NUM_THREADS where the environment sets it (the GPU box gives
+    each GPU's jobs a share of the host and sets it to that share)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
+    return {"model": model, "nproc": nproc, "affinity_cpus": avail, "omp_num_threads": omp or None,
+            "threads": threads}
+
+
 def cpu_baseline(n_sample, threads):
     import ctypes as C
     from oracle import oracle as O
@@ -61,70 +85,36 @@ def cpu_baseline(n_sample, threads):
     return n_sample / dt, dt, s[0] / n_sample
 
 
-def time_to_solution(ctx, mi, target=1e-3, flow=None):
-    """GPU time (sum of device-side kernel times of every launch) until sigma_err/sigma <= target:
-    VEGAS adaptation (+ flow training) + a 2^20-point pilot + the final pass sized for the target.
+SOLVE_DECISIONS = {0: "plain VEGAS", 1: "fallback to VEGAS before training",
+                   2: "fallback to VEGAS after training", 3: "flow", 4: "flow (forced)"}
 
-    flow=None: plain VEGAS.  flow="fixed": 200 training steps on 2^16-point batches.
-    flow="adaptive": training in chunks of 10 steps, stopped when a chunk costs more GPU time than
-    it saves on the final pass, judged from the chunk's last-batch CV and the flow's cost per
-    point measured on the pilot of the untrained (identity) flow."""
+
+def time_to_solution(ctx, mi, sampler, target=1e-3, **opts):
+    """GPU time (device time of every pass) until sigma_err/sigma <= target: the VEGAS adaptation
+    (3-D, 64 bins, 8 x 2^20 points, alpha 1.5), then mi_solve -- a 2^20-point VEGAS pilot, for the
+    flow arms the identity-flow probe, training and the flow's pilot, and the final pass sized for
+    the target."""
     phys = mi.physics_default(1)
     t0 = time.perf_counter()
     ctx.vegas_init(3, 64)
     info = ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
-    gpu_ms = info.kernel_ms
-    sampler = mi.MI_SAMPLER_VEGAS
-    extra = {"adapt_ms": info.kernel_ms}
-    pilot_n = 1 << 20
-    if flow == "fixed":
-        ctx.flow_init(7)
-        fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), 200, 1 << 16, 2e-3)
-        gpu_ms += fi.kernel_ms
-        sampler = mi.MI_SAMPLER_FLOW
-        extra.update(train_steps=200, train_ms=fi.kernel_ms, cv_first_batch=fi.cv_vegas, cv_last_batch=fi.cv_flow)
-    elif flow == "adaptive":
-        ctx.flow_init(7)
-        sampler = mi.MI_SAMPLER_FLOW
-        probe = ctx.integrate(mi.make_run(phys, pilot_n, key=46, sampler=sampler))   # identity flow
-        gpu_ms += probe.kernel_ms
-        ms_per_point = probe.kernel_ms / pilot_n
-        cv = probe.sigmaErr() / probe.sigmaGen() * math.sqrt(pilot_n)
-        final_ms = (cv / target) ** 2 * ms_per_point
-        steps, train_ms, chunk, batch = 0, 0.0, 10, 1 << 16
-        while steps < 200:
-            fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43, counter0=steps * batch), chunk, batch, 2e-3)
-            steps += chunk
-            train_ms += fi.kernel_ms
-            new_final = (fi.cv_flow / target) ** 2 * ms_per_point
-            saved = final_ms - new_final
-            final_ms = min(final_ms, new_final)
-            if saved < fi.kernel_ms:
-                break
-        gpu_ms += train_ms
-        extra.update(train_steps=steps, train_ms=train_ms, identity_pilot_ms=probe.kernel_ms,
-                     flow_ms_per_point=ms_per_point)
-    # pilot for the variance, then the final pass sized for the target
-    pilot = ctx.integrate(mi.make_run(phys, pilot_n, key=44, sampler=sampler))
-    gpu_ms += pilot.kernel_ms
-    rel = pilot.sigmaErr() / pilot.sigmaGen()
-    n = int(math.ceil(pilot_n * (rel / target) ** 2 * 1.1))
-    n = max(n, 1 << 16)
-    fin = ctx.integrate(mi.make_run(phys, n, key=45, sampler=sampler))
-    gpu_ms += fin.kernel_ms
+    est, si = ctx.solve(mi.make_run(phys, 1, key=44), target, sampler, **opts)
     wall = time.perf_counter() - t0
-    return {"gpu_ms": gpu_ms, "wall_s": wall, "points_final": n, "sigma": fin.sigmaGen(),
-            "rel_err": fin.sigmaErr() / fin.sigmaGen(), "pilot_rel_err": rel, "final_ms": fin.kernel_ms, **extra}
+    d = si.as_dict()
+    d["solve_gpu_ms"] = d.pop("gpu_ms")
+    return {**d, "gpu_ms": info.kernel_ms + si.gpu_ms, "adapt_ms": info.kernel_ms, "wall_s": wall,
+            "sigma": est.sigmaGen(), "rel_err": est.sigmaErr() / est.sigmaGen(),
+            "decision_name": SOLVE_DECISIONS.get(si.decision, str(si.decision))}
 
 
 def tts_all(ctx, mi):
-    tv = time_to_solution(ctx, mi)
-    out = {"vegas": tv}
-    for mode in ("fixed", "adaptive"):
-        tf = time_to_solution(ctx, mi, flow=mode)
-        out[f"flow_vegas_{mode}"] = tf
-        out[f"speedup_{mode}"] = tv["gpu_ms"] / tf["gpu_ms"]
-    return out
+    """Plain VEGAS; flow + VEGAS with plain VEGAS as the fallback (mi_solve's own decisions); and
+    flow + VEGAS forced to train 200 steps on 2^16-point batches and finish with the flow."""
+    tv = time_to_solution(ctx, mi, mi.MI_SAMPLER_VEGAS)
+    ta = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, flow_seed=7)
+    tf = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, flow_seed=7, force_flow=1, max_steps=200)
+    return {"target_rel_err": 1e-3, "vegas": tv, "flow_vegas_fallback": ta, "flow_vegas_forced": tf,
+            "speedup_fallback": tv["gpu_ms"] / ta["gpu_ms"], "speedup_forced": tv["gpu_ms"] / tf["gpu_ms"]}
 
 
 # Flow: per layer the MLP 2 -> 64 -> 64 -> 49 is 2*64 + 64*64 + 64*49 = 7360 multiply-adds per
@@ -339,7 +329,7 @@ def main():
                                "FP32 ME + FP64 sums, Philox4x32-10 key 42",
                    "points_per_step": n, "precision": "fp32 ME / fp64 accumulation",
                    "sigma_mb": est.sigmaGen(), "sigma_err_mb": est.sigmaErr()},
-        "roofline": {"bound": "mfma", "engine": "fp32 vector ALU (compute roof; the FP32 matrix and vector peaks are both 157.3)",
+        "roofline": {"bound": "valu", "engine": "fp32 vector ALU (MI355X FP32 vector peak 157.3 TFLOP/s)",
                      "kernel": "k_integrate_had<float,flat>", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(), "flops_per_point": FLOPS_PER_POINT,
@@ -351,9 +341,13 @@ def main():
                                "sample": f"{args.cpu_sample} points of the same workload through the FP64 "
                                          f"Pythia-form oracle (flux lists, literal sigmaHat), flat map, "
                                          f"{dt1:.1f} s on one host core"}
-        nthr = min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        nthr = host["threads"]
         rateN, dtN, _ = cpu_baseline(args.cpu_sample * 4, nthr)
-        out["cpu_baseline"]["all_cores"] = {"value": rateN, "cores": nthr, "seconds": dtN}
+        out["cpu_baseline"]["host"] = host
+        out["cpu_baseline"]["all_cores"] = {"value": rateN, "cores": nthr, "seconds": dtN,
+                                            "sample": f"{args.cpu_sample * 4} points, OpenMP over disjoint "
+                                                      f"counter ranges on {nthr} threads"}
     if not args.no_workloads:
         out["workloads"] = workloads(ctx, mi)
     if not args.no_tts:

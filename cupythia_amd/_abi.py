@@ -94,6 +94,25 @@ class MiFlowInfo(C.Structure):
                 ("kernel_ms", D)]
 
 
+class MiSolveOpts(C.Structure):
+    _fields_ = [("rel_target", D), ("sampler", I), ("chunk_steps", I), ("max_steps", I), ("force_flow", I),
+                ("pilot_points", U64), ("probe_points", U64), ("batch", U64), ("min_points", U64), ("lr", D),
+                ("flow_seed", U64)]
+
+
+MI_SOLVE_VEGAS, MI_SOLVE_FALLBACK_PROBE, MI_SOLVE_FALLBACK_TRAINED, MI_SOLVE_FLOW, MI_SOLVE_FLOW_FORCED = range(5)
+
+
+class MiSolveInfo(C.Structure):
+    _fields_ = [("sampler_used", I), ("decision", I), ("train_steps", I), ("reserved", I), ("gpu_ms", D),
+                ("pilot_ms", D), ("probe_ms", D), ("train_ms", D), ("flow_pilot_ms", D), ("final_ms", D),
+                ("vegas_ms_per_point", D), ("flow_ms_per_point", D), ("cv_vegas", D), ("cv_flow", D),
+                ("vegas_final_ms_proj", D), ("flow_rest_ms_proj", D), ("final_points", U64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 VP = C.c_void_p
 P = C.POINTER
 
@@ -121,6 +140,8 @@ PROTOTYPES = {
     "mi_pdf_load": ([VP, C.c_char_p], I),
     "mi_pdf_load_slot": ([VP, I, C.c_char_p], I),
     "mi_reweight": ([VP, P(MiPhysics), U64, VP, VP, VP, VP, VP, VP, I, P(MiWeightVar), VP], I),
+    "mi_reweight_events": ([VP, P(MiRun), U64, VP, VP, VP, I, P(MiWeightVar), VP], I),
+    "mi_solve": ([VP, P(MiRun), P(MiSolveOpts), P(MiSigmaEst), P(MiSolveInfo)], I),
     "mi_pdf_xf": ([VP, U64, VP, VP, VP], I),
     "mi_flow_init": ([VP, U64], I),
     "mi_flow_train": ([VP, P(MiRun), I, U64, D, P(MiFlowInfo)], I),

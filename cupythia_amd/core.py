@@ -125,13 +125,32 @@ class EventRecord:
 
 
 def _ptr(t: Optional[torch.Tensor]):
+    """Device pointer of a tensor already checked by _arr (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _arr(t: Optional[torch.Tensor], name: str, dtype: torch.dtype, shape, device: int, optional: bool = False):
+    """Checks a bulk array against what the C ABI reads or writes through its pointer -- dtype,
+    exact shape (None entries are free), contiguity, a CUDA tensor on the context's device -- and
+    raises ValueError (TypeError for a non-tensor) before anything is launched."""
     if t is None:
-        return None
-    if not t.is_cuda:
-        raise ValueError("device tensors required (bulk arrays cross the ABI as device pointers)")
+        if optional:
+            return None
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if len(shape) != t.dim() or any(s is not None and s != d for s, d in zip(shape, t.shape)):
+        want = "(" + ", ".join("*" if s is None else str(s) for s in shape) + ")"
+        raise ValueError(f"{name} must have shape {want}, got {tuple(t.shape)}")
     if not t.is_contiguous():
-        raise ValueError("contiguous tensors required")
-    return C.c_void_p(t.data_ptr())
+        raise ValueError(f"{name} must be contiguous")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor (bulk arrays cross the ABI as device pointers)")
+    if t.device.index != device:
+        raise ValueError(f"{name} is on cuda:{t.device.index}, the context on cuda:{device}")
+    return t
 
 
 class Context:
@@ -189,7 +208,13 @@ class Context:
     # -------------------------------------------------------------- sigmaHat
     def sigma_hat(self, code: int, sH, tH, alpS, uH=None, id1=None, id2=None, nQuarkNew: int = 3,
                   kfactor: float = 1.0) -> torch.Tensor:
-        n = sH.numel()
+        f64, i32, dv = torch.float64, torch.int32, self.device
+        n = _arr(sH, "sH", f64, (None,), dv).shape[0]
+        _arr(tH, "tH", f64, (n,), dv)
+        _arr(alpS, "alpS", f64, (n,), dv)
+        _arr(uH, "uH", f64, (n,), dv, optional=True)
+        _arr(id1, "id1", i32, (n,), dv, optional=True)
+        _arr(id2, "id2", i32, (n,), dv, optional=True)
         out = torch.empty(n, dtype=torch.float64, device=self.dev)
         self.sync_stream()
         self._check(self.lib.mi_sigma_hat(self.h, code, n, _ptr(sH), _ptr(tH), _ptr(uH), _ptr(alpS),
@@ -262,10 +287,19 @@ class Context:
                     wused.value, SigmaEstimate(est))
         return ev
 
+    def _events_raw(self, ev: "Events") -> A.MiEvents:
+        """The mi_events view of ev after checking each array (length ev.n, the SoA dtypes)."""
+        f64, dv, n = torch.float64, self.device, int(ev.n)
+        for name in ("sHat", "tHat", "x1", "x2", "weight"):
+            _arr(getattr(ev, name), name, f64, (n,), dv)
+        _arr(ev.pid, "pid", torch.int32, (n,), dv)
+        _arr(ev.index, "index", torch.int64, (n,), dv)
+        return A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
+                          _ptr(ev.index), n)
+
     def write_lhe(self, path: str, phys: A.MiPhysics, ev: "Events", key: int) -> None:
         """LHEF 3.0 file of compacted events (flavours, colour flow, lab momenta; pb)."""
-        raw = A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
-                         _ptr(ev.index), ev.n)
+        raw = self._events_raw(ev)
         est = ev.estimate.raw if ev.estimate is not None else A.MiSigmaEst()
         self.sync_stream()
         self._check(self.lib.mi_write_lhe(self.h, path.encode(), C.byref(phys), C.byref(raw), ev.n, key,
@@ -274,13 +308,12 @@ class Context:
     def build_events(self, phys: A.MiPhysics, ev: "Events", key: int, max_part: int = 4) -> EventRecord:
         """The partons of compacted events on the device: flavours, colour flow, lab four-momenta
         (Philox streams 3 and 4 of each event's index under key)."""
-        n, d = ev.n, self.dev
+        raw = self._events_raw(ev)
+        n, d = int(ev.n), self.dev
         f64, i32 = dict(dtype=torch.float64, device=d), dict(dtype=torch.int32, device=d)
         shp = (n, max_part)
         r = EventRecord(*[torch.zeros(shp, **f64) for _ in range(5)], *[torch.zeros(shp, **i32) for _ in range(8)],
                         torch.zeros(n, **i32), torch.zeros(n, **f64))
-        raw = A.MiEvents(_ptr(ev.sHat), _ptr(ev.tHat), _ptr(ev.x1), _ptr(ev.x2), _ptr(ev.weight), _ptr(ev.pid),
-                         _ptr(ev.index), n)
         out = A.MiParticles(_ptr(r.px), _ptr(r.py), _ptr(r.pz), _ptr(r.e), _ptr(r.m), _ptr(r.pdg), _ptr(r.status),
                             _ptr(r.col), _ptr(r.acol), _ptr(r.mo1), _ptr(r.mo2), _ptr(r.d1), _ptr(r.d2),
                             _ptr(r.n_part), _ptr(r.scale), n, max_part, 0)
@@ -290,30 +323,64 @@ class Context:
         return r
 
     # -------------------------------------------------------------- reweighting
+    @staticmethod
+    def _variations(variations):
+        vs = [tuple(v) for v in variations]
+        if not 1 <= len(vs) <= 64:
+            raise ValueError("1..64 variations")
+        return (A.MiWeightVar * len(vs))(*[A.MiWeightVar(float(r), float(f), int(sl)) for (r, f, sl) in vs]), len(vs)
+
     def reweight_scales(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, w: torch.Tensor,
                         factors, pid: Optional[torch.Tensor] = None) -> torch.Tensor:
-        """Weights at renormalisation-scale factors (n x nvar), one pass over the events; pid (the
+        """Weights at renormalisation-scale factors (n x nvar), one pass over stored events; pid (the
         events' process codes, int32) gives the heavy pairs their scale pT^2 + m^2."""
+        f64, dv = torch.float64, self.device
+        n = _arr(sHat, "sHat", f64, (None,), dv).shape[0]
+        _arr(tHat, "tHat", f64, (n,), dv)
+        _arr(w, "w", f64, (n,), dv)
+        _arr(pid, "pid", torch.int32, (n,), dv, optional=True)
+        factors = [float(f) for f in factors]
+        if not 1 <= len(factors) <= 64:
+            raise ValueError("1..64 scale factors")
         fac = (C.c_double * len(factors))(*factors)
-        n = sHat.numel()
         out = torch.empty((n, len(factors)), dtype=torch.float64, device=self.dev)
         self.sync_stream()
-        self._check(self.lib.mi_reweight_scales(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat),
-                                                _ptr(pid) if pid is not None else None, _ptr(w),
-                                                len(factors), fac, _ptr(out)), "mi_reweight_scales")
+        self._check(self.lib.mi_reweight_scales(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat), _ptr(pid),
+                                                _ptr(w), len(factors), fac, _ptr(out)), "mi_reweight_scales")
         return out
 
     def reweight(self, phys: A.MiPhysics, sHat: torch.Tensor, tHat: torch.Tensor, x1: torch.Tensor,
                  x2: torch.Tensor, pid: torch.Tensor, w: torch.Tensor, variations) -> torch.Tensor:
-        """Weights under (renorm_fac, fact_fac, pdf_slot) variations (n x nvar): each event's own
-        process weight recomputed at the varied scales / PDFs over the run's (mi_reweight)."""
-        vs = (A.MiWeightVar * len(variations))(*[A.MiWeightVar(float(r), float(f), int(sl))
-                                                 for (r, f, sl) in variations])
-        n = sHat.numel()
-        out = torch.empty((n, len(variations)), dtype=torch.float64, device=self.dev)
+        """Weights under (renorm_fac, fact_fac, pdf_slot) variations (n x nvar) of stored events: each
+        event's own process weight recomputed from its (sHat, tHat, x1, x2) at the varied scales /
+        PDFs over the run's (mi_reweight)."""
+        f64, dv = torch.float64, self.device
+        n = _arr(sHat, "sHat", f64, (None,), dv).shape[0]
+        for name, t in (("tHat", tHat), ("x1", x1), ("x2", x2), ("w", w)):
+            _arr(t, name, f64, (n,), dv)
+        _arr(pid, "pid", torch.int32, (n,), dv)
+        vs, nvar = self._variations(variations)
+        out = torch.empty((n, nvar), dtype=torch.float64, device=self.dev)
         self.sync_stream()
         self._check(self.lib.mi_reweight(self.h, C.byref(phys), n, _ptr(sHat), _ptr(tHat), _ptr(x1), _ptr(x2),
-                                         _ptr(pid), _ptr(w), len(variations), vs, _ptr(out)), "mi_reweight")
+                                         _ptr(pid), _ptr(w), nvar, vs, _ptr(out)), "mi_reweight")
+        return out
+
+    def reweight_events(self, run: A.MiRun, index: torch.Tensor, pid: torch.Tensor, variations,
+                        w: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Weights under (renorm_fac, fact_fac, pdf_slot) variations (n x nvar) of events regenerated
+        from their counters under run (mi_reweight_events): each event's own process weight in FP64
+        at each variation -- w * W_k / W_0 with w given (e.g. unweighted events), else W_k itself,
+        bit for bit the per-point weight of a run at those scales."""
+        dv = self.device
+        n = _arr(index, "index", torch.int64, (None,), dv).shape[0]
+        _arr(pid, "pid", torch.int32, (n,), dv)
+        _arr(w, "w", torch.float64, (n,), dv, optional=True)
+        vs, nvar = self._variations(variations)
+        out = torch.empty((n, nvar), dtype=torch.float64, device=self.dev)
+        self.sync_stream()
+        self._check(self.lib.mi_reweight_events(self.h, C.byref(run), n, _ptr(index), _ptr(pid), _ptr(w), nvar, vs,
+                                                _ptr(out)), "mi_reweight_events")
         return out
 
     # -------------------------------------------------------------- PDF grids
@@ -325,12 +392,32 @@ class Context:
 
     def pdf_xf(self, x: torch.Tensor, Q2: torch.Tensor) -> torch.Tensor:
         """Raw interpolated xf (n x 12 FP64: g, d, u, s, c, b, dbar, ubar, sbar, cbar, bbar, 0)."""
-        x = x.to(device=self.dev, dtype=torch.float64).contiguous()
-        Q2 = Q2.to(device=self.dev, dtype=torch.float64).contiguous()
+        x = torch.as_tensor(x).to(device=self.dev, dtype=torch.float64).contiguous().reshape(-1)
+        Q2 = torch.as_tensor(Q2).to(device=self.dev, dtype=torch.float64).contiguous().reshape(-1)
+        if Q2.numel() != x.numel():
+            raise ValueError(f"x has {x.numel()} values, Q2 {Q2.numel()}")
         out = torch.empty((x.numel(), 12), dtype=torch.float64, device=self.dev)
         self.sync_stream()
         self._check(self.lib.mi_pdf_xf(self.h, x.numel(), _ptr(x), _ptr(Q2), _ptr(out)), "mi_pdf_xf")
         return out
+
+    # -------------------------------------------------------------- time to precision
+    def solve(self, run: A.MiRun, rel_target: float, sampler: int = A.MI_SAMPLER_VEGAS, **opts):
+        """sigma to a relative error (mi_solve): plain VEGAS, or the flow on top of VEGAS with plain
+        VEGAS as the fallback.  opts: chunk_steps, max_steps, force_flow, pilot_points,
+        probe_points, batch, min_points, lr, flow_seed (0 = the library's defaults).  Returns the
+        final pass's SigmaEstimate and the MiSolveInfo of the decisions and costs."""
+        o = A.MiSolveOpts()
+        o.rel_target = float(rel_target)
+        o.sampler = int(sampler)
+        for k, v in opts.items():
+            if k not in dict(A.MiSolveOpts._fields_):
+                raise ValueError(f"unknown solve option {k}")
+            setattr(o, k, v)
+        est, info = A.MiSigmaEst(), A.MiSolveInfo()
+        self.sync_stream()
+        self._check(self.lib.mi_solve(self.h, C.byref(run), C.byref(o), C.byref(est), C.byref(info)), "mi_solve")
+        return SigmaEstimate(est), info
 
     # -------------------------------------------------------------- adjoint
     def adjoint(self, run: A.MiRun) -> A.MiSigmaGrad:
@@ -349,8 +436,12 @@ class Context:
         self._check(self.lib.mi_flow_train(self.h, C.byref(run), steps, batch, lr, C.byref(info)), "mi_flow_train")
         return info
 
+    def _coords(self, y: torch.Tensor, name: str) -> int:
+        """n of an (n, 3) float32 coordinate array on the context's device."""
+        return _arr(y, name, torch.float32, (None, 3), self.device).shape[0]
+
     def flow_forward(self, y: torch.Tensor):
-        n = y.shape[0]
+        n = self._coords(y, "y")
         z = torch.empty_like(y)
         ld = torch.empty(n, dtype=torch.float32, device=self.dev)
         self.sync_stream()
@@ -358,7 +449,7 @@ class Context:
         return z, ld
 
     def flow_inverse(self, z: torch.Tensor):
-        n = z.shape[0]
+        n = self._coords(z, "z")
         y = torch.empty_like(z)
         ld = torch.empty(n, dtype=torch.float32, device=self.dev)
         self.sync_stream()
@@ -366,7 +457,8 @@ class Context:
         return y, ld
 
     def flow_backward(self, y: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
-        n = y.shape[0]
+        n = self._coords(y, "y")
+        _arr(g, "g", torch.float32, (n,), self.device)
         grad = torch.empty(A.MI_FLOW_NPARAMS, dtype=torch.float32, device=self.dev)
         self.sync_stream()
         self._check(self.lib.mi_flow_backward(self.h, n, _ptr(y), _ptr(g), _ptr(grad)), "mi_flow_backward")
@@ -382,6 +474,7 @@ class Context:
     def flow_set_params(self, params):
         import numpy as np
         p = np.ascontiguousarray(params, dtype=np.float32)
-        assert p.size == A.MI_FLOW_NPARAMS
+        if p.size != A.MI_FLOW_NPARAMS:
+            raise ValueError(f"{A.MI_FLOW_NPARAMS} parameters expected, got {p.size}")
         self._check(self.lib.mi_flow_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
                     "mi_flow_set_params")

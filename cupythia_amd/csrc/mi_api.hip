@@ -817,4 +817,53 @@ int mi_reweight(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *s
     return MI_OK;
 }
 
+int mi_reweight_events(mi_ctx *ctx, const mi_run *run, uint64_t n, const uint64_t *index_dev, const int32_t *pid_dev,
+                       const double *w_dev, int nvar, const mi_weight_var *vars_host, double *out_dev)
+{
+    int rc = check_run(ctx, run);
+    if (rc) return rc;
+    const mi_physics &p = run->phys;
+    if (p.mode != MI_MODE_HADRONIC_3D) return fail(ctx, MI_ERR_ARG, "mi_reweight_events needs the hadronic mode");
+    if (run->sampler == MI_SAMPLER_FLOW) return fail(ctx, MI_ERR_ARG, "mi_reweight_events regenerates the flat and VEGAS maps");
+    if (run->rng != MI_RNG_PHILOX) return fail(ctx, MI_ERR_ARG, "mi_reweight_events regenerates the Philox stream");
+    if (nvar < 1 || nvar > 64) return fail(ctx, MI_ERR_RANGE, "nvar must be 1..64");
+    if (!vars_host) return fail(ctx, MI_ERR_ARG, "vars is NULL");
+    if (n && (!index_dev || !pid_dev || !out_dev)) return fail(ctx, MI_ERR_ARG, "NULL array");
+    const bool grid = p.pdf_source == MI_PDF_GRID;
+    std::vector<RwRegen> vars((size_t)nvar + 1);
+    for (int v = 0; v <= nvar; ++v) {
+        mi_physics q = p;
+        int slot = 0;
+        if (v > 0) {
+            const mi_weight_var &wv = vars_host[v - 1];
+            if (!(wv.renorm_fac > 0.0) || !(wv.fact_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "scale factors must be > 0");
+            if (wv.pdf_slot < 0 || wv.pdf_slot >= MI_PDF_SLOTS) return fail(ctx, MI_ERR_ARG, "pdf_slot out of range");
+            if (!grid && wv.pdf_slot != 0)
+                return fail(ctx, MI_ERR_ARG, "PDF slots other than 0 need pdf_source = MI_PDF_GRID");
+            q.renorm_fac = wv.renorm_fac;
+            q.fact_fac = wv.fact_fac;
+            slot = wv.pdf_slot;
+        }
+        fill_had_consts_f64(q, vars[(size_t)v].k);
+        vars[(size_t)v].g = GridPdf<double>{};
+        if (grid) {
+            rc = pdf_slot_view(ctx, slot, &vars[(size_t)v].g);
+            if (rc) return rc;
+            vars[(size_t)v].g.factFac2 = q.fact_fac * q.fact_fac;
+        }
+    }
+    if (n == 0) return MI_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(RwRegen) * vars.size();
+    rc = ensure(ctx, &ctx->hist, &ctx->hist_cap, (bytes + sizeof(double) - 1) / sizeof(double));
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hist, vars.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    const bool vegas = run->sampler == MI_SAMPLER_VEGAS;
+    HIPCHK(ctx, launch_reweight_regen(ctx->stream, reinterpret_cast<const RwRegen *>(ctx->hist), nvar, p.proc_mask,
+                                      vegas, run->key, n, index_dev, pid_dev, w_dev,
+                                      vegas ? ctx->grid.tab64 : nullptr, vegas ? ctx->grid.nbins : 0, out_dev));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MI_OK;
+}
+
 }  // extern "C"

@@ -202,4 +202,12 @@ hipError_t launch_build_events(hipStream_t s, const EvConst &c, const GridPdf<do
 hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n, const double *sH, const double *tH,
                            const int32_t *pid, const double *w, int nvar, const double *fac, double *out);
 
+// Variations of events regenerated from their counters (mi_kernels.hip): per variation the run's
+// FP64 constants at the varied scales and the FP64 view of its PDF slot (active = 0: toy PDFs).
+// vars_dev holds nvar + 1 entries, [0] = the run's own.
+struct RwRegen { HadConst<double> k; GridPdf<double> g; };
+hipError_t launch_reweight_regen(hipStream_t s, const RwRegen *vars_dev, int nvar, int mask, bool vegas,
+                                 uint64_t key, uint64_t n, const uint64_t *idx, const int32_t *pid, const double *w,
+                                 const double *gtab, int nbins, double *out);
+
 }  // namespace mi

@@ -772,4 +772,65 @@ hipError_t launch_reweight(hipStream_t s, const HadConst<double> &k, uint64_t n,
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------ regenerated variations
+// The reference's pinned-seed reweighting (cuPythia/pipeline/reweight.cu:36-67: each event's weights
+// recomputed from its own counter seed): event i is the point of counter idx[i] (Philox stream 0
+// under key, the flat or VEGAS map), evaluated in FP64 at every variation v with that variation's
+// physics constants and PDF view -- the same device function as the FP64 per-point dump, so W_v is
+// bit for bit the dump of a run at those scales.  W_v is the weight of the event's own process
+// code.  vars[0] holds the run's own constants (W_0) when w is given: out = w * W_v / W_0; with w
+// null, out = W_v.  Codes outside mask (or no process) give 0.
+template <bool VEGAS, bool HEAVY>
+__global__ __launch_bounds__(kBlock) void k_reweight_regen(const RwRegen *__restrict__ vars, int nvar, int mask,
+                                                           uint64_t key, uint64_t n, const uint64_t *__restrict__ idx,
+                                                           const int32_t *__restrict__ pid,
+                                                           const double *__restrict__ w,
+                                                           const double *__restrict__ gtab, int nbins,
+                                                           double *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int code = pid[i];
+    const int slot = (code >= 111 && code <= 116) ? code - 111 : ((code >= 121 && code <= 124) ? code - 115 : -1);
+    if (slot < 0 || !((mask >> slot) & 1)) {
+        for (int v = 0; v < nvar; ++v) out[i * nvar + v] = 0.0;
+        return;
+    }
+    const uint64_t e = idx[i];
+    const int v0 = w ? 0 : 1;               // vars[0]: the run's own scales (W_0), only with w
+    double W0 = 0.0;
+    for (int v = v0; v <= nvar; ++v) {
+        const RwRegen &q = vars[v];
+        double ws[kMaxProc];
+        int bins[3];
+        had_candidate<double, VEGAS, HEAVY>(q.k, mask, key, e, gtab, nbins, (double)nbins, ws, bins, (double *)nullptr,
+                                            (double *)nullptr, (double *)nullptr, (double *)nullptr,
+                                            q.g.active ? &q.g : nullptr);
+        double Wv = 0.0;
+#pragma unroll
+        for (int p = 0; p < kMaxProc; ++p) Wv = (p == slot) ? ws[p] : Wv;   // selects: no dynamic register index
+        if (v == 0) { W0 = Wv; continue; }
+        double r = Wv;
+        if (w) r = W0 > 0.0 ? w[i] * (Wv / W0) : 0.0;
+        out[i * nvar + (v - 1)] = r;
+    }
+}
+
+hipError_t launch_reweight_regen(hipStream_t s, const RwRegen *vars_dev, int nvar, int mask, bool vegas,
+                                 uint64_t key, uint64_t n, const uint64_t *idx, const int32_t *pid, const double *w,
+                                 const double *gtab, int nbins, double *out)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((n + kBlock - 1) / kBlock)), blk(kBlock);
+    const bool hv = has_heavy(mask);
+    if (vegas) {
+        if (hv) hipLaunchKernelGGL((k_reweight_regen<true, true>), grid, blk, 0, s, vars_dev, nvar, mask, key, n, idx, pid, w, gtab, nbins, out);
+        else    hipLaunchKernelGGL((k_reweight_regen<true, false>), grid, blk, 0, s, vars_dev, nvar, mask, key, n, idx, pid, w, gtab, nbins, out);
+    } else {
+        if (hv) hipLaunchKernelGGL((k_reweight_regen<false, true>), grid, blk, 0, s, vars_dev, nvar, mask, key, n, idx, pid, w, gtab, nbins, out);
+        else    hipLaunchKernelGGL((k_reweight_regen<false, false>), grid, blk, 0, s, vars_dev, nvar, mask, key, n, idx, pid, w, gtab, nbins, out);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace mi

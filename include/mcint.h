@@ -313,6 +313,78 @@ int mi_reweight(mi_ctx *ctx, const mi_physics *phys, uint64_t n, const double *s
                 const double *x1_dev, const double *x2_dev, const int32_t *pid_dev, const double *w_dev,
                 int nvar, const mi_weight_var *vars_host, double *out_dev);
 
+/* Scale and PDF variations of events regenerated from their counters, as the reference reweights
+ * (cuPythia/pipeline/reweight.cu:36-67: every weight recomputed from the event's own counter seed,
+ * one pass equal to pinned re-runs).  Event i is the point of counter index_dev[i] under run (its
+ * key, physics and map: flat, or the context's VEGAS grid, which must still be the one the events
+ * were drawn with; run->n_points and counter0 are not used), evaluated in FP64 at each variation k:
+ * run->phys with vars[k].renorm_fac, vars[k].fact_fac and the PDFs of slot vars[k].pdf_slot (grid
+ * PDFs only; with the toy PDFs fact_fac has no effect and the slot must be 0).  W_k is the FP64
+ * weight of the event's own process pid_dev[i]: bit for bit mi_points (FP64, same map) of the varied
+ * run, column of that code.  out[i * nvar + k] = W_k when w_dev is NULL, else w_dev[i] * W_k / W_0,
+ * W_0 at run->phys itself (e.g. the weights of unweighted events).  Codes the run's proc_mask does
+ * not select (or 0) give 0.  FP64 for either run precision: with FP32 runs the ratio is that of the
+ * FP64 point at the same uniforms.  nvar <= 64. */
+int mi_reweight_events(mi_ctx *ctx, const mi_run *run, uint64_t n, const uint64_t *index_dev,
+                       const int32_t *pid_dev, const double *w_dev, int nvar, const mi_weight_var *vars_host,
+                       double *out_dev);
+
+/* ------------------------------------------------------- time to precision ---- */
+/* sigma of run's processes to the relative error rel_target, with plain VEGAS or with the flow on
+ * top of VEGAS and plain VEGAS as the fallback.  The context's VEGAS grid is used as it is (adapt it
+ * first with mi_vegas_adapt).  Passes, on consecutive counter ranges from run->counter0 (training on
+ * counters from run->counter0 + 2^41):
+ *   VEGAS pilot (pilot_points) -> the points N_V a VEGAS final pass needs and its cost;
+ *   sampler = MI_SAMPLER_FLOW: fall back at once, with no flow pass at all, if one training chunk
+ *     (chunk_steps x batch points through the flow) plus the flow's pilot and smallest final pass
+ *     would cost more than the VEGAS final pass even at VEGAS's own cost per point (a flow point
+ *     evaluates the same integrand through the same map, plus the flow); else mi_flow_init(flow_seed)
+ *     and an identity-flow probe (probe_points) for the flow's cost per point, the same test with
+ *     it; else train in chunks while the last chunk saved more than it cost on the projected flow
+ *     pilot + final pass and another chunk saving as much would end below both that projection and
+ *     VEGAS's final pass; then the flow (pilot, final pass) if its projection is below VEGAS's
+ *     final pass, otherwise VEGAS;
+ *   the final pass of max(min_points, 1.1 P (rel_pilot / rel_target)^2) points: its estimate is out.
+ * The reference has no such driver: it integrates fixed sizes (cuPythia/11_vegas_importance/
+ * vegas.cu:106-124); the flow is its roadmap item (cuPythia/README.md:75).  Zero fields of opts take
+ * the defaults in brackets. */
+typedef struct mi_solve_opts {
+    double   rel_target;     /* sigmaErr / sigmaGen of the final pass, in (0, 1) */
+    int      sampler;        /* MI_SAMPLER_VEGAS or MI_SAMPLER_FLOW (flow + VEGAS, VEGAS fallback) */
+    int      chunk_steps;    /* training steps between decisions [10] */
+    int      max_steps;      /* [200] */
+    int      force_flow;     /* 1: train max_steps and finish with the flow regardless (measurement) */
+    uint64_t pilot_points;   /* [2^20] */
+    uint64_t probe_points;   /* [2^20] */
+    uint64_t batch;          /* training batch [2^16] */
+    uint64_t min_points;     /* smallest final pass [2^16] */
+    double   lr;             /* Adam learning rate [2e-3] */
+    uint64_t flow_seed;
+} mi_solve_opts;
+
+#define MI_SOLVE_VEGAS            0   /* plain VEGAS was asked for */
+#define MI_SOLVE_FALLBACK_PROBE   1   /* flow asked for; VEGAS's final pass is cheaper than any continuation of the flow
+                                         (from VEGAS's cost per point, or the probe's: probe_ms tells which) */
+#define MI_SOLVE_FALLBACK_TRAINED 2   /* flow trained, then its projected pilot + final pass cost more than VEGAS's */
+#define MI_SOLVE_FLOW             3   /* the flow finished the solve */
+#define MI_SOLVE_FLOW_FORCED      4   /* force_flow */
+
+typedef struct mi_solve_info {
+    int      sampler_used;         /* sampler of the final pass */
+    int      decision;             /* MI_SOLVE_* */
+    int      train_steps;
+    int      reserved;
+    double   gpu_ms;               /* device time of every pass of the solve (adaptation not included) */
+    double   pilot_ms, probe_ms, train_ms, flow_pilot_ms, final_ms;
+    double   vegas_ms_per_point, flow_ms_per_point;
+    double   cv_vegas, cv_flow;    /* per-point coefficients of variation (flow: its pilot, else the last training batch) */
+    double   vegas_final_ms_proj;  /* projected VEGAS final pass */
+    double   flow_rest_ms_proj;    /* projected flow pilot + final pass after training (0 if not trained) */
+    uint64_t final_points;
+} mi_solve_info;
+
+int mi_solve(mi_ctx *ctx, const mi_run *run, const mi_solve_opts *opts, mi_sigma_est *out, mi_solve_info *info);
+
 /* ---------------------------------------------------------------- adjoint ---- */
 /* d sigma / d {alphaS(MZ), renorm_fac, gluon a, gluon b} in the same pass as sigma, on the
  * same points, FP64 accumulation.  Hadronic mode, flat sampler, any proc_mask within bits 0..9

@@ -35,10 +35,11 @@ def test_struct_layouts_match_header():
     import subprocess
     import tempfile
     from cupythia_amd import _abi as A
-    src = '#include <stdio.h>\n#include "mcint.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",' \
+    src = '#include <stdio.h>\n#include "mcint.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",' \
           'sizeof(mi_physics),sizeof(mi_run),sizeof(mi_sigma_est),sizeof(mi_vegas_info),sizeof(mi_events),' \
           'sizeof(mi_sigma_grad),sizeof(mi_flow_info),offsetof(mi_run,n_points),sizeof(mi_particles),' \
-          'offsetof(mi_particles,max_part));return 0;}'
+          'offsetof(mi_particles,max_part),sizeof(mi_solve_opts),sizeof(mi_solve_info),' \
+          'offsetof(mi_solve_info,final_points));return 0;}'
     src = src.replace('#include <stdio.h>', '#include <stdio.h>\n#include <stddef.h>')
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "t.c"), "w").write(src)
@@ -47,7 +48,8 @@ def test_struct_layouts_match_header():
         out = subprocess.run([os.path.join(d, "t")], check=True, capture_output=True, text=True).stdout.split()
     got = [C.sizeof(A.MiPhysics), C.sizeof(A.MiRun), C.sizeof(A.MiSigmaEst), C.sizeof(A.MiVegasInfo),
            C.sizeof(A.MiEvents), C.sizeof(A.MiSigmaGrad), C.sizeof(A.MiFlowInfo), A.MiRun.n_points.offset,
-           C.sizeof(A.MiParticles), A.MiParticles.max_part.offset]
+           C.sizeof(A.MiParticles), A.MiParticles.max_part.offset, C.sizeof(A.MiSolveOpts),
+           C.sizeof(A.MiSolveInfo), A.MiSolveInfo.final_points.offset]
     assert [int(x) for x in out] == got
 
 
@@ -89,3 +91,58 @@ def test_product_never_imports_the_oracle():
         for f in fs:
             if f.endswith((".py", ".hip", ".h", ".cpp")) or f == "Makefile":
                 assert not pat.search(open(os.path.join(dp, f)).read()), f
+
+
+def _bare_context():
+    """A Context whose wrappers can be called without a device: validation runs before any call
+    into the library, so a bad argument raises before anything is launched (the library handle
+    is absent, so a call that got past the checks would fail with AttributeError instead)."""
+    import cupythia_amd as mi
+    c = object.__new__(mi.Context)
+    c.device, c.lib, c.h = 0, None, None
+    return c
+
+
+@pytest.mark.parametrize("case", [
+    "flow_forward_flat", "flow_forward_f64", "flow_inverse_cols", "flow_backward_g_len", "sigma_hat_f32",
+    "sigma_hat_len", "sigma_hat_id_int64", "reweight_pid_int64", "reweight_w_f32", "reweight_scales_len",
+    "reweight_events_index_int32", "reweight_events_w_len", "events_index_dtype", "cpu_tensor", "noncontiguous",
+    "not_a_tensor", "too_many_variations"])
+def test_wrappers_reject_bad_arrays_before_any_launch(case):
+    import torch
+    import cupythia_amd as mi
+    c = _bare_context()
+    f64 = dict(dtype=torch.float64)
+    n = 8
+    a = torch.zeros(n, **f64)
+    p = mi.physics_default(1)
+    run = mi.make_run(p, n)
+    i32 = torch.zeros(n, dtype=torch.int32)
+    calls = {
+        "flow_forward_flat": lambda: c.flow_forward(torch.zeros(3 * n, dtype=torch.float32)),
+        "flow_forward_f64": lambda: c.flow_forward(torch.zeros((n, 3), **f64)),
+        "flow_inverse_cols": lambda: c.flow_inverse(torch.zeros((n, 4), dtype=torch.float32)),
+        "flow_backward_g_len": lambda: c.flow_backward(torch.zeros((n, 3), dtype=torch.float32),
+                                                       torch.zeros(n + 1, dtype=torch.float32)),
+        "sigma_hat_f32": lambda: c.sigma_hat(111, a.float(), a, a),
+        "sigma_hat_len": lambda: c.sigma_hat(111, a, a[:4], a),
+        "sigma_hat_id_int64": lambda: c.sigma_hat(114, a, a, a, id1=torch.zeros(n, dtype=torch.int64)),
+        "reweight_pid_int64": lambda: c.reweight(p, a, a, a, a, torch.zeros(n, dtype=torch.int64), a, [(1, 1, 0)]),
+        "reweight_w_f32": lambda: c.reweight(p, a, a, a, a, i32, a.float(), [(1, 1, 0)]),
+        "reweight_scales_len": lambda: c.reweight_scales(p, a, a[:3], a, [1.0]),
+        "reweight_events_index_int32": lambda: c.reweight_events(run, i32, i32, [(1, 1, 0)]),
+        "reweight_events_w_len": lambda: c.reweight_events(run, torch.zeros(n, dtype=torch.int64), i32,
+                                                           [(1, 1, 0)], w=a[:5]),
+        "events_index_dtype": lambda: c.build_events(p, mi.Events(a, a, a, a, a, i32, i32, n, 1.0), key=1),
+        "cpu_tensor": lambda: c.sigma_hat(111, a, a, a),
+        "noncontiguous": lambda: c.sigma_hat(111, torch.zeros(2 * n, **f64)[::2], a, a),
+        "not_a_tensor": lambda: c.sigma_hat(111, [1.0] * n, a, a),
+        "too_many_variations": lambda: c.reweight_events(run, torch.zeros(n, dtype=torch.int64), i32,
+                                                         [(1, 1, 0)] * 65),
+    }
+    with pytest.raises((ValueError, TypeError)) as e:
+        calls[case]()
+    if case == "cpu_tensor":
+        assert "device tensor" in str(e.value)
+    if case == "not_a_tensor":
+        assert e.type is TypeError

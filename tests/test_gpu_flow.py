@@ -88,6 +88,100 @@ def test_training_and_unbiased_integration(ctx, O):
     assert info.loss_last < info.loss_first, (info.loss_first, info.loss_last)
     est = ctx.integrate(mi.make_run(phys, 1 << 22, key=78, sampler=mi.MI_SAMPLER_FLOW))
     q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
-    assert abs(est.sigmaGen() - q) < 3.5 * est.sigmaErr(), (est.sigmaGen(), est.sigmaErr(), q)
+    assert abs(est.sigmaGen() - q) < 3.0 * est.sigmaErr(), (est.sigmaGen(), est.sigmaErr(), q)
     print(f"flow: loss {info.loss_first:.4f} -> {info.loss_last:.4f}, err VEGAS {vegas.sigmaErr():.3e} "
           f"flow {est.sigmaErr():.3e}, train {info.kernel_ms:.1f} ms")
+
+
+def test_c5_configured_size_gains_over_vegas(ctx, O):
+    """C5 at its configured sizes: the VEGAS grid adapted as C3 (3-D, 64 bins, alpha 1.5,
+    12 x 2^24), the flow initialised from it, 40 training steps on 2^20-point batches, then a
+    2^26-point integration through MI_SAMPLER_FLOW: sigma within 3 sigma_stat of the quadrature,
+    relative error below the 0.1% target, and the gain the trained network brings -- a smaller
+    coefficient of variation than plain VEGAS on the same 2^26 counters (measured: 0.31 vs 0.96)."""
+    phys = mi.physics_default(1)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 24, key=42), 12, 1.5)
+    ctx.flow_init(7)
+    info = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), 40, 1 << 20, 2e-3)
+    assert info.loss_last < 0.5 * info.loss_first, (info.loss_first, info.loss_last)
+    n = 1 << 26
+    fe = ctx.integrate(mi.make_run(phys, n, key=47, sampler=mi.MI_SAMPLER_FLOW))
+    ve = ctx.integrate(mi.make_run(phys, n, key=47, sampler=mi.MI_SAMPLER_VEGAS))
+    q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
+    for e in (fe, ve):
+        assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+        assert e.sigmaErr() / e.sigmaGen() < 1e-3
+    cv_f = fe.sigmaErr() / fe.sigmaGen() * math.sqrt(n)
+    cv_v = ve.sigmaErr() / ve.sigmaGen() * math.sqrt(n)
+    print(f"C5: cv flow {cv_f:.3f} vs VEGAS {cv_v:.3f}; flow {fe.kernel_ms:.2f} ms, VEGAS {ve.kernel_ms:.3f} ms, "
+          f"train {info.kernel_ms:.1f} ms")
+    assert cv_f < 0.5 * cv_v, (cv_f, cv_v)
+
+
+def test_solve_plain_vegas_is_the_fallback(ctx, O):
+    """mi_solve at 0.1%: plain VEGAS, and flow + VEGAS, which on this integrand falls back to VEGAS
+    before any flow pass: one training chunk (6.6e5 points) plus the flow's pilot (2^20) already
+    evaluate the integrand more often than VEGAS's whole final pass (~1e6 points), whatever the
+    flow costs on top.  The fallback runs plain VEGAS's passes on its counters: the same estimate,
+    bit for bit, at plain VEGAS's GPU time."""
+    phys = mi.physics_default(1)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
+    run = mi.make_run(phys, 1, key=44)
+    ev, iv = ctx.solve(run, 1e-3, mi.MI_SAMPLER_VEGAS)
+    ef, jf = ctx.solve(run, 1e-3, mi.MI_SAMPLER_FLOW, flow_seed=7)
+    q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
+    assert iv.decision == mi._abi.MI_SOLVE_VEGAS and iv.sampler_used == mi.MI_SAMPLER_VEGAS
+    assert jf.decision == mi._abi.MI_SOLVE_FALLBACK_PROBE and jf.sampler_used == mi.MI_SAMPLER_VEGAS
+    assert jf.train_steps == 0 and jf.probe_ms == 0.0 and jf.final_points == iv.final_points
+    assert ef.sumW() == ev.sumW() and ef.sumW2() == ev.sumW2()
+    for e in (ev, ef):
+        assert e.sigmaErr() / e.sigmaGen() < 1.1e-3
+        assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    # the same passes (device-time jitter of a few microseconds per pass)
+    assert jf.gpu_ms <= 1.2 * iv.gpu_ms + 0.01, (jf.as_dict(), iv.as_dict())
+    print(f"solve 1e-3: VEGAS {iv.gpu_ms:.4f} ms ({iv.final_points} points), flow+VEGAS {jf.gpu_ms:.4f} ms")
+
+
+def test_solve_tight_target_probes_then_decides(ctx, O):
+    """At 3e-5 VEGAS's final pass needs ~1.1e9 points, so the VEGAS-cost bound leaves room for a flow:
+    the identity flow is probed (its cost per point is above VEGAS's), and whatever the solve then
+    decides is consistent with its own projections; the estimate meets the target and agrees with
+    the quadrature within 3 sigma_stat."""
+    phys = mi.physics_default(1)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
+    e, info = ctx.solve(mi.make_run(phys, 1, key=60), 3e-5, mi.MI_SAMPLER_FLOW, flow_seed=7)
+    d = info.decision
+    assert info.probe_ms > 0 and info.flow_ms_per_point > info.vegas_ms_per_point
+    if d == mi._abi.MI_SOLVE_FALLBACK_PROBE:
+        assert info.train_steps == 0
+    elif d == mi._abi.MI_SOLVE_FALLBACK_TRAINED:
+        assert info.train_steps > 0 and info.flow_rest_ms_proj >= info.vegas_final_ms_proj
+    else:
+        assert d == mi._abi.MI_SOLVE_FLOW and info.flow_rest_ms_proj < info.vegas_final_ms_proj
+    assert (info.sampler_used == mi.MI_SAMPLER_FLOW) == (d == mi._abi.MI_SOLVE_FLOW)
+    q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
+    assert e.sigmaErr() / e.sigmaGen() < 3.3e-5
+    assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    print("solve 3e-5:", info.as_dict())
+
+
+def test_solve_forced_flow(ctx, O):
+    """force_flow: the flow trains max_steps and finishes the solve (pilot and final pass through
+    the flow), at 0.3% here; sigma within 3 sigma_stat of the quadrature."""
+    phys = mi.physics_default(1)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
+    e, info = ctx.solve(mi.make_run(phys, 1, key=90), 3e-3, mi.MI_SAMPLER_FLOW, force_flow=1, max_steps=20,
+                        flow_seed=3)
+    assert info.decision == mi._abi.MI_SOLVE_FLOW_FORCED and info.sampler_used == mi.MI_SAMPLER_FLOW
+    assert info.train_steps == 20 and info.flow_pilot_ms > 0 and info.train_ms > 0
+    q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
+    assert e.sigmaErr() / e.sigmaGen() < 3.3e-3
+    assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    with pytest.raises(mi.MIError):
+        ctx.solve(mi.make_run(phys, 1, key=90, precision=mi.MI_PREC_FP64), 1e-3, mi.MI_SAMPLER_FLOW)
+    with pytest.raises(mi.MIError):
+        ctx.solve(mi.make_run(phys, 1, key=90), 0.0, mi.MI_SAMPLER_VEGAS)

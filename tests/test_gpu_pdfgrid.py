@@ -88,9 +88,9 @@ def test_grid_integral_vs_quadrature(ctx, O, grid_files):
     q, qp = O.had_quadrature(O.hadcfg_default(grid=g), 12, 12, 24)
     phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
     e = ctx.integrate(mi.make_run(phys, 1 << 24, key=42))
-    assert abs(e.sigmaGen() - q) < 3.5 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
     for i, code in enumerate((111, 112, 113, 114, 115)):
-        assert abs(e.sigmaGen(code) - qp[i]) < 3.5 * e.sigmaErr(code) + 1e-12, code
+        assert abs(e.sigmaGen(code) - qp[i]) < 3.0 * e.sigmaErr(code) + 1e-12, code
     # the toy run differs (the grid carries a Q dependence)
     et = ctx.integrate(mi.make_run(mi.physics_default(1), 1 << 24, key=42))
     assert abs(et.sigmaGen() - e.sigmaGen()) > 5 * e.sigmaErr()
