@@ -128,6 +128,7 @@ PROTOTYPES = {
     "mi_sigma_hat": ([VP, I, U64, VP, VP, VP, VP, VP, VP, I, D, VP], I),
     "mi_points": ([VP, P(MiRun), VP, VP], I),
     "mi_integrate": ([VP, P(MiRun), P(MiSigmaEst)], I),
+    "mi_integrate_noalloc": ([VP, P(MiRun), P(MiSigmaEst)], I),
     "mi_vegas_init": ([VP, I, I], I),
     "mi_vegas_adapt": ([VP, P(MiRun), I, D, P(MiVegasInfo)], I),
     "mi_vegas_get_grid": ([VP, P(D), P(I), P(I)], I),

@@ -9,9 +9,20 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 using namespace mi;
+
+// Scratch of mi_integrate_noalloc: the block records of one launch (at most kScratchBlocks
+// blocks) and the reduced record after them, 1,179,936 B.  One copy per GPU, held until the
+// process ends; every call on a GPU shares it (calls that overlap on one GPU race).  Host code
+// uses the address hipGetSymbolAddress returns on each device, never the array's name.
+namespace mi {
+constexpr uint32_t kScratchBlocks = 4096;
+constexpr size_t kScratchDoubles = ((size_t)kScratchBlocks + 1) * kRec;
+}
+static __device__ __align__(256) double scratch[mi::kScratchDoubles];
 
 namespace {
 
@@ -180,6 +191,22 @@ int check_run(mi_ctx *ctx, const mi_run *run)
     return MI_OK;
 }
 
+// Device address of `scratch` on the current device `device`, looked up once per device.
+int scratch_address(int device, double **ptr)
+{
+    static std::mutex mu;
+    static double *addr[64] = {};
+    if (device < 0 || device >= 64) return MI_ERR_RANGE;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MI_ERR_HIP;
+        addr[device] = static_cast<double *>(p);
+    }
+    *ptr = addr[device];
+    return MI_OK;
+}
+
 double c1_pref(const mi_physics &p)
 {
     return kConvert2mb * kPi * p.alphaS_value * p.alphaS_value * p.cos_max / p.sHat * p.kfactor;
@@ -271,10 +298,15 @@ Geom geom_integrate(const mi_ctx *ctx, const mi_run *run, uint64_t n, bool vegas
 // over [ctr0, ctr0 + n) and the reduction into out_dev (kRec doubles); reduce = false leaves the
 // block records in ctx->partials for the caller's own reduction.
 int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, bool vegas, double *hist_parts,
-                  Geom g, double *out_dev, hipEvent_t after_kernel = nullptr, bool reduce = true)
+                  Geom g, double *out_dev, hipEvent_t after_kernel = nullptr, bool reduce = true,
+                  double *partials = nullptr)
 {
-    int rc = ensure(ctx, &ctx->partials, &ctx->partials_cap, (size_t)g.blocks * kRec);
-    if (rc) return rc;
+    int rc = MI_OK;
+    if (!partials) {                        // the context's records buffer, grown on demand
+        rc = ensure(ctx, &ctx->partials, &ctx->partials_cap, (size_t)g.blocks * kRec);
+        if (rc) return rc;
+        partials = ctx->partials;
+    }
     const mi_physics &p = run->phys;
     if (p.mode == MI_MODE_HADRONIC_3D) {
         HadLaunch L{};
@@ -284,17 +316,17 @@ int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, boo
         rc = pdf_views(ctx, p, &L.gf, &L.gd);
         if (rc) return rc;
         L.win = had_window(ctx, run, vegas, hist_parts != nullptr, &L.gf, &L.win_smem);
-        L.grid = &ctx->grid; L.partials = ctx->partials; L.hist = hist_parts;
+        L.grid = &ctx->grid; L.partials = partials; L.hist = hist_parts;
         HIPCHK(ctx, launch_integrate_had(ctx->stream, L));
     } else {
         C1Launch L{};
         L.vegas = vegas; L.key = run->key; L.ctr0 = ctr0; L.n = n; L.g = g;
         L.pref = c1_pref(p); L.cMax = p.cos_max;
-        L.grid = &ctx->grid; L.partials = ctx->partials; L.hist = hist_parts;
+        L.grid = &ctx->grid; L.partials = partials; L.hist = hist_parts;
         HIPCHK(ctx, launch_integrate_c1(ctx->stream, L));
     }
     if (after_kernel) HIPCHK(ctx, hipEventRecord(after_kernel, ctx->stream));
-    if (reduce) HIPCHK(ctx, launch_reduce_records(ctx->stream, ctx->partials, g.blocks, out_dev));
+    if (reduce) HIPCHK(ctx, launch_reduce_records(ctx->stream, partials, g.blocks, out_dev));
     return MI_OK;
 }
 
@@ -326,6 +358,15 @@ int mi_ctx_create(int device, mi_ctx **out)
         delete ctx;
         return MI_ERR_HIP;
     }
+    // the non-allocating entry point's scratch on this device, and room for any error message
+    // (fail() formats at most 511 characters), so that its error path allocates nothing either
+    const int src = scratch_address(device, &ctx->scratch);
+    if (src != MI_OK) {
+        (void)hipFree(ctx->red);
+        delete ctx;
+        return src;
+    }
+    ctx->err.reserve(512);
     *out = ctx;
     return MI_OK;
 }
@@ -469,6 +510,36 @@ int mi_integrate(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out)
     if (rc) return rc;
     double r[kRec];
     HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    fill_estimate(r, (int64_t)run->n_points, run->phys.mode == MI_MODE_PARTONIC_1D, run->phys.proc_mask, ms, out);
+    if (!std::isfinite(out->sigmaGen[0])) return fail(ctx, MI_ERR_NUMERIC, "non-finite sigma");
+    return MI_OK;
+}
+
+int mi_integrate_noalloc(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out)
+{
+    int rc = check_run(ctx, run);
+    if (rc) return rc;
+    if (!out) return fail(ctx, MI_ERR_ARG, "out is NULL");
+    if (run->rng != MI_RNG_PHILOX) return fail(ctx, MI_ERR_ARG, "integration uses the Philox stream");
+    if (run->sampler == MI_SAMPLER_FLOW)
+        return fail(ctx, MI_ERR_ARG, "mi_integrate_noalloc samples the flat and VEGAS maps (the flow sampler "
+                                     "allocates its buffers)");
+    if (!ctx->scratch) return fail(ctx, MI_ERR_STATE, "no scratch address on this device");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const bool vegas = run->sampler == MI_SAMPLER_VEGAS;
+    const Geom g = geom_integrate(ctx, run, run->n_points, vegas, false);
+    if (g.blocks > kScratchBlocks)
+        return fail(ctx, MI_ERR_RANGE, "the launch needs %u block records, the scratch holds %u", g.blocks,
+                    kScratchBlocks);
+    double *rec = ctx->scratch + (size_t)kScratchBlocks * kRec;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rc = run_integrand(ctx, run, run->counter0, run->n_points, vegas, nullptr, g, rec, ctx->ev1, true, ctx->scratch);
+    if (rc) return rc;
+    double r[kRec];
+    HIPCHK(ctx, hipMemcpyAsync(r, rec, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

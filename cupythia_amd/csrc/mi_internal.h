@@ -60,6 +60,7 @@ struct mi_ctx {
     uint64_t *masks = nullptr;   size_t masks_cap = 0;      // unweighting accept ballots
     unsigned char *wbuf = nullptr; size_t wbuf_cap = 0;     // unweighting per-candidate weights (bytes)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    double *scratch = nullptr;   // device address of mi_integrate_noalloc's static scratch on this device
     mi::Grid grid;
     mi::FlowState *flow = nullptr;
     mi::PdfState *pdf = nullptr;                          // slot 0: the grid of pdf_source = MI_PDF_GRID

@@ -216,6 +216,18 @@ int mi_points(mi_ctx *ctx, const mi_run *run, double *w_dev, double *kin_dev);
  * sum w^2, reduced deterministically on the device. */
 int mi_integrate(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out);
 
+/* The same estimate as mi_integrate for the flat and VEGAS samplers (bit for bit: the same kernels,
+ * launch geometry and reduction), with its block records and reduced record in a static device
+ * array of the library instead of the context's buffers: the call makes no device allocation (no
+ * hipMalloc, hipFree or allocating library routine; the context, the VEGAS grid and any PDF grid
+ * are set up beforehand).  No counterpart in the reference: it is for callers that cannot
+ * allocate -- tight serving loops, harnesses that forbid hipMalloc.  Limit: one launch of at most
+ * 4096 blocks (the scratch is 1,179,936 B per GPU, held until the process ends; MI355X launches
+ * at most 8 blocks per CU, 2048); MI_ERR_RANGE beyond it.  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (several streams or threads) race.  The flow sampler is
+ * refused (MI_ERR_ARG): it allocates its buffers. */
+int mi_integrate_noalloc(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out);
+
 /* ----------------------------------------------------------------- VEGAS ---- */
 /* Uniform grid with nbins (<= 256) bins per dimension (1 for partonic, 3 for hadronic). */
 int mi_vegas_init(mi_ctx *ctx, int ndim, int nbins);
