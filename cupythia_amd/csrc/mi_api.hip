@@ -157,6 +157,7 @@ int check_physics(mi_ctx *ctx, const mi_physics &p)
             return fail(ctx, MI_ERR_ARG, "heavy-quark codes need mass_Q[%d] > 0 and eCM > 2 mT", f);
     }
     if (!(p.alphaS_value > 0.0)) return fail(ctx, MI_ERR_ARG, "alphaS_value must be > 0");
+    if (!(p.kfactor >= 0.0) || !std::isfinite(p.kfactor)) return fail(ctx, MI_ERR_ARG, "kfactor must be finite and >= 0");
     if (!(p.renorm_fac > 0.0)) return fail(ctx, MI_ERR_ARG, "renorm_fac must be > 0");
     if (!(p.pdf_g[0] > -1.0) || !(p.pdf_val[0] > 0.0) || !(p.pdf_sea[0] > -1.0) || !(p.pdf_g[1] > -1.0) ||
         !(p.pdf_val[1] > -1.0) || !(p.pdf_sea[1] > -1.0))
@@ -638,6 +639,11 @@ int mi_vegas_adapt(mi_ctx *ctx, const mi_run *run, int n_iter, double alpha, mi_
     std::vector<double> rec((size_t)n_iter * kRec);
     HIPCHK(ctx, hipMemcpyAsync(rec.data(), ctx->red, sizeof(double) * rec.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int it = 0; it < n_iter; ++it)
+        if (!std::isfinite(rec[(size_t)it * kRec + kRS]) || !std::isfinite(rec[(size_t)it * kRec + kRQ]))
+            return fail(ctx, MI_ERR_NUMERIC,
+                        "non-finite weights in VEGAS iteration %d: the grid was refined from them (mi_vegas_init "
+                        "starts a new one)", it);
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (out) {
@@ -769,6 +775,8 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
         }
         stored = nch == 1;
         wmax = R[kRM];
+        if (!std::isfinite(wmax) || !std::isfinite(R[kRS]))
+            return fail(ctx, MI_ERR_NUMERIC, "non-finite candidate weights");
         if (!(wmax > 0.0)) return fail(ctx, MI_ERR_NUMERIC, "all candidate weights are zero");
     }
     const bool recs_done = haveR;
@@ -800,6 +808,8 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (n_acc) *n_acc = total;
     if (wmax_used) *wmax_used = wmax;
+    if (!std::isfinite(R[kRS]) || !std::isfinite(R[kRQ]))
+        return fail(ctx, MI_ERR_NUMERIC, "non-finite candidate weights (the accepted events are not meaningful)");
     if (est) {
         fill_estimate(R, (int64_t)run->n_points, partonic, p.proc_mask, ms, est);
         est->nAcc = (int64_t)total;

@@ -630,7 +630,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             float wi = (mask & (1 << i)) ? common * w[i] : 0.0f;
-            if (!(wi > 0.0f)) wi = 0.0f;
+            wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;          // negative -> 0, NaN kept
             ws[i] = wi;
             tot += wi;
         }
@@ -651,7 +651,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             float wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0f;
-            if (!(wi > 0.0f)) wi = 0.0f;
+            wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;          // negative -> 0, NaN kept
             ws[lo + i] = wi;
             tot += wi;
         }
@@ -816,7 +816,7 @@ __device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int 
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
         double wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0;
-        if (!(wi > 0.0)) wi = 0.0;
+        wi = (wi > 0.0 || wi != wi) ? wi : 0.0;                // negative -> 0, NaN kept
         ws[lo + i] = wi;
         tot += wi;
     }

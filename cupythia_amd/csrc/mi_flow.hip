@@ -1121,6 +1121,9 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
         out->cv_flow = std::sqrt(std::max(0.0, out->loss_last));
         out->kernel_ms = ms;
     }
+    for (int s = 0; s < steps; ++s)
+        if (!std::isfinite(h[2 * s]) || !std::isfinite(h[2 * s + 1]))
+            return ferr(ctx, MI_ERR_NUMERIC, "non-finite weights in a training batch (the flow parameters are not meaningful)");
     return MI_OK;
 }
 

@@ -73,10 +73,10 @@ int parse_lhagrid1(mi_ctx *ctx, const char *path, PdfState &st)
         if (starts.empty()) {
             xs = xrow;
         } else {
-            if (xrow.size() != xs.size()) return pfail(ctx, MI_ERR_ARG, "mismatched subgrid x nodes");
+            if (xrow.size() != xs.size()) return pfail(ctx, MI_ERR_ARG, "a Q subgrid lists a different number of x nodes than the first");
             for (size_t i = 0; i < xs.size(); ++i)
                 if (std::fabs(std::log(xrow[i]) - std::log(xs[i])) > 1e-5)
-                    return pfail(ctx, MI_ERR_ARG, "mismatched subgrid x spacing");
+                    return pfail(ctx, MI_ERR_ARG, "the x nodes of a Q subgrid differ from the first subgrid's (ln x by more than 1e-5)");
         }
         if (!std::getline(is, line)) return pfail(ctx, MI_ERR_ARG, "truncated PDF grid file (Q line)");
         std::istringstream isq(line);
@@ -85,7 +85,7 @@ int parse_lhagrid1(mi_ctx *ctx, const char *path, PdfState &st)
         const int nqNow = (int)qs.size() - q0;
         if (nqNow < 2) return pfail(ctx, MI_ERR_ARG, "a Q subgrid needs at least 2 nodes");
         if (q0 > 0 && std::fabs(qs[q0] / qs[q0 - 1] - 1.0) > 1e-5)
-            return pfail(ctx, MI_ERR_ARG, "mismatched subgrid Q borders");
+            return pfail(ctx, MI_ERR_ARG, "a Q subgrid does not start at the last Q node of the previous one (relative 1e-5)");
         starts.push_back(q0);
         if (!std::getline(is, line)) return pfail(ctx, MI_ERR_ARG, "truncated PDF grid file (flavour line)");
         std::istringstream isid(line);
@@ -101,6 +101,11 @@ int parse_lhagrid1(mi_ctx *ctx, const char *path, PdfState &st)
                 double *dst = blk.data() + ((size_t)ix * nqNow + iq) * kGridNF;
                 for (size_t k = 0; k < slots.size(); ++k) {
                     if (!(isv >> v)) return pfail(ctx, MI_ERR_ARG, "short PDF value line");
+                    // the FP32 kernels keep a rounded copy: a value beyond the FP32 range would
+                    // turn into inf there (and into NaN in the interpolation)
+                    if (!(std::fabs(v) <= 3.4028234663852886e38))
+                        return pfail(ctx, MI_ERR_RANGE, "PDF grid value " + std::to_string(v) +
+                                                            " is not finite in FP32");
                     if (slots[k] >= 0) dst[slots[k]] = v;
                 }
             }

@@ -410,7 +410,7 @@ static float family32(const float *k, const orc32_grid *g, int fam, int procMask
     float tot = 0.0f;
     for (int i = lo; i < hi; ++i) {
         float wi = (procMask & (1 << i)) ? common * ws[i] : 0.0f;
-        if (!(wi > 0.0f)) wi = 0.0f;
+        wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;              /* negative -> 0, NaN kept */
         w[i] = wi;
         tot += wi;
     }

@@ -84,14 +84,14 @@ int orc_lhagrid_load(const char *path, orc_lhagrid **out)
         } else {
             if (ntmp != nx) goto fail;
             for (int i = 0; i < nx; ++i)
-                if (fabs(log(tmp[i]) - log(xs[i])) > 1e-5) goto fail;   /* mismatched subgrid x spacing */
+                if (fabs(log(tmp[i]) - log(xs[i])) > 1e-5) goto fail;   /* x nodes differ between Q subgrids */
         }
         if (getline(&line, &lcap, f) < 0) goto fail;              /* Q line */
         int q0 = nq;
         int nqNow = read_doubles(line, &qs, &nq, &qcap);
         if (nqNow <= 0) goto fail;
         if (nsub > 0) {
-            if (fabs(qs[q0] / qs[q0 - 1] - 1.0) > 1e-5) goto fail;   /* mismatched subgrid Q borders */
+            if (fabs(qs[q0] / qs[q0 - 1] - 1.0) > 1e-5) goto fail;   /* a Q subgrid does not start where the previous ended */
             /* qGrid border averaged; the log grid keeps the values as read (Pythia :539-544) */
         }
         subStart[nsub] = q0;
