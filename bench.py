@@ -263,6 +263,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--points", type=int, default=1 << 28)
+    ap.add_argument("--precision", choices=["fp32hw", "fp32"], default="fp32hw",
+                    help="headline precision: fp32hw = FP32 with the hardware exp2/log2/rcp/sqrt (per point within "
+                         "1e-5 of the FP64 oracle), fp32 = FP32 with the library's own bit-reproducible math; the "
+                         "other one is timed as well and reported beside it")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution comparison")
@@ -286,6 +290,8 @@ def main():
     ctx = mi.Context(0)
     phys = mi.physics_default(1)
     n = args.points
+    precs = {"fp32hw": mi.MI_PREC_FP32_HW, "fp32": mi.MI_PREC_FP32}
+    prec = precs[args.precision]
     if args.only == "tts":
         print(json.dumps({"time_to_solution": tts_all(ctx, mi)}))
         ctx.close()
@@ -296,7 +302,7 @@ def main():
         return
     if args.pmc_probe:
         for k in range(3):
-            ctx.integrate(mi.make_run(phys, n, key=42, counter0=k * n))
+            ctx.integrate(mi.make_run(phys, n, key=42, counter0=k * n, precision=prec))
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")      # 1 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -305,16 +311,23 @@ def main():
         ctx.close()
         return
     for w in range(args.warmup):
-        ctx.integrate(mi.make_run(phys, n, key=42, counter0=w * n))
+        ctx.integrate(mi.make_run(phys, n, key=42, counter0=w * n, precision=prec))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     kern_ms = []
     est = None
     for k in range(args.steps):
-        est = ctx.integrate(mi.make_run(phys, n, key=42, counter0=(args.warmup + k) * n))
+        est = ctx.integrate(mi.make_run(phys, n, key=42, counter0=(args.warmup + k) * n, precision=prec))
         kern_ms.append(est.kernel_ms)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # the other FP32 mode on the same counters (after the timed region)
+    other = "fp32" if args.precision == "fp32hw" else "fp32hw"
+    o_ms = []
+    for k in range(max(3, args.steps // 4)):
+        e_o = ctx.integrate(mi.make_run(phys, n, key=42, counter0=(args.warmup + k) * n, precision=precs[other]))
+        o_ms.append(e_o.kernel_ms)
+    o_ms = sorted(o_ms)[len(o_ms) // 2]
     dt = t1 - t0
     value = n * args.steps / dt
     avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
@@ -327,10 +340,16 @@ def main():
         "config": {"workload": "5-process QCD 2->2 library (gg->gg, gg->qqbar, qg->qg, qq'->qq', qqbar->gg), "
                                "sqrt(S)=13.6 TeV, pT>20 GeV, toy PDFs, 1-loop alphaS(MZ)=0.118, 3-D (tau,y,cos), "
                                "FP32 ME + FP64 sums, Philox4x32-10 key 42",
-                   "points_per_step": n, "precision": "fp32 ME / fp64 accumulation",
-                   "sigma_mb": est.sigmaGen(), "sigma_err_mb": est.sigmaErr()},
+                   "points_per_step": n,
+                   "precision": ("fp32 ME with the hardware exp2/log2/rcp/sqrt" if args.precision == "fp32hw" else
+                                 "fp32 ME with the library's own bit-reproducible exp/log") + " / fp64 accumulation",
+                   "sigma_mb": est.sigmaGen(), "sigma_err_mb": est.sigmaErr(),
+                   "other_precision": {"precision": other, "median_kernel_ms": o_ms,
+                                       "kernel_points_per_s": n / (o_ms / 1e3),
+                                       "note": "kernel time of mi_integrate in the other FP32 mode, same counters"}},
         "roofline": {"bound": "valu", "engine": "fp32 vector ALU (MI355X FP32 vector peak 157.3 TFLOP/s)",
-                     "kernel": "k_integrate_had<float,flat>", "achieved": achieved,
+                     "kernel": "k_integrate_had<float,flat%s>" % (",hw" if args.precision == "fp32hw" else ""),
+                     "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(), "flops_per_point": FLOPS_PER_POINT,
                      "avg_kernel_ms": avg_kernel_s * 1e3},

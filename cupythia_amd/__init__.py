@@ -17,7 +17,7 @@ The names mirror the reference interfaces the path replaces:
 """
 from ._abi import (  # noqa: F401
     LIB_PATH, LibraryMissing, MI_FLOW_NPARAMS, MI_GG2GG, MI_GG2QQBAR, MI_MAX_PROC,
-    MI_MODE_HADRONIC_3D, MI_MODE_PARTONIC_1D, MI_PREC_FP32, MI_PREC_FP64, MI_QG2QG, MI_QQ2QQ,
+    MI_MODE_HADRONIC_3D, MI_MODE_PARTONIC_1D, MI_PREC_FP32, MI_PREC_FP64, MI_PREC_FP32_HW, MI_QG2QG, MI_QQ2QQ,
     MI_QQBAR2GG, MI_QQBAR2QQBARNEW, MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT, MI_SAMPLER_FLAT,
     MI_SAMPLER_FLOW, MI_SAMPLER_VEGAS, MI_PDF_TOY, MI_PDF_GRID, MI_ABI_VERSION, MI_GG2CCBAR,
     MI_QQBAR2CCBAR, MI_GG2BBBAR, MI_QQBAR2BBBAR, PROC_CODES,

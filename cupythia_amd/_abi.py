@@ -26,7 +26,7 @@ MI_MAX_PROC = 10
 PROC_CODES = (111, 112, 113, 114, 115, 116, 121, 122, 123, 124)
 
 MI_MODE_PARTONIC_1D, MI_MODE_HADRONIC_3D = 0, 1
-MI_PREC_FP32, MI_PREC_FP64 = 0, 1
+MI_PREC_FP32, MI_PREC_FP64, MI_PREC_FP32_HW = 0, 1, 2
 MI_SAMPLER_FLAT, MI_SAMPLER_VEGAS, MI_SAMPLER_FLOW = 0, 1, 2
 MI_RNG_PHILOX, MI_RNG_SPLITMIX_COMPAT = 0, 1
 MI_PDF_TOY, MI_PDF_GRID = 0, 1

@@ -176,7 +176,11 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     const mi_physics &p = run->phys;
     if (p.mode != MI_MODE_HADRONIC_3D) { ctx->err = "the adjoint needs the hadronic mode"; return MI_ERR_ARG; }
     if (run->sampler != MI_SAMPLER_FLAT) { ctx->err = "the adjoint samples the flat map"; return MI_ERR_ARG; }
-    if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64) { ctx->err = "bad precision"; return MI_ERR_ARG; }
+    if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64 && run->precision != MI_PREC_FP32_HW) {
+        ctx->err = "bad precision";
+        return MI_ERR_ARG;
+    }
+    const int prec = run->precision == MI_PREC_FP64 ? MI_PREC_FP64 : MI_PREC_FP32;   // FP32_HW runs as FP32
     if (run->n_points == 0 || run->n_points > (1ULL << 40)) { ctx->err = "n_points must be 1..2^40"; return MI_ERR_RANGE; }
     if (p.proc_mask == 0 || (p.proc_mask & ~0x3FF)) {
         ctx->err = "proc_mask must select codes among 111..116, 121..124 (bits 0..9)";
@@ -203,7 +207,7 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     }
     if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return MI_ERR_HIP; }
     const Geom g = geom_for(run->n_points, 16, (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) *
-                                                   occ_adjoint(run->precision, has_heavy(p.proc_mask), gf.active != 0));
+                                                   occ_adjoint(prec, has_heavy(p.proc_mask), gf.active != 0));
     if (g.blocks * (size_t)kRec > ctx->partials_cap) {
         (void)hipFree(ctx->partials);
         ctx->partials = nullptr;
@@ -220,7 +224,7 @@ extern "C" int mi_adjoint(mi_ctx *ctx, const mi_run *run, mi_sigma_grad *out)
     const bool fixed = p.alphaS_order <= 0;
     const double as_fac = fixed ? 2.0 : 2.0 / (p.alphaS_value * p.alphaS_value);
     hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
-    if (run->precision == MI_PREC_FP32) {
+    if (prec == MI_PREC_FP32) {
         HadConst<float> k;
         fill_had_consts_f32(p, k);
         AdjConst<float> ac{(float)dA_da, (float)dA_db, (float)as_fac, (float)p.renorm_fac};

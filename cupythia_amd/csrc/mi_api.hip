@@ -176,8 +176,8 @@ int check_run(mi_ctx *ctx, const mi_run *run)
     if (!run) return fail(ctx, MI_ERR_ARG, "run is NULL");
     int rc = check_physics(ctx, run->phys);
     if (rc) return rc;
-    if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64)
-        return fail(ctx, MI_ERR_ARG, "precision must be MI_PREC_FP32 or MI_PREC_FP64");
+    if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP64 && run->precision != MI_PREC_FP32_HW)
+        return fail(ctx, MI_ERR_ARG, "precision must be MI_PREC_FP32, MI_PREC_FP64 or MI_PREC_FP32_HW");
     if (run->sampler < MI_SAMPLER_FLAT || run->sampler > MI_SAMPLER_FLOW)
         return fail(ctx, MI_ERR_ARG, "unknown sampler %d", run->sampler);
     if (run->n_points == 0) return fail(ctx, MI_ERR_ARG, "n_points must be > 0");
@@ -259,7 +259,7 @@ Geom geom_for(uint64_t n, uint32_t min_ppt, uint32_t max_blocks, uint32_t nt)
 // table and histograms (one copy per wave) and the reduction's static LDS.
 bool had_window(const mi_ctx *ctx, const mi_run *run, bool vegas, bool hist, GridPdf<float> *g, size_t *smem)
 {
-    if (run->precision != MI_PREC_FP32 || run->phys.mode != MI_MODE_HADRONIC_3D ||
+    if ((run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP32_HW) || run->phys.mode != MI_MODE_HADRONIC_3D ||
         run->phys.pdf_source != MI_PDF_GRID || ctx->lds_max <= 0)
         return false;
     const size_t tab = vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 7) & ~(size_t)7 : 0;
@@ -311,7 +311,8 @@ int run_integrand(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, boo
     const mi_physics &p = run->phys;
     if (p.mode == MI_MODE_HADRONIC_3D) {
         HadLaunch L{};
-        L.prec = run->precision; L.vegas = vegas; L.mask = p.proc_mask;
+        L.hw = run->precision == MI_PREC_FP32_HW;
+        L.prec = L.hw ? MI_PREC_FP32 : run->precision; L.vegas = vegas; L.mask = p.proc_mask;
         L.key = run->key; L.ctr0 = ctr0; L.n = n; L.g = g;
         fill_had_consts_f32(p, L.kf); fill_had_consts_f64(p, L.kd);
         rc = pdf_views(ctx, p, &L.gf, &L.gd);
@@ -477,7 +478,8 @@ int mi_points(mi_ctx *ctx, const mi_run *run, double *w_dev, double *kin_dev)
     const mi_physics &p = run->phys;
     if (p.mode == MI_MODE_HADRONIC_3D) {
         HadLaunch L{};
-        L.prec = run->precision; L.vegas = vegas; L.mask = p.proc_mask;
+        L.hw = run->precision == MI_PREC_FP32_HW;
+        L.prec = L.hw ? MI_PREC_FP32 : run->precision; L.vegas = vegas; L.mask = p.proc_mask;
         L.key = run->key; L.ctr0 = run->counter0; L.n = run->n_points;
         fill_had_consts_f32(p, L.kf); fill_had_consts_f64(p, L.kd);
         rc = pdf_views(ctx, p, &L.gf, &L.gd);
@@ -696,7 +698,9 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
     HIPCHK(ctx, hipSetDevice(ctx->device));
     // Candidates in chunks of at most kUnwChunk (the stored weights of a chunk: 4 or 8 B each).
     const uint64_t n = run->n_points;
-    const size_t tsz = (partonic || run->precision == MI_PREC_FP64) ? sizeof(double) : sizeof(float);
+    // the unweighting decisions use the bit-reproducible FP32 math also for an FP32_HW run
+    const int prec = run->precision == MI_PREC_FP32_HW ? MI_PREC_FP32 : run->precision;
+    const size_t tsz = (partonic || prec == MI_PREC_FP64) ? sizeof(double) : sizeof(float);
     const uint64_t chunk = std::min<uint64_t>(n, kUnwChunk);
     const uint64_t nch = (n + chunk - 1) / chunk;
     const uint32_t ncu = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256);
@@ -713,14 +717,14 @@ int mi_unweight_compact(mi_ctx *ctx, const mi_run *run, double wmax, const mi_ev
         L.c.pref = c1_pref(p); L.c.cMax = p.cos_max; L.c.grid = &ctx->grid; L.c.partials = ctx->partials;
         L.sHat = p.sHat;
     } else {
-        L.h.prec = run->precision; L.h.vegas = run->sampler == MI_SAMPLER_VEGAS; L.h.mask = p.proc_mask;
+        L.h.prec = prec; L.h.vegas = run->sampler == MI_SAMPLER_VEGAS; L.h.mask = p.proc_mask;
         L.h.key = run->key;
         fill_had_consts_f32(p, L.h.kf); fill_had_consts_f64(p, L.h.kd);
         rc = pdf_views(ctx, p, &L.h.gf, &L.h.gd);
         if (rc) return rc;
         L.h.grid = &ctx->grid; L.h.partials = ctx->partials;
         // FP32 grid candidates: the eval kernel reads the reachable part of the grid from LDS
-        if (run->precision == MI_PREC_FP32 && L.h.gf.active && ctx->lds_max > 0) {
+        if (prec == MI_PREC_FP32 && L.h.gf.active && ctx->lds_max > 0) {
             const size_t tab = L.h.vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 15) & ~(size_t)15 : 0;
             const size_t stat = (size_t)(kWinBlock / 64) * kRec * sizeof(double) + 1024;
             GridPdf<float> gw = L.h.gf;

@@ -220,6 +220,45 @@ __device__ __forceinline__ float own_logf(float x)
     return bad ? -1.0e30f : v;
 }
 
+// ------------------------------------------------------------------ FP32 math of the integrand
+// HW = false: the own forms above plus IEEE division and square root (bit-reproducible on the
+// host).  HW = true (MI_PREC_FP32_HW): the hardware's base-2 exponential and logarithm
+// (v_exp_f32, v_log_f32) and its reciprocal and square root (v_rcp_f32, v_sqrt_f32), about 1 ulp
+// each; e^x keeps the rounding error of x log2(e) (|x| up to ~90 here) as a first-order
+// correction, 2^(t + e) = 2^t (1 + e ln 2), so its relative error stays ~2e-7.
+template <bool HW>
+__device__ __forceinline__ float fexp(float x)
+{
+    if constexpr (!HW) {
+        return own_expf(x);
+    } else {
+        const float t = x * 1.4426950408889634f;
+        float e = __builtin_fmaf(x, 1.4426950408889634f, -t);      // exact rounding error of t
+        e = __builtin_fmaf(x, 1.925963033500011e-8f, e);             // log2(e) - RN(log2(e))
+        const float r = __builtin_amdgcn_exp2f(t);
+        const float v = __builtin_fmaf(r, e * 0.6931471805599453f, r);
+        return x < -86.0f ? 0.0f : v;
+    }
+}
+template <bool HW>
+__device__ __forceinline__ float flog(float x)
+{
+    if constexpr (!HW) return own_logf(x);
+    else return (x > 0.0f) ? __builtin_amdgcn_logf(x) * 0.6931471805599453f : -1.0e30f;
+}
+template <bool HW>
+__device__ __forceinline__ float frcp(float x)
+{
+    if constexpr (!HW) return 1.0f / x;
+    else return __builtin_amdgcn_rcpf(x);
+}
+template <bool HW>
+__device__ __forceinline__ float fsqrt(float x)
+{
+    if constexpr (!HW) return sqrtf(x);
+    else return __builtin_amdgcn_sqrtf(x);
+}
+
 // ------------------------------------------------------------------ constants of the integrand
 // Filled on the host by fill_consts (mi_api.hip).
 template <typename T>
@@ -535,7 +574,7 @@ struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 // ws[6..7], ws[8..9]).  Inputs: (u, 1-u) for each dimension (exact pairs); writes the family's
 // slots (0 if masked out) and returns their sum.  Operation-for-operation identical to the
 // host restatement used to check compaction indices.
-template <int FAM>
+template <int FAM, bool HW = false>
 __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int mask,
                                                 float u0, float om0, float u1, float om1,
                                                 float u2, float om2, float* ws, float* sHo, float* tHo,
@@ -544,15 +583,15 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
 {
     const float LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
     const float targ = -(u0 * LT);
-    const float r = own_expf(targ);                        // tauMin / tau
+    const float r = fexp<HW>(targ);                        // tauMin / tau
     const float yMax = 0.5f * (LT * om0);
     const float twoYMax = yMax + yMax;
     const float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
-    const float x1 = own_expf(lx1), x2 = own_expf(lx2);
-    const float zMax = sqrtf(one_minus_expf(targ, r));
+    const float x1 = fexp<HW>(lx1), x2 = fexp<HW>(lx2);
+    const float zMax = fsqrt<HW>(one_minus_expf(targ, r));
     // one division gives both 1 / r and 1 / (1 + zMax)
     const float opz = 1.0f + zMax;
-    const float R1 = 1.0f / (r * opz);
+    const float R1 = frcp<HW>(r * opz);
     const float omz = (r * r) * R1;                        // r / (1 + zMax) = 1 - zMax
     const float sH = ((FAM == 0 ? k.fourPT2 : k.fam[FAM - 1].fourMT2) * opz) * R1;   // 4 mT^2 / r
     const float twoz = zMax + zMax;
@@ -565,17 +604,17 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     float alpS, inv_ab, cfo = 0.0f;
     if (k.as_order == 0.0f) {
         alpS = k.as_fixed;
-        inv_ab = 1.0f / ab;
+        inv_ab = frcp<HW>(ab);
     } else {
         float Q2 = k.xiR2 * pT2;
         Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
-        const float lq = own_logf(Q2);
+        const float lq = flog<HW>(Q2);
         const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0f, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
         const float lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
         const float cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
         // one division gives both 1 / (a b) and alphaS = cf / (lq - lam)
         const float den = lq - lam;
-        const float R2 = 1.0f / (ab * den);
+        const float R2 = frcp<HW>(ab * den);
         inv_ab = den * R2;
         alpS = (cf * ab) * R2;
         cfo = cf;
@@ -586,18 +625,18 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     if (gp) {
         // PDF grid at Q2_F = factFac^2 pT2 (q = sqrt, ln q = ln Q2_F / 2)
         const float Q2F = gp->factFac2 * pT2;
-        const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lx1, x2, lx2, sqrtf(Q2F),
-                                               0.5f * own_logf(Q2F));
+        const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lx1, x2, lx2, fsqrt<HW>(Q2F),
+                                               0.5f * flog<HW>(Q2F));
         Lgg = L.gg; Lqg = L.qg; LQQ = L.QQ; Lsame = L.same; Lopp = L.opp;
         g1 = L.g1; g2 = L.g2; Q1 = L.Q1; Q2s = L.Q2;
     } else {
-        l1a = own_logf(one_minus_expf(lx1, x1)); l1b = own_logf(one_minus_expf(lx2, x2));
-        g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
-        g2 = own_expf(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
-        const float S1 = own_expf(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
-        const float S2 = own_expf(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
-        const float V1 = own_expf(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
-        const float V2 = own_expf(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
+        l1a = flog<HW>(one_minus_expf(lx1, x1)); l1b = flog<HW>(one_minus_expf(lx2, x2));
+        g1 = fexp<HW>(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
+        g2 = fexp<HW>(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
+        const float S1 = fexp<HW>(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
+        const float S2 = fexp<HW>(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
+        const float V1 = fexp<HW>(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
+        const float V2 = fexp<HW>(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
         const float Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
         const float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
         Q1 = __builtin_fmaf(k.nSea, S1, V1 + Vd1); Q2s = __builtin_fmaf(k.nSea, S2, V2 + Vd2);
@@ -668,7 +707,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
 // massless family only (codes 111..116) and writes ws[6..9] = 0; kinematics outputs are then
 // single values.  HEAVY = true: sHo .. x2o (if given) are arrays of 3, one entry per family,
 // written for the families that mask selects.
-template <bool HEAVY = false>
+template <bool HEAVY = false, bool HW = false>
 __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
                                                float u0, float om0, float u1, float om1,
                                                float u2, float om2, float* ws, float* sHo, float* tHo,
@@ -678,18 +717,18 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     if (!HEAVY) {
 #pragma unroll
         for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0f;
-        return had_family_f32<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        return had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
     }
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0f;
     float tot = 0.0f;
     if (mask & kFamMask[0])
-        tot += had_family_f32<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        tot += had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
     if (mask & kFamMask[1])
-        tot += had_family_f32<1>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
+        tot += had_family_f32<1, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
                                  x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp);
     if (mask & kFamMask[2])
-        tot += had_family_f32<2>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
+        tot += had_family_f32<2, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
                                  x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp);
     return tot;
 }

@@ -1135,7 +1135,8 @@ int mi_flow_integrate_internal(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out
     int rc = check_flow(ctx);
     if (rc) return rc;
     if (run->phys.mode != MI_MODE_HADRONIC_3D) return ferr(ctx, MI_ERR_ARG, "the flow sampler needs the hadronic mode");
-    if (run->precision != MI_PREC_FP32) return ferr(ctx, MI_ERR_ARG, "the flow sampler evaluates FP32 weights");
+    if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP32_HW)
+        return ferr(ctx, MI_ERR_ARG, "the flow sampler evaluates FP32 weights");
     if (ctx->grid.ndim != 3) return ferr(ctx, MI_ERR_STATE, "the flow composes with a 3-D VEGAS grid");
     const uint64_t chunk = std::min<uint64_t>(run->n_points, 1ULL << 22);
     rc = flow_alloc(ctx, chunk);

@@ -122,6 +122,7 @@ struct HadLaunch {
     GridPdf<double> gd;
     bool win = false;      // FP32 grid with its LDS window (gf.woff): 1024-thread blocks
     size_t win_smem = 0;   // dynamic LDS bytes of the window launch (tables + window)
+    bool hw = false;       // FP32 with the hardware transcendentals (run precision MI_PREC_FP32_HW)
 };
 constexpr int kWinBlock = 512;                 // threads per block of the LDS-window launch
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L);

@@ -126,16 +126,17 @@ __device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbin
     return m;
 }
 
-// gp: the PDF grid, or nullptr for the toy PDFs.  HEAVY: see had_point_f32.
-template <typename T, bool HEAVY> struct HadPoint;
-template <bool HEAVY> struct HadPoint<float, HEAVY> {
+// gp: the PDF grid, or nullptr for the toy PDFs.  HEAVY: see had_point_f32.  HW: the FP32 point
+// with the hardware transcendentals (MI_PREC_FP32_HW); the FP64 point ignores it.
+template <typename T, bool HEAVY, bool HW = false> struct HadPoint;
+template <bool HEAVY, bool HW> struct HadPoint<float, HEAVY, HW> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
                                                  float u1, float o1, float u2, float o2, float *ws,
                                                  float *sH, float *tH, float *x1, float *x2,
                                                  const GridPdf<float> *gp)
-    { return had_point_f32<HEAVY>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
+    { return had_point_f32<HEAVY, HW>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
 };
-template <bool HEAVY> struct HadPoint<double, HEAVY> {
+template <bool HEAVY, bool HW> struct HadPoint<double, HEAVY, HW> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
                                                   double *sH, double *tH, double *x1, double *x2,
@@ -147,7 +148,7 @@ template <bool HEAVY> struct HadPoint<double, HEAVY> {
 // uniforms -> (optional VEGAS map) -> weights (mapped by the grid Jacobian).  HEAVY = false:
 // the massless codes only (ws[6..9] = 0); HEAVY = true: every family of mask, kinematics
 // outputs (if given) one per family.
-template <typename T, bool VEGAS, bool HEAVY = false>
+template <typename T, bool VEGAS, bool HEAVY = false, bool HW = false>
 __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint64_t key, uint64_t e,
                                            const T *stab, int nbins, T nbf, T *ws, int *bins,
                                            T *sH, T *tH, T *x1, T *x2, const GridPdf<T> *gp = nullptr)
@@ -161,7 +162,7 @@ __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint6
     } else {
         m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
     }
-    T tot = HadPoint<T, HEAVY>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
+    T tot = HadPoint<T, HEAVY, HW>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
     if (VEGAS) {
         T jac = (T)1;
         jac = jac * m0.jac;

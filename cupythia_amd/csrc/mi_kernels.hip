@@ -125,8 +125,9 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
 // massless codes are computed and accumulated.  NT threads per block.  WIN (FP32 grid): the
 // block first copies the window of the grid's node values that gpdf describes (rows wq0..wq1,
 // columns wx0..nx-1) into its dynamic LDS at gpdf.woff (after the VEGAS table and histograms),
-// so the interpolation stencils read LDS instead of gathering from L2.
-template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, int NT = kBlock, bool WIN = false>
+// so the interpolation stencils read LDS instead of gathering from L2.  HW (FP32): the hardware
+// transcendentals (MI_PREC_FP32_HW).  Only the slots of the processes mask selects are summed.
+template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, int NT = kBlock, bool WIN = false, bool HW = false>
 __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
                                                       uint64_t ctr0, uint64_t n, uint32_t ppt,
                                                       const T *__restrict__ gtab, int nbins,
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
         if (i >= n) break;
         T ws[kMaxProc];
         int bins[3];
-        const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
+        const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
                                                      (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
                                                      GRID ? &gpdf : nullptr);
         const double dt = (double)tot;
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
         nnz += (tot > (T)0) ? 1u : 0u;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
+            if (!((mask >> p) & 1)) continue;                   // uniform: the slot stays 0
             const double d = (double)ws[p];
             sw[1 + p] += d;
             sw2[1 + p] = fma(d, d, sw2[1 + p]);
@@ -218,7 +220,7 @@ static size_t had_smem(const HadLaunch &L)
     return tb + (L.hist ? sizeof(double) * 4 * 3 * L.grid->nbins : 0);
 }
 
-template <typename T, bool GRID, bool HEAVY>
+template <typename T, bool GRID, bool HEAVY, bool HW = false>
 static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                 const GridPdf<T> &gp)
 {
@@ -226,22 +228,22 @@ static hipError_t launch_had_tg(hipStream_t s, const HadLaunch &L, const HadCons
     const int nb = L.vegas ? L.grid->nbins : 0;
     dim3 grid(L.g.blocks), blk(kBlock);
     if (!L.vegas)
-        hipLaunchKernelGGL((k_integrate_had<T, false, false, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
-                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, false, false, GRID, HEAVY, kBlock, false, HW>), grid, blk, sm, s, k,
+                           L.mask, L.key, L.ctr0, L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else if (!L.hist)
-        hipLaunchKernelGGL((k_integrate_had<T, true, false, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
-                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, true, false, GRID, HEAVY, kBlock, false, HW>), grid, blk, sm, s, k,
+                           L.mask, L.key, L.ctr0, L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     else
-        hipLaunchKernelGGL((k_integrate_had<T, true, true, GRID, HEAVY>), grid, blk, sm, s, k, L.mask, L.key, L.ctr0,
-                           L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
+        hipLaunchKernelGGL((k_integrate_had<T, true, true, GRID, HEAVY, kBlock, false, HW>), grid, blk, sm, s, k,
+                           L.mask, L.key, L.ctr0, L.n, L.g.ppt, tab, nb, L.partials, L.hist, gp);
     return hipGetLastError();
 }
 
 // FP32 grid integration with the LDS window: one kWinBlock-thread block per CU.
-template <bool VEGAS, bool HIST, bool HEAVY>
+template <bool VEGAS, bool HIST, bool HEAVY, bool HW>
 static hipError_t launch_had_win(hipStream_t s, const HadLaunch &L, const float *tab)
 {
-    auto kern = k_integrate_had<float, VEGAS, HIST, true, HEAVY, kWinBlock, true>;
+    auto kern = k_integrate_had<float, VEGAS, HIST, true, HEAVY, kWinBlock, true, HW>;
     // allows dynamic LDS above the default limit where the runtime asks for it; a runtime that
     // has no such limit may refuse the call, which the launch below then does not depend on
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -252,7 +254,7 @@ static hipError_t launch_had_win(hipStream_t s, const HadLaunch &L, const float 
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool HW = false>
 static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                const GridPdf<T> &gp)
 {
@@ -260,21 +262,22 @@ static hipError_t launch_had_t(hipStream_t s, const HadLaunch &L, const HadConst
         if (L.win && gp.active) {
             const bool hv = has_heavy(L.mask);
             if (L.vegas && L.hist)
-                return hv ? launch_had_win<true, true, true>(s, L, tab) : launch_had_win<true, true, false>(s, L, tab);
+                return hv ? launch_had_win<true, true, true, HW>(s, L, tab) : launch_had_win<true, true, false, HW>(s, L, tab);
             if (L.vegas)
-                return hv ? launch_had_win<true, false, true>(s, L, tab) : launch_had_win<true, false, false>(s, L, tab);
-            return hv ? launch_had_win<false, false, true>(s, L, tab) : launch_had_win<false, false, false>(s, L, tab);
+                return hv ? launch_had_win<true, false, true, HW>(s, L, tab) : launch_had_win<true, false, false, HW>(s, L, tab);
+            return hv ? launch_had_win<false, false, true, HW>(s, L, tab) : launch_had_win<false, false, false, HW>(s, L, tab);
         }
     }
     if (has_heavy(L.mask))
-        return gp.active ? launch_had_tg<T, true, true>(s, L, k, tab, gp) : launch_had_tg<T, false, true>(s, L, k, tab, gp);
-    return gp.active ? launch_had_tg<T, true, false>(s, L, k, tab, gp) : launch_had_tg<T, false, false>(s, L, k, tab, gp);
+        return gp.active ? launch_had_tg<T, true, true, HW>(s, L, k, tab, gp) : launch_had_tg<T, false, true, HW>(s, L, k, tab, gp);
+    return gp.active ? launch_had_tg<T, true, false, HW>(s, L, k, tab, gp) : launch_had_tg<T, false, false, HW>(s, L, k, tab, gp);
 }
 
 hipError_t launch_integrate_had(hipStream_t s, const HadLaunch &L)
 {
     if (L.prec == MI_PREC_FP32)
-        return launch_had_t<float>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr, L.gf);
+        return L.hw ? launch_had_t<float, true>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr, L.gf)
+                    : launch_had_t<float, false>(s, L, L.kf, L.vegas ? L.grid->tab32 : nullptr, L.gf);
     return launch_had_t<double>(s, L, L.kd, L.vegas ? L.grid->tab64 : nullptr, L.gd);
 }
 
@@ -343,7 +346,7 @@ hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L)
 
 // ------------------------------------------------------------------------------------ per-point dumps
 // Kinematics of the first family that mask selects (massless, c cbar, b bbar).
-template <typename T, bool VEGAS, bool HEAVY>
+template <typename T, bool VEGAS, bool HEAVY, bool HW = false>
 __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, uint64_t key, uint64_t ctr0,
                                                        uint64_t n, const T *__restrict__ gtab, int nbins,
                                                        double *__restrict__ w, double *__restrict__ kin,
@@ -355,8 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, 
     int bins[3];
     constexpr int NF = HEAVY ? 3 : 1;
     T sH[NF], tH[NF], x1[NF], x2[NF];
-    const T tot = had_candidate<T, VEGAS, HEAVY>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, sH, tH,
-                                                 x1, x2, gpdf.active ? &gpdf : nullptr);
+    const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, sH,
+                                                     tH, x1, x2, gpdf.active ? &gpdf : nullptr);
     w[i * kRecN] = (double)tot;
 #pragma unroll
     for (int p = 0; p < kMaxProc; ++p) w[i * kRecN + 1 + p] = (double)ws[p];
@@ -367,14 +370,14 @@ __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, 
     }
 }
 
-template <typename T, bool HEAVY>
+template <typename T, bool HEAVY, bool HW = false>
 static void launch_points_had_t(hipStream_t s, const HadLaunch &L, const HadConst<T> &k, const T *tab,
                                 const GridPdf<T> &gp, double *w, double *kin)
 {
     dim3 grid((uint32_t)((L.n + kBlock - 1) / kBlock)), blk(kBlock);
     const int nb = L.vegas ? L.grid->nbins : 0;
-    if (L.vegas) hipLaunchKernelGGL((k_points_had<T, true, HEAVY>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
-    else         hipLaunchKernelGGL((k_points_had<T, false, HEAVY>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
+    if (L.vegas) hipLaunchKernelGGL((k_points_had<T, true, HEAVY, HW>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
+    else         hipLaunchKernelGGL((k_points_had<T, false, HEAVY, HW>), grid, blk, 0, s, k, L.mask, L.key, L.ctr0, L.n, tab, nb, w, kin, gp);
 }
 
 hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, double *kin)
@@ -383,8 +386,13 @@ hipError_t launch_points_had(hipStream_t s, const HadLaunch &L, double *w, doubl
     const bool hv = has_heavy(L.mask);
     if (L.prec == MI_PREC_FP32) {
         const float *tab = L.vegas ? L.grid->tab32 : nullptr;
-        if (hv) launch_points_had_t<float, true>(s, L, L.kf, tab, L.gf, w, kin);
-        else    launch_points_had_t<float, false>(s, L, L.kf, tab, L.gf, w, kin);
+        if (L.hw) {
+            if (hv) launch_points_had_t<float, true, true>(s, L, L.kf, tab, L.gf, w, kin);
+            else    launch_points_had_t<float, false, true>(s, L, L.kf, tab, L.gf, w, kin);
+        } else {
+            if (hv) launch_points_had_t<float, true>(s, L, L.kf, tab, L.gf, w, kin);
+            else    launch_points_had_t<float, false>(s, L, L.kf, tab, L.gf, w, kin);
+        }
     } else {
         const double *tab = L.vegas ? L.grid->tab64 : nullptr;
         if (hv) launch_points_had_t<double, true>(s, L, L.kd, tab, L.gd, w, kin);
@@ -612,34 +620,38 @@ hipError_t launch_vegas_tables(hipStream_t s, Grid &g)
 
 // ------------------------------------------------------------------------------------ occupancy
 // Resident blocks per CU of each integration kernel instance (for one-wave, equal-work grids).
-template <typename T, bool GRID, bool HEAVY>
+template <typename T, bool GRID, bool HEAVY, bool HW>
 static int occ_had_t(bool vegas, bool hist, size_t sm)
 {
     int nb = 0;
     hipError_t e;
     if (!vegas)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, false, false, GRID, HEAVY>, kBlock, sm);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_integrate_had<T, false, false, GRID, HEAVY, kBlock, false, HW>, kBlock, sm);
     else if (!hist)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, false, GRID, HEAVY>, kBlock, sm);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_integrate_had<T, true, false, GRID, HEAVY, kBlock, false, HW>, kBlock, sm);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_integrate_had<T, true, true, GRID, HEAVY>, kBlock, sm);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_integrate_had<T, true, true, GRID, HEAVY, kBlock, false, HW>, kBlock, sm);
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-template <typename T>
+template <typename T, bool HW = false>
 static int occ_had_tt(bool vegas, bool hist, size_t sm, bool grid, bool heavy)
 {
-    if (heavy) return grid ? occ_had_t<T, true, true>(vegas, hist, sm) : occ_had_t<T, false, true>(vegas, hist, sm);
-    return grid ? occ_had_t<T, true, false>(vegas, hist, sm) : occ_had_t<T, false, false>(vegas, hist, sm);
+    if (heavy) return grid ? occ_had_t<T, true, true, HW>(vegas, hist, sm) : occ_had_t<T, false, true, HW>(vegas, hist, sm);
+    return grid ? occ_had_t<T, true, false, HW>(vegas, hist, sm) : occ_had_t<T, false, false, HW>(vegas, hist, sm);
 }
 
 int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid, bool heavy)
 {
     HadLaunch L{};
-    L.prec = prec; L.vegas = vegas;
+    L.prec = prec == MI_PREC_FP32_HW ? MI_PREC_FP32 : prec; L.vegas = vegas;
     Grid g; g.nbins = nbins; g.ndim = 3;
     L.grid = &g; L.hist = hist ? (double *)1 : nullptr;
     const size_t sm = had_smem(L);
+    if (prec == MI_PREC_FP32_HW) return occ_had_tt<float, true>(vegas, hist, sm, grid, heavy);
     if (prec == MI_PREC_FP32) return occ_had_tt<float>(vegas, hist, sm, grid, heavy);
     return occ_had_tt<double>(vegas, hist, sm, grid, heavy);
 }

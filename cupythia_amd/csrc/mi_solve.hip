@@ -41,7 +41,7 @@ int mi_solve(mi_ctx *ctx, const mi_run *run, const mi_solve_opts *opts_in, mi_si
     if (o.sampler != MI_SAMPLER_VEGAS && o.sampler != MI_SAMPLER_FLOW)
         return serr(ctx, MI_ERR_ARG, "mi_solve samples with MI_SAMPLER_VEGAS or MI_SAMPLER_FLOW");
     if (run->phys.mode != MI_MODE_HADRONIC_3D) return serr(ctx, MI_ERR_ARG, "mi_solve integrates the hadronic mode");
-    if (o.sampler == MI_SAMPLER_FLOW && run->precision != MI_PREC_FP32)
+    if (o.sampler == MI_SAMPLER_FLOW && run->precision == MI_PREC_FP64)
         return serr(ctx, MI_ERR_ARG, "the flow sampler evaluates FP32 weights: run->precision must be FP32");
     if (o.chunk_steps == 0) o.chunk_steps = 10;
     if (o.max_steps == 0) o.max_steps = 200;

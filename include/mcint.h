@@ -64,7 +64,14 @@ typedef struct mi_ctx mi_ctx;   /* opaque */
 #define MI_PROC_CODE(i)   ((i) < 6 ? 111 + (i) : 121 + (i) - 6)
 
 enum mi_mode      { MI_MODE_PARTONIC_1D = 0, MI_MODE_HADRONIC_3D = 1 };
-enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1 };
+/* MI_PREC_FP32: ME and phase space in FP32 with this library's own exp / log and IEEE division
+ * and square root -- every FP32 weight reproducible bit for bit on the host, which makes the
+ * unweighting decisions and compacted indices bit-exact.  MI_PREC_FP32_HW: the same FP32 algebra
+ * with the hardware exp2 / log2 / reciprocal / square root (v_exp_f32, v_log_f32, v_rcp_f32,
+ * v_sqrt_f32; ~1 ulp each) in mi_integrate, mi_integrate_noalloc, mi_points, mi_vegas_adapt and
+ * mi_solve's VEGAS passes; per-point weights within 1e-5 of the FP64 form, not bit-reproducible on
+ * the host.  Every other entry point runs an FP32_HW run as FP32.  Sums are FP64 in all modes. */
+enum mi_precision { MI_PREC_FP32 = 0, MI_PREC_FP64 = 1, MI_PREC_FP32_HW = 2 };
 enum mi_sampler   { MI_SAMPLER_FLAT = 0, MI_SAMPLER_VEGAS = 1, MI_SAMPLER_FLOW = 2 };
 enum mi_pdf_source { MI_PDF_TOY = 0, MI_PDF_GRID = 1 };
 #define MI_PDF_SLOTS      8    /* PDF grids a context holds: slot 0 (runs) + variation slots 1..7 */
