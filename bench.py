@@ -31,15 +31,17 @@ UNIT = "weighted events/s"
 # five processes).  Philox's integer work is excluded (reported by rocprof instead).
 FLOPS_PER_POINT = 180
 PEAK_FP32_TFLOPS = 157.3          # MI355X vector FP32 (vendor figure, counts packed FP32)
-PMC_FILE = os.path.join(ROOT, "profiles", "pmc_integrate_had_2p28.txt")
+PMC_FILES = {"fp32hw": os.path.join(ROOT, "profiles", "pmc_integrate_had_hw_2p28.txt"),
+             "fp32": os.path.join(ROOT, "profiles", "pmc_integrate_had_2p28.txt")}
 
 
-def traffic_from_profile():
+def traffic_from_profile(precision):
     """HBM bytes per launch of the integration kernel from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel calibration)."""
-    if not os.path.exists(PMC_FILE):
+    path = PMC_FILES[precision]
+    if not os.path.exists(path):
         return None
-    for line in open(PMC_FILE):
+    for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             try:
                 return float(line.split(":", 1)[1])
@@ -351,7 +353,7 @@ def main():
                      "kernel": "k_integrate_had<float,flat%s>" % (",hw" if args.precision == "fp32hw" else ""),
                      "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profile(), "flops_per_point": FLOPS_PER_POINT,
+                     "traffic": traffic_from_profile(args.precision), "flops_per_point": FLOPS_PER_POINT,
                      "avg_kernel_ms": avg_kernel_s * 1e3},
     }
     if not args.no_cpu:
