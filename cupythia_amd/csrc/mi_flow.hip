@@ -20,7 +20,7 @@
 // -E_q[w^2 grad log q(y)] (score form, samples held fixed), so the loss surrogate is
 // sum_i g_i log q(y_i) with g_i = -w_i^2 / (N <w>^2).  The initial flow is the identity (last
 // layers zero), so flow + VEGAS starts as VEGAS; the weight f / q stays the unbiased estimator.
-#include "mi_kernel_common.h"
+#include "mi_flow_common.h"
 
 #include <cmath>
 #include <cstring>
@@ -28,306 +28,6 @@
 #include <vector>
 
 namespace mi {
-
-constexpr int FL = MI_FLOW_LAYERS, FK = MI_FLOW_BINS, FH = MI_FLOW_HIDDEN, FO = MI_FLOW_NOUT;
-constexpr int FPL = MI_FLOW_NPARAMS / MI_FLOW_LAYERS;   // 7537 parameters per layer
-constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
-static_assert(PB3 + FO == FPL, "flow parameter layout");
-constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
-constexpr int HS = 72;      // LDS row stride (bf16) of 64-wide hidden activations (144 B rows)
-constexpr int GS = 68;      // LDS row stride (fp32) of 64-wide gradient rows
-constexpr int FWB = 4;      // waves per block of the backward layer kernel (one per SIMD)
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// Per-layer weights in the layout the kernels copy into LDS.
-struct PackedLayer {
-    float W1[2 * FH];          // BF16-rounded values, [j][i]
-    float b1[FH];
-    float b2[FH];
-    float b3[64];              // 49 used, rest 0
-    uint16_t W2[FH * FH];      // BF16 [out][in]
-    uint16_t W3[64 * FH];      // BF16 [o][h], rows 49..63 zero
-};
-static_assert(sizeof(PackedLayer) % 16 == 0, "PackedLayer copies as uint4");
-// LDS of the backward layer kernel: the staged layer and per wave H1, H2 (bf16, 64 x HS), G (fp32,
-// 64 x GS) and the conditioning coordinates (fp32, 64 x 2); one block of FWB = 4 waves per CU
-// (159 KB of the 160 KB).  After the tiles the block's four gradient rows reuse it: two rows in the
-// G buffers, one in the H1 buffers, one in the H2 buffers.
-constexpr size_t kBwdLds = sizeof(PackedLayer) + FWB * (sizeof(uint16_t) * 2 * 64 * HS + sizeof(float) * (64 * GS + 128));
-static_assert(FWB == 4 && kBwdLds <= 160 * 1024, "backward LDS layout");
-static_assert(2 * FPL * sizeof(float) <= sizeof(float) * FWB * 64 * GS &&
-              FPL * sizeof(float) <= sizeof(uint16_t) * FWB * 64 * HS, "gradient rows in the backward LDS");
-
-// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32 converts two at a time).
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
-{
-    const f32x2 v = {a, b};
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
-}
-__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
-__device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
-
-// Orders the LDS accesses of one wave around a point where lanes start reading what other lanes
-// of the same wave wrote (the LDS executes one wave's operations in issue order; this keeps the
-// compiler from moving them across).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Eight operand elements p[0], p[stride], ... as one MFMA fragment (FP32 sources are rounded).
-__device__ __forceinline__ bf16x8 frag8(const uint16_t *p, int stride)
-{
-    bf16x8 r;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, p[j * stride]);
-    return r;
-}
-__device__ __forceinline__ bf16x8 frag8(const float *p, int stride)
-{
-    uint4 w;
-    w.x = pack_bf16x2(p[0], p[stride]);
-    w.y = pack_bf16x2(p[2 * stride], p[3 * stride]);
-    w.z = pack_bf16x2(p[4 * stride], p[5 * stride]);
-    w.w = pack_bf16x2(p[6 * stride], p[7 * stride]);
-    return __builtin_bit_cast(bf16x8, w);
-}
-
-struct Acc { f32x16 c[2][2]; };
-
-__device__ __forceinline__ void acc_zero(Acc &a)
-{
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) a.c[i][j][v] = 0.0f;
-}
-
-// acc += A (64 x 64) * B (64 x 64), element (m, k) of A at A[m*a_sm + k*a_sk], (k, n) of B at
-// B[k*b_sk + n*b_sn].  32x32x16 MFMA operand layout: lane l holds A[l%32][8*(l/32) .. +7] and
-// B[8*(l/32) .. +7][l%32]; the result element (m, n) of a 32x32 block sits in lane
-// n + 32*((m/4)%2), register 4*(m/8) + m%4.
-template <typename EA, typename EB>
-__device__ __forceinline__ void wgemm(Acc &acc, const EA *A, int a_sm, int a_sk, const EB *B, int b_sk, int b_sn)
-{
-    const int l = threadIdx.x & 63, r = l & 31, hh = l >> 5;
-#pragma unroll 1
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k0 = 16 * kk + 8 * hh;
-        const bf16x8 a0 = frag8(A + r * a_sm + k0 * a_sk, a_sk);
-        const bf16x8 a1 = frag8(A + (32 + r) * a_sm + k0 * a_sk, a_sk);
-        const bf16x8 b0 = frag8(B + k0 * b_sk + r * b_sn, b_sk);
-        const bf16x8 b1 = frag8(B + k0 * b_sk + (32 + r) * b_sn, b_sk);
-        acc.c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc.c[0][0], 0, 0, 0);
-        acc.c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc.c[0][1], 0, 0, 0);
-        acc.c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc.c[1][0], 0, 0, 0);
-        acc.c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc.c[1][1], 0, 0, 0);
-    }
-}
-
-template <typename F>
-__device__ __forceinline__ void acc_each(const Acc &a, F fn)
-{
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int m = 32 * mi + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
-                const int n = 32 * ni + (l & 31);
-                fn(m, n, a.c[mi][ni][v]);
-            }
-}
-
-// acc holds C = A W^T for A (64 x 64, row m at A + m*as, k contiguous) and W (64 x 64, row n at
-// W + n*ws, k contiguous), computed with the operands swapped so that the result comes out
-// transposed: lane l owns row m = 32*mb + (l & 31) and the columns n = 32*nb + 8q + 4*(l/32) + r,
-// q, r = 0..3, in acc.c[mb][nb][4q + r] -- four consecutive columns per group, which the stores
-// below write as one 8-byte (bf16) or 16-byte (fp32) LDS access.
-template <typename EW>
-__device__ __forceinline__ void wgemm_nt(Acc &acc, const uint16_t *A, int as, const EW *W, int ws)
-{
-    const int l = threadIdx.x & 63, r = l & 31, hh = l >> 5;
-#pragma unroll 1
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k0 = 16 * kk + 8 * hh;
-        const bf16x8 a0 = frag8(A + r * as + k0, 1);
-        const bf16x8 a1 = frag8(A + (32 + r) * as + k0, 1);
-        const bf16x8 w0 = frag8(W + r * ws + k0, 1);
-        const bf16x8 w1 = frag8(W + (32 + r) * ws + k0, 1);
-        acc.c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a0, acc.c[0][0], 0, 0, 0);
-        acc.c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a0, acc.c[0][1], 0, 0, 0);
-        acc.c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a1, acc.c[1][0], 0, 0, 0);
-        acc.c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a1, acc.c[1][1], 0, 0, 0);
-    }
-}
-
-// Starts a wgemm_nt accumulation at the bias: C[m][n] = b[n].
-__device__ __forceinline__ void acc_bias_nt(Acc &a, const float *b)
-{
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 v = *reinterpret_cast<const float4 *>(b + 32 * nb + 8 * q + 4 * (l >> 5));
-                a.c[mb][nb][4 * q] = v.x; a.c[mb][nb][4 * q + 1] = v.y;
-                a.c[mb][nb][4 * q + 2] = v.z; a.c[mb][nb][4 * q + 3] = v.w;
-            }
-}
-
-// H[m][n] = bf16(relu(C[m][n])) from a wgemm_nt result (bias included in C).
-__device__ __forceinline__ void store_nt_bf16_relu(const Acc &a, uint16_t *H)
-{
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int m = 32 * mb + (l & 31), n = 32 * nb + 8 * q + 4 * (l >> 5);
-                uint32_t w[2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    w[t] = pack_bf16x2(fmaxf(a.c[mb][nb][4 * q + 2 * t], 0.0f), fmaxf(a.c[mb][nb][4 * q + 2 * t + 1], 0.0f));
-                *reinterpret_cast<uint2 *>(H + m * HS + n) = make_uint2(w[0], w[1]);
-            }
-}
-
-// O[m][n] = C[m][n] for the column groups with n + 3 < ncol (row stride os, a multiple of 4).
-__device__ __forceinline__ void store_nt_f32(const Acc &a, float *O, int os, int ncol)
-{
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int m = 32 * mb + (l & 31), n = 32 * nb + 8 * q + 4 * (l >> 5);
-                if (n + 3 < ncol) {
-                    float4 v;
-                    v.x = a.c[mb][nb][4 * q];
-                    v.y = a.c[mb][nb][4 * q + 1];
-                    v.z = a.c[mb][nb][4 * q + 2];
-                    v.w = a.c[mb][nb][4 * q + 3];
-                    *reinterpret_cast<float4 *>(O + m * os + n) = v;
-                }
-            }
-}
-
-__device__ __forceinline__ void stage_layer(PackedLayer &dst, const PackedLayer *src)
-{
-    const uint4 *s = reinterpret_cast<const uint4 *>(src);
-    uint4 *d = reinterpret_cast<uint4 *>(&dst);
-    for (int i = threadIdx.x; i < (int)(sizeof(PackedLayer) / 16); i += blockDim.x) d[i] = s[i];
-    __syncthreads();
-}
-
-// Element i (0..2) of a 3-vector by selects, so that a per-launch index keeps the vector in registers.
-__device__ __forceinline__ float pick3(const float *v, int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
-__device__ __forceinline__ void set3(float *v, int i, float x)
-{
-    v[0] = i == 0 ? x : v[0];
-    v[1] = i == 1 ? x : v[1];
-    v[2] = i == 2 ? x : v[2];
-}
-__device__ __forceinline__ int cond0(int d) { return d == 0 ? 1 : 0; }
-__device__ __forceinline__ int cond1(int d) { return d == 2 ? 1 : 2; }
-
-// First layer of one 64-point tile (lane = point): H1[lane][j] = bf16(relu(W1[j] . c + b1[j])),
-// four hidden units per 8-byte store.
-__device__ __forceinline__ void layer1(const PackedLayer &P, float c0, float c1, uint16_t *H1)
-{
-    const int p = threadIdx.x & 63;
-#pragma unroll 4
-    for (int j = 0; j < FH; j += 4) {
-        uint32_t w[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int j0 = j + 2 * t, j1 = j0 + 1;
-            const float h0 = fmaxf(__builtin_fmaf(P.W1[2 * j0], c0, __builtin_fmaf(P.W1[2 * j0 + 1], c1, P.b1[j0])), 0.0f);
-            const float h1 = fmaxf(__builtin_fmaf(P.W1[2 * j1], c0, __builtin_fmaf(P.W1[2 * j1 + 1], c1, P.b1[j1])), 0.0f);
-            w[t] = pack_bf16x2(h0, h1);
-        }
-        *reinterpret_cast<uint2 *>(H1 + p * HS + j) = make_uint2(w[0], w[1]);
-    }
-}
-
-// MLP of one 64-point tile: H2 = relu(H1 W2^T + b2) (bf16), O = H2 W3^T + b3 (fp32, the first
-// ocol columns, row stride os).  H2 may alias H1 and O may alias both: each buffer is written
-// only after the GEMM that reads the previous one has issued all its LDS reads.
-__device__ __forceinline__ void mlp_tile(const PackedLayer &P, float c0, float c1, uint16_t *H1, uint16_t *H2,
-                                         float *O, int os, int ocol)
-{
-    layer1(P, c0, c1, H1);
-    wave_sync();
-    Acc a;
-    acc_bias_nt(a, P.b2);
-    wgemm_nt(a, H1, HS, P.W2, FH);
-    wave_sync();
-    store_nt_bf16_relu(a, H2);
-    wave_sync();
-    acc_bias_nt(a, P.b3);
-    wgemm_nt(a, H2, HS, P.W3, FH);
-    wave_sync();
-    store_nt_f32(a, O, os, ocol);
-    wave_sync();
-}
-
-// Spline parameters of one point from its 49 outputs: softmax widths / heights (each exponential
-// once, one reciprocal per softmax), softplus knot derivatives; locate the bin of x (forward) or
-// y (inverse).
-struct Bin { int k; float xk, wk, yk, hk, dk, dk1; };
-
-__device__ __forceinline__ float softplusf(float x) { return fmaxf(x, 0.0f) + log1pf(__expf(-fabsf(x))); }
-
-__device__ __forceinline__ void softmax16(const float *o, float *p)
-{
-    float m = o[0];
-#pragma unroll
-    for (int i = 1; i < FK; ++i) m = fmaxf(m, o[i]);
-    float s = 0.0f;
-#pragma unroll
-    for (int i = 0; i < FK; ++i) { p[i] = __expf(o[i] - m); s += p[i]; }
-    const float r = 1.0f / s;
-#pragma unroll
-    for (int i = 0; i < FK; ++i) p[i] *= r;
-}
-
-__device__ __forceinline__ Bin locate(const float *o, const float *pw, const float *ph, float v, bool by_y)
-{
-    const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
-    Bin b;
-    b.k = 0; b.xk = 0.0f; b.yk = 0.0f;
-    b.wk = kMinW + cw * pw[0];
-    b.hk = kMinH + ch * ph[0];
-    float cx = b.wk, cy = b.hk;
-#pragma unroll
-    for (int i = 1; i < FK; ++i) {
-        const float wi = kMinW + cw * pw[i];
-        const float hi = kMinH + ch * ph[i];
-        if (by_y ? v >= cy : v >= cx) { b.k = i; b.xk = cx; b.yk = cy; b.wk = wi; b.hk = hi; }
-        cx += wi;
-        cy += hi;
-    }
-    b.dk = kMinD + softplusf(o[2 * FK + b.k]);
-    b.dk1 = kMinD + softplusf(o[2 * FK + b.k + 1]);
-    return b;
-}
 
 // ------------------------------------------------------------------------------------ forward / inverse
 // The forward / inverse kernel runs all FL layers of a 64-point tile (lane = point) with the
@@ -340,58 +40,6 @@ __device__ __forceinline__ Bin locate(const float *o, const float *pw, const flo
 // the two conditioners enter split into three BF16 parts each (exact), the bias b1 likewise
 // against a constant-one operand.  The layer-3 outputs are brought to the point's own lane with
 // v_permlane32_swap, and the spline runs on them in registers.
-__host__ __device__ constexpr int swz(int k) { return (k & ~12) | ((k & 4) << 1) | ((k & 8) >> 1); }
-
-// Weights of one layer as v_mfma_f32_32x32x16_bf16 A operands, one 16-byte fragment per lane
-// (lane l: row 32 mb + l % 32, K elements 8 (l / 32) .. + 7), laid out [K step][half][row] so
-// that a wave's fragment load reads 1 KB of consecutive LDS.
-struct FwdLayer {
-    uint4 W1f[2][64];        // [0][m]: (W1[m][0], W1[m][1]) x 3, 0, 0; [1][m]: b1[m] in 3 BF16 parts, 0 x 5
-    uint4 W2f[4][2][64];     // [kk][hh][m]: W2[m][swz(16 kk + 8 hh + j)], j = 0..7
-    uint4 W3f[4][2][64];     // the same for W3; rows 49..63 zero
-    float b2[64], b3[64];    // b3: 49 used, rest 0
-};
-static_assert(sizeof(FwdLayer) % 16 == 0, "FwdLayer copies as uint4");
-constexpr int FWV = 8;      // waves per block of the forward / inverse kernel (one block per CU)
-
-__device__ __forceinline__ bf16x8 as_frag(uint4 w) { return __builtin_bit_cast(bf16x8, w); }
-
-// bf16(relu(a)), bf16(relu(b)) packed: ReLU on the BF16 bit patterns as a signed 16-bit maximum
-// with 0 (negative values and -0 have the sign bit set).
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b)
-{
-    const s16x2 v = __builtin_bit_cast(s16x2, pack_bf16x2(a, b));
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, (s16x2){0, 0}));
-}
-// B operand of K step kk from an accumulator block holding units 32 (kk / 2) + ..: registers
-// 8 (kk % 2) .. + 7.
-__device__ __forceinline__ bf16x8 relu_frag(const f32x16 &c, int base)
-{
-    uint4 w;
-    w.x = relu_bf16x2(c[base], c[base + 1]);
-    w.y = relu_bf16x2(c[base + 2], c[base + 3]);
-    w.z = relu_bf16x2(c[base + 4], c[base + 5]);
-    w.w = relu_bf16x2(c[base + 6], c[base + 7]);
-    return __builtin_bit_cast(bf16x8, w);
-}
-// Accumulator block of rows 32 mb .. + 31 started at the bias: register 4 i + r = b[32 mb + 8 i + 4 hh + r].
-__device__ __forceinline__ f32x16 bias_block(const float *b, int mb, int hh)
-{
-    f32x16 c;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float4 v = *reinterpret_cast<const float4 *>(b + 32 * mb + 8 * i + 4 * hh);
-        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
-    }
-    return c;
-}
-__device__ __forceinline__ uint32_t swap_lo(uint32_t v)
-{
-    // the value of lane l ^ 32 in lanes 0..31 (lanes 32..63 keep their own)
-    const auto t = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-    return t[1];
-}
 
 // MLP outputs o[0 .. FO-1] of the point of this lane for conditioners (c0, c1).
 __device__ __forceinline__ void chain_mlp(const FwdLayer &W, float c0, float c1, float (&o)[FO])
@@ -399,21 +47,14 @@ __device__ __forceinline__ void chain_mlp(const FwdLayer &W, float c0, float c1,
     const int lane = threadIdx.x & 63, hh = lane >> 5, r32 = lane & 31;
     // ---- layer 1: K slots (c0, c1) x (hi, mid, lo) against (W1[m][0], W1[m][1]) x 3, and
     // (1, 1, 1) against the three parts of b1
-    const uint32_t p = pack_bf16x2(c0, c1);
-    const float e0 = c0 - __uint_as_float(p << 16), e1 = c1 - __uint_as_float(p & 0xFFFF0000u);
-    const uint32_t q = pack_bf16x2(e0, e1);
-    const uint32_t s = pack_bf16x2(e0 - __uint_as_float(q << 16), e1 - __uint_as_float(q & 0xFFFF0000u));
-    const uint32_t ps = swap_lo(p), qs = swap_lo(q), ss = swap_lo(s);
-    const bool lo = hh == 0;
-    const uint4 b0 = lo ? make_uint4(p, q, s, 0u) : make_uint4(0x3F803F80u, 0x3F80u, 0u, 0u);
-    const uint4 b1 = lo ? make_uint4(ps, qs, ss, 0u) : make_uint4(0x3F803F80u, 0x3F80u, 0u, 0u);
+    const CondFrag cf = cond_frag(c0, c1);
     f32x16 C[2][2];
     const f32x16 zero = {};
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
         const bf16x8 a = as_frag(W.W1f[hh][32 * mb + r32]);
-        C[mb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, as_frag(b0), zero, 0, 0, 0);
-        C[mb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, as_frag(b1), zero, 0, 0, 0);
+        C[mb][0] = mfma16(a, as_frag(cf.frag[0]), zero);
+        C[mb][1] = mfma16(a, as_frag(cf.frag[1]), zero);
     }
     // ---- layers 2 and 3
 #pragma unroll
@@ -432,41 +73,12 @@ __device__ __forceinline__ void chain_mlp(const FwdLayer &W, float c0, float c1,
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
                 const bf16x8 a = as_frag(Wf[kk][hh][32 * mb + r32]);
-                C[mb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kk][0], C[mb][0], 0, 0, 0);
-                C[mb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[kk][1], C[mb][1], 0, 0, 0);
+                C[mb][0] = mfma16(a, B[kk][0], C[mb][0]);
+                C[mb][1] = mfma16(a, B[kk][1], C[mb][1]);
             }
         }
     }
-    // ---- outputs to the point's lane: block (mb, nb) register v = 4 i + r holds output
-    // 32 mb + 8 i + 4 hh + r of point 32 nb + l % 32; swapping the upper half of nb = 0 with the
-    // lower half of nb = 1 leaves output 32 mb + 8 i + r of the lane's own point in the first
-    // register and output 32 mb + 8 i + 4 + r in the second.
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int oa = 32 * mb + 8 * (v >> 2) + (v & 3), ob = oa + 4;
-            if (oa < FO) {
-                const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(C[mb][0][v]),
-                                                                __float_as_uint(C[mb][1][v]), false, false);
-                o[oa] = __uint_as_float(t[0]);
-                if (ob < FO) o[ob] = __uint_as_float(t[1]);
-            }
-        }
-}
-
-__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
-__device__ __forceinline__ float log_hw(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
-__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// softplus(x) = max(x, 0) + log1p(e^-|x|), log1p(e) as log(u) e / (u - 1) with u = 1 + e (exact
-// to a few ulp where u rounds; e itself where u == 1).
-__device__ __forceinline__ float softplus_hw(float x)
-{
-    const float e = exp2_hw(-fabsf(x) * 1.4426950408889634f);
-    const float u = 1.0f + e;
-    const float l = u == 1.0f ? e : log_hw(u) * (e * rcp_hw(u - 1.0f));
-    return fmaxf(x, 0.0f) + l;
+    rows_to_lane(C, o);
 }
 
 // Spline of one point from its MLP outputs (widths o[0..15], heights o[16..31], knot-derivative
@@ -603,203 +215,14 @@ __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restr
     }
 }
 
-// ------------------------------------------------------------------------------------ backward
-// Adjoint of one layer for L = sum_i (abar_out_i . x_out_i + g_i log S'_i) summed with the
-// later layers' contributions: abar (n x 3) holds dL/d(output coords) on entry and dL/d(input
-// coords) on exit.  FWB waves per block, each with its own tiles; the weight gradients dW2, dW3
-// accumulate in the matrix-core accumulators of the wave across all its tiles.  At the end the
-// waves write their parameter gradients into the block's LDS and the block writes their sum
-// (in wave order) to row blockIdx.x of gp (rows x FPL).
-__global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const PackedLayer *__restrict__ layers, int l, uint64_t n,
-                                                             const float *__restrict__ xin, const float *__restrict__ g,
-                                                             float *__restrict__ abar, float *__restrict__ gp)
-{
-    __shared__ __align__(16) PackedLayer P;
-    __shared__ __align__(16) uint16_t H1s[FWB][64 * HS];
-    __shared__ __align__(16) uint16_t H2s[FWB][64 * HS];      // H2, then dH2 (bf16)
-    __shared__ __align__(16) float Gs[FWB][64 * GS];          // MLP outputs, then dO, then dH1
-    __shared__ float Ccs[FWB][128];
-    stage_layer(P, layers + l);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t *H1 = H1s[wave], *H2 = H2s[wave];
-    float *G = Gs[wave], *Cc = Ccs[wave];
-    const int d = l % 3, ca = cond0(d), cb = cond1(d);
-    Acc gW2, gW3;
-    acc_zero(gW2);
-    acc_zero(gW3);
-    // bias / first-layer sums of the two columns lane & 31 and 32 + (lane & 31) this lane holds
-    // in the standard accumulator layout (the other half of the rows sits in lane ^ 32)
-    float gb1[2] = {0.0f, 0.0f}, gw10[2] = {0.0f, 0.0f}, gw11[2] = {0.0f, 0.0f}, gb2[2] = {0.0f, 0.0f};
-    float gb3 = 0.0f;
-    const float cw = 1.0f - FK * kMinW, ch = 1.0f - FK * kMinH;
-    const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = (uint64_t)blockIdx.x * FWB + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWB) {
-        const uint64_t i = tile * 64 + lane;
-        const bool live = i < n;
-        float x[3] = {0.5f, 0.5f, 0.5f}, ab[3] = {0.0f, 0.0f, 0.0f}, gi = 0.0f;
-        if (live) {
-            x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2];
-            ab[0] = abar[3 * i]; ab[1] = abar[3 * i + 1]; ab[2] = abar[3 * i + 2];
-            gi = g[i];
-        }
-        Cc[2 * lane] = pick3(x, ca);
-        Cc[2 * lane + 1] = pick3(x, cb);
-        mlp_tile(P, pick3(x, ca), pick3(x, cb), H1, H2, G, GS, 64);
-        // ---- spline backward (lane = point), dO written over the outputs
-        {
-            float *o = G + lane * GS;
-            float pw[FK], ph[FK];
-            softmax16(o, pw);
-            softmax16(o + FK, ph);
-            const float xd = pick3(x, d);
-            const Bin b = locate(o, pw, ph, xd, false);
-            const float iw = 1.0f / b.wk;
-            const float xi = (xd - b.xk) * iw, s = b.hk * iw, t = xi * (1.0f - xi);
-            const float A = b.dk1 + b.dk - 2.0f * s, den = s + A * t;
-            const float iden = 1.0f / den, iden2 = iden * iden;
-            const float P2 = s * xi * xi + b.dk * t, R = P2 * iden;
-            const float n2 = b.dk1 * xi * xi + 2.0f * s * t + b.dk * (1.0f - xi) * (1.0f - xi);
-            const float in2 = 1.0f / n2;
-            const float aS = live ? pick3(ab, d) : 0.0f, gL = live ? gi : 0.0f;
-            const float omx = 1.0f - xi, om2t = 1.0f - 2.0f * t;
-            // d R / d (xi, s, dk, dk1)
-            const float Pxi = 2.0f * s * xi + b.dk * (1.0f - 2.0f * xi), denxi = A * (1.0f - 2.0f * xi);
-            const float R_xi = (Pxi * den - P2 * denxi) * iden2;
-            const float R_s = (xi * xi * den - P2 * om2t) * iden2;
-            const float R_dk = t * (den - P2) * iden2;
-            const float R_dk1 = -P2 * t * iden2;
-            // d log S' / d (xi, s, dk, dk1)
-            const float n2xi = 2.0f * b.dk1 * xi + 2.0f * s * (1.0f - 2.0f * xi) - 2.0f * b.dk * omx;
-            const float L_xi = n2xi * in2 - 2.0f * denxi * iden;
-            const float L_s = 2.0f / s + 2.0f * t * in2 - 2.0f * om2t * iden;
-            const float L_dk = omx * omx * in2 - 2.0f * t * iden;
-            const float L_dk1 = xi * xi * in2 - 2.0f * t * iden;
-            const float hR = aS * b.hk;
-            const float Gxi = hR * R_xi + gL * L_xi, Gs_ = hR * R_s + gL * L_s;
-            const float Gdk = hR * R_dk + gL * L_dk, Gdk1 = hR * R_dk1 + gL * L_dk1;
-            const float dx = Gxi * iw;                    // dL / d x_in[d]
-            const float dxk = -dx;
-            const float dwk = -(Gxi * xi + Gs_ * s) * iw;
-            const float dhk = aS * R + Gs_ * iw;
-            const float dyk = aS;
-            // widths / heights: cumulative sums, then the softmax Jacobian
-            float dotw = 0.0f, doth = 0.0f;
-#pragma unroll
-            for (int q = 0; q < FK; ++q) {
-                const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
-                const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
-                dotw += pw[q] * gwq;
-                doth += ph[q] * ghq;
-            }
-            // knot-derivative logits (read before the row is overwritten)
-            const float zk = o[2 * FK + b.k], zk1 = o[2 * FK + b.k + 1];
-            const float sgk = 1.0f / (1.0f + __expf(-zk)), sgk1 = 1.0f / (1.0f + __expf(-zk1));
-#pragma unroll
-            for (int q = 0; q < FK; ++q) {
-                const float gwq = cw * ((q < b.k ? dxk : 0.0f) + (q == b.k ? dwk : 0.0f));
-                const float ghq = ch * ((q < b.k ? dyk : 0.0f) + (q == b.k ? dhk : 0.0f));
-                o[q] = live ? pw[q] * (gwq - dotw) : 0.0f;
-                o[FK + q] = live ? ph[q] * (ghq - doth) : 0.0f;
-            }
-#pragma unroll
-            for (int q = 2 * FK; q < 64; ++q) o[q] = 0.0f;
-            if (live) {
-                o[2 * FK + b.k] = Gdk * sgk;
-                o[2 * FK + b.k + 1] = Gdk1 * sgk1;
-            }
-            set3(ab, d, live ? dx : 0.0f);
-        }
-        wave_sync();
-        // ---- MLP backward
-#pragma unroll 8
-        for (int p = 0; p < 64; ++p) gb3 += G[p * GS + lane];                // columns >= 49 are 0
-        wgemm(gW3, G, 1, GS, H2, HS, 1);                                     // dW3 += dO^T H2
-        Acc a;
-        acc_zero(a);
-        wgemm(a, G, GS, 1, P.W3, FH, 1);                                     // dH2 = dO W3
-        wave_sync();
-        acc_each(a, [&](int m, int nn, float v) {                            // masked by H2 > 0, over H2
-            const float dv = (H2[m * HS + nn] != 0) ? v : 0.0f;
-            gb2[nn >> 5] += dv;
-            H2[m * HS + nn] = f2bf(dv);
-        });
-        wave_sync();
-        wgemm(gW2, H2, 1, HS, H1, HS, 1);                                    // dW2 += dH2^T H1
-        acc_zero(a);
-        wgemm(a, H2, HS, 1, P.W2, FH, 1);                                    // dH1 = dH2 W2
-        wave_sync();
-        acc_each(a, [&](int m, int nn, float v) {                            // masked by H1 > 0, over dO
-            const float dv = (H1[m * HS + nn] != 0) ? v : 0.0f;
-            gb1[nn >> 5] += dv;
-            gw10[nn >> 5] += dv * Cc[2 * m];
-            gw11[nn >> 5] += dv * Cc[2 * m + 1];
-            G[m * GS + nn] = dv;
-        });
-        wave_sync();
-        float dc0 = 0.0f, dc1 = 0.0f;
-#pragma unroll 8
-        for (int j = 0; j < FH; ++j) {
-            const float v = G[lane * GS + j];
-            dc0 += v * P.W1[2 * j];
-            dc1 += v * P.W1[2 * j + 1];
-        }
-        if (live) {
-            set3(ab, ca, pick3(ab, ca) + dc0);
-            set3(ab, cb, pick3(ab, cb) + dc1);
-            abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
-        }
-        wave_sync();                                   // row reads done before the next tile
-    }
-    // lanes l and l ^ 32 hold the two halves of the rows of the same columns
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        gb1[h] += __shfl_xor(gb1[h], 32, 64);
-        gw10[h] += __shfl_xor(gw10[h], 32, 64);
-        gw11[h] += __shfl_xor(gw11[h], 32, 64);
-        gb2[h] += __shfl_xor(gb2[h], 32, 64);
-    }
-    __syncthreads();                                   // every wave is past its tiles: LDS holds the rows now
-    // the FWB = 4 gradient rows of the block: two in Gs, one each in H1s and H2s
-    auto rowp = [&](int w) -> float * {
-        return w < 2 ? &Gs[0][0] + (size_t)w * FPL : reinterpret_cast<float *>(w == 2 ? &H1s[0][0] : &H2s[0][0]);
-    };
-    float *out = rowp(wave);
-    if (lane < 32) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = 32 * h + lane;
-            out[PW1 + 2 * c] = gw10[h];
-            out[PW1 + 2 * c + 1] = gw11[h];
-            out[PB1 + c] = gb1[h];
-            out[PB2 + c] = gb2[h];
-        }
-    }
-    if (lane < FO) out[PB3 + lane] = gb3;
-    acc_each(gW2, [&](int m, int nn, float v) { out[PW2 + m * FH + nn] = v; });
-    acc_each(gW3, [&](int m, int nn, float v) { if (m < FO) out[PW3 + m * FH + nn] = v; });
-    __syncthreads();
-    float *dst = gp + (size_t)blockIdx.x * FPL;
-    for (int j = threadIdx.x; j < FPL; j += 64 * FWB) {
-        float t = rowp(0)[j];
-#pragma unroll
-        for (int w = 1; w < FWB; ++w) t += rowp(w)[j];
-        dst[j] = t;
-    }
-}
-
 // ------------------------------------------------------------------------------------ small kernels
-__global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__restrict__ out, FwdLayer *__restrict__ fwd)
+// The weight images of layer blockIdx.x from the FP32 parameters (BF16, round to nearest even).
+__global__ void k_flow_pack(const float *__restrict__ params, FwdLayer *__restrict__ fwd, BwdExtra *__restrict__ bwd)
 {
     const int l = blockIdx.x;
     const float *p = params + (size_t)l * FPL;
-    PackedLayer &q = out[l];
-    for (int i = threadIdx.x; i < 2 * FH; i += blockDim.x) q.W1[i] = bf2f(f2bf(p[PW1 + i]));
-    for (int i = threadIdx.x; i < FH; i += blockDim.x) { q.b1[i] = p[PB1 + i]; q.b2[i] = p[PB2 + i]; }
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) q.b3[i] = i < FO ? p[PB3 + i] : 0.0f;
-    for (int i = threadIdx.x; i < FH * FH; i += blockDim.x) q.W2[i] = f2bf(p[PW2 + i]);
-    for (int i = threadIdx.x; i < 64 * FH; i += blockDim.x) q.W3[i] = (i / FH) < FO ? f2bf(p[PW3 + i]) : (uint16_t)0;
-    // the forward / inverse kernel's fragments
     FwdLayer &f = fwd[l];
+    BwdExtra &e = bwd[l];
     for (int m = threadIdx.x; m < FH; m += blockDim.x) {
         const uint32_t w = (uint32_t)f2bf(p[PW1 + 2 * m]) | ((uint32_t)f2bf(p[PW1 + 2 * m + 1]) << 16);
         f.W1f[0][m] = make_uint4(w, w, w, 0u);
@@ -822,6 +245,26 @@ __global__ void k_flow_pack(const float *__restrict__ params, PackedLayer *__res
         }
         f.W2f[kk][hh][m] = make_uint4(w2[0], w2[1], w2[2], w2[3]);
         f.W3f[kk][hh][m] = make_uint4(w3[0], w3[1], w3[2], w3[3]);
+        // transposed: column m of W2 / W3 with its rows permuted
+        uint32_t t2[4], t3[4];
+        for (int j = 0; j < 8; j += 2) {
+            const int ka = swz(16 * kk + 8 * hh + j), kb = swz(16 * kk + 8 * hh + j + 1);
+            t2[j / 2] = (uint32_t)f2bf(p[PW2 + ka * FH + m]) | ((uint32_t)f2bf(p[PW2 + kb * FH + m]) << 16);
+            const uint32_t lo = ka < FO ? (uint32_t)f2bf(p[PW3 + ka * FH + m]) : 0u;
+            const uint32_t hi = kb < FO ? (uint32_t)f2bf(p[PW3 + kb * FH + m]) : 0u;
+            t3[j / 2] = lo | (hi << 16);
+        }
+        e.W2Tf[kk][hh][m] = make_uint4(t2[0], t2[1], t2[2], t2[3]);
+        e.W3Tf[kk][hh][m] = make_uint4(t3[0], t3[1], t3[2], t3[3]);
+    }
+    for (int idx = threadIdx.x; idx < 4 * 2 * 2; idx += blockDim.x) {
+        const int kk = idx >> 2, hh = (idx >> 1) & 1, c = idx & 1;
+        uint32_t t1[4];
+        for (int j = 0; j < 8; j += 2) {
+            const int ka = swz(16 * kk + 8 * hh + j), kb = swz(16 * kk + 8 * hh + j + 1);
+            t1[j / 2] = (uint32_t)f2bf(p[PW1 + 2 * ka + c]) | ((uint32_t)f2bf(p[PW1 + 2 * kb + c]) << 16);
+        }
+        e.W1Tf[kk][hh][c] = make_uint4(t1[0], t1[1], t1[2], t1[3]);
     }
 }
 
@@ -940,8 +383,8 @@ namespace mi {
 
 struct FlowState {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
-    PackedLayer *packed = nullptr;  // the backward kernel's layout
-    FwdLayer *fwd = nullptr;        // the forward / inverse kernel's layout
+    FwdLayer *fwd = nullptr;        // weight images of the kernels (k_flow_pack)
+    BwdExtra *bwd = nullptr;
     // work buffers sized for `cap` points
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
@@ -1063,8 +506,8 @@ void mi_flow_release(mi_ctx *ctx)
 {
     if (!ctx || !ctx->flow) return;
     FlowState &F = *ctx->flow;
-    (void)hipFree(F.params); (void)hipFree(F.m); (void)hipFree(F.v); (void)hipFree(F.grad); (void)hipFree(F.packed);
-    (void)hipFree(F.fwd);
+    (void)hipFree(F.params); (void)hipFree(F.m); (void)hipFree(F.v); (void)hipFree(F.grad);
+    (void)hipFree(F.fwd); (void)hipFree(F.bwd);
     (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
     (void)hipFree(F.gp); (void)hipFree(F.hist); (void)hipFree(F.acc);
     delete ctx->flow;
@@ -1083,14 +526,14 @@ int mi_flow_set_params(mi_ctx *ctx, const float *params_host)
         FCHK(hipMalloc(&F.m, sizeof(float) * MI_FLOW_NPARAMS));
         FCHK(hipMalloc(&F.v, sizeof(float) * MI_FLOW_NPARAMS));
         FCHK(hipMalloc(&F.grad, sizeof(float) * MI_FLOW_NPARAMS));
-        FCHK(hipMalloc(&F.packed, sizeof(PackedLayer) * FL));
         FCHK(hipMalloc(&F.fwd, sizeof(FwdLayer) * FL));
+        FCHK(hipMalloc(&F.bwd, sizeof(BwdExtra) * FL));
         FCHK(hipMalloc(&F.acc, sizeof(double) * kRec));
     }
     FCHK(hipMemcpyAsync(F.params, params_host, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyHostToDevice, ctx->stream));
     FCHK(hipMemsetAsync(F.m, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
     FCHK(hipMemsetAsync(F.v, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
-    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed, F.fwd);
+    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(ctx->stream));
     F.step = 0;
@@ -1170,9 +613,8 @@ static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev)
     const uint32_t rows = blocks;
     if (rows > F.gp_rows) return ferr(ctx, MI_ERR_STATE, "gradient partials buffer too small");
     for (int l = FL - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64 * FWB), 0, ctx->stream, F.packed, l, n,
-                           F.xs + (size_t)3 * F.cap * l, g_dev, F.abar, F.gp + (size_t)l * F.gp_rows * FPL);
-        FCHK(hipGetLastError());
+        FCHK(launch_flow_bwd(ctx->stream, blocks, F.fwd, F.bwd, l, n, F.xs + (size_t)3 * F.cap * l, g_dev, F.abar,
+                             F.gp + (size_t)l * F.gp_rows * FPL));
     }
     hipLaunchKernelGGL(k_grad_sum, dim3((FL * FPL + 63) / 64), dim3(256), 0, ctx->stream, F.gp, rows, F.gp_rows,
                        F.grad);
@@ -1282,7 +724,7 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
         hipLaunchKernelGGL(k_adam, dim3((MI_FLOW_NPARAMS + 255) / 256), dim3(256), 0, ctx->stream, F.params, F.m, F.v,
                            F.grad, MI_FLOW_NPARAMS, (float)lr, b1, b2, 1e-8f, c1, c2);
         FCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.packed, F.fwd);
+        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
         FCHK(hipGetLastError());
     }
     FCHK(hipEventRecord(ctx->ev1, ctx->stream));
