@@ -217,13 +217,15 @@ __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restr
 
 // ------------------------------------------------------------------------------------ small kernels
 // The weight images of layer blockIdx.x from the FP32 parameters (BF16, round to nearest even).
+constexpr int kPackSplit = 8;   // blocks per layer of the pack kernel (blockIdx.y)
 __global__ void k_flow_pack(const float *__restrict__ params, FwdLayer *__restrict__ fwd, BwdExtra *__restrict__ bwd)
 {
     const int l = blockIdx.x;
+    const int t0 = blockIdx.y * blockDim.x + threadIdx.x, ts = gridDim.y * blockDim.x;
     const float *p = params + (size_t)l * FPL;
     FwdLayer &f = fwd[l];
     BwdExtra &e = bwd[l];
-    for (int m = threadIdx.x; m < FH; m += blockDim.x) {
+    for (int m = t0; m < FH; m += ts) {
         const uint32_t w = (uint32_t)f2bf(p[PW1 + 2 * m]) | ((uint32_t)f2bf(p[PW1 + 2 * m + 1]) << 16);
         f.W1f[0][m] = make_uint4(w, w, w, 0u);
         const float b = p[PB1 + m];
@@ -235,7 +237,7 @@ __global__ void k_flow_pack(const float *__restrict__ params, FwdLayer *__restri
         f.b2[m] = p[PB2 + m];
         f.b3[m] = m < FO ? p[PB3 + m] : 0.0f;
     }
-    for (int idx = threadIdx.x; idx < 4 * 2 * 64; idx += blockDim.x) {
+    for (int idx = t0; idx < 4 * 2 * 64; idx += ts) {
         const int kk = idx >> 7, hh = (idx >> 6) & 1, m = idx & 63;
         uint32_t w2[4], w3[4];
         for (int j = 0; j < 8; j += 2) {
@@ -257,7 +259,7 @@ __global__ void k_flow_pack(const float *__restrict__ params, FwdLayer *__restri
         e.W2Tf[kk][hh][m] = make_uint4(t2[0], t2[1], t2[2], t2[3]);
         e.W3Tf[kk][hh][m] = make_uint4(t3[0], t3[1], t3[2], t3[3]);
     }
-    for (int idx = threadIdx.x; idx < 4 * 2 * 2; idx += blockDim.x) {
+    for (int idx = t0; idx < 4 * 2 * 2; idx += ts) {
         const int kk = idx >> 2, hh = (idx >> 1) & 1, c = idx & 1;
         uint32_t t1[4];
         for (int j = 0; j < 8; j += 2) {
@@ -319,23 +321,17 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
     block_record(v, partials + (size_t)blockIdx.x * kRec);
 }
 
-// g_i = -w_i^2 / (n <w>^2) (variance-loss score coefficient), abar = 0; stats = sum w, sum w^2.
-__global__ void k_flow_coef(const float *__restrict__ w, const double *__restrict__ rec, uint64_t n,
-                            float *__restrict__ g, float *__restrict__ abar, double *__restrict__ hist, int step)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double mean = rec[kRS] / (double)n;
-    if (i == 0) { hist[2 * step] = rec[kRS]; hist[2 * step + 1] = rec[kRQ]; }
-    if (i >= n) return;
-    const double r = (double)w[i] / mean;
-    g[i] = (float)(-(r * r) / (double)n);
-    abar[3 * i] = 0.0f; abar[3 * i + 1] = 0.0f; abar[3 * i + 2] = 0.0f;
-}
+// Adam's step (Kingma, Ba 2015) with bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t.
+struct AdamStep {
+    float *p, *m, *v;
+    float lr, b1, b2, eps, c1, c2;
+};
 
 // Parameter gradient of all layers: grad[l*FPL + j] = sum over r < rows of
-// parts[(l*stride + r)*FPL + j], in a fixed order (four interleaved row groups, then their sum).
+// parts[(l*stride + r)*FPL + j], in a fixed order (four interleaved row groups, then their sum);
+// with adam.p set, also Adam's step of parameter l*FPL + j.
 __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ parts, uint32_t rows, uint32_t stride,
-                                                  float *__restrict__ grad)
+                                                  float *__restrict__ grad, AdamStep adam)
 {
     __shared__ float red[4][64];
     const uint32_t c = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
@@ -348,21 +344,17 @@ __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ part
     }
     red[rg][threadIdx.x & 63] = s;
     __syncthreads();
-    if (threadIdx.x < 64 && c < (uint32_t)(FL * FPL))
-        grad[c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
-}
-
-__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
-                       const float *__restrict__ g, int n, float lr, float b1, float b2, float eps, float c1, float c2)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    if (threadIdx.x < 64 && c < (uint32_t)(FL * FPL)) {
+        const float gi = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        grad[c] = gi;
+        if (adam.p) {
+            const float mi = adam.b1 * adam.m[c] + (1.0f - adam.b1) * gi;
+            const float vi = adam.b2 * adam.v[c] + (1.0f - adam.b2) * gi * gi;
+            adam.m[c] = mi;
+            adam.v[c] = vi;
+            adam.p[c] -= adam.lr * (mi / adam.c1) / (sqrtf(vi / adam.c2) + adam.eps);
+        }
+    }
 }
 
 // Adds a kRec record (sum slots add, max slots max) into an accumulating record.
@@ -388,7 +380,7 @@ struct FlowState {
     // work buffers sized for `cap` points
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
-    float *ld = nullptr, *w = nullptr, *g = nullptr, *abar = nullptr;
+    float *ld = nullptr, *w = nullptr, *abar = nullptr;
     float *gp = nullptr;        // per-block layer gradients, FL x gp_rows x FPL
     uint32_t gp_rows = 0;
     double *hist = nullptr;     // training history (sum w, sum w^2 per step)
@@ -448,12 +440,11 @@ int flow_alloc(mi_ctx *ctx, uint64_t npts)
 {
     FlowState &F = *ctx->flow;
     if (npts > F.cap) {
-        (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
-        F.xs = nullptr; F.ld = F.w = F.g = F.abar = nullptr; F.cap = 0;
+        (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.abar);
+        F.xs = nullptr; F.ld = F.w = F.abar = nullptr; F.cap = 0;
         FCHK(hipMalloc(&F.xs, sizeof(float) * 3 * (FL + 1) * npts));
         FCHK(hipMalloc(&F.ld, sizeof(float) * npts));
         FCHK(hipMalloc(&F.w, sizeof(float) * npts));
-        FCHK(hipMalloc(&F.g, sizeof(float) * npts));
         FCHK(hipMalloc(&F.abar, sizeof(float) * 3 * npts));
         F.cap = npts;
     }
@@ -508,7 +499,7 @@ void mi_flow_release(mi_ctx *ctx)
     FlowState &F = *ctx->flow;
     (void)hipFree(F.params); (void)hipFree(F.m); (void)hipFree(F.v); (void)hipFree(F.grad);
     (void)hipFree(F.fwd); (void)hipFree(F.bwd);
-    (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.g); (void)hipFree(F.abar);
+    (void)hipFree(F.xs); (void)hipFree(F.ld); (void)hipFree(F.w); (void)hipFree(F.abar);
     (void)hipFree(F.gp); (void)hipFree(F.hist); (void)hipFree(F.acc);
     delete ctx->flow;
     ctx->flow = nullptr;
@@ -533,7 +524,7 @@ int mi_flow_set_params(mi_ctx *ctx, const float *params_host)
     FCHK(hipMemcpyAsync(F.params, params_host, sizeof(float) * MI_FLOW_NPARAMS, hipMemcpyHostToDevice, ctx->stream));
     FCHK(hipMemsetAsync(F.m, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
     FCHK(hipMemsetAsync(F.v, 0, sizeof(float) * MI_FLOW_NPARAMS, ctx->stream));
-    hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
+    hipLaunchKernelGGL(k_flow_pack, dim3(FL, kPackSplit), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(ctx->stream));
     F.step = 0;
@@ -604,20 +595,22 @@ int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, f
     return MI_OK;
 }
 
-// Backward through all layers from the layer inputs in F.xs and abar (dL/dz), then the parameter
-// gradient of every layer into F.grad.
-static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev)
+// Backward through all layers from the layer inputs in F.xs (dL/dz = 0), then the parameter
+// gradient of every layer into F.grad.  g_dev: dL/d(log q) per point, or null for the training
+// step's variance-loss coefficient from F.w and the batch record rec (its sums go to F.hist[step]);
+// adam: the optimizer step on top.
+static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev, const double *rec, int step,
+                           const AdamStep *adam)
 {
     FlowState &F = *ctx->flow;
     const uint32_t blocks = bwd_blocks(ctx, n);
     const uint32_t rows = blocks;
     if (rows > F.gp_rows) return ferr(ctx, MI_ERR_STATE, "gradient partials buffer too small");
-    for (int l = FL - 1; l >= 0; --l) {
-        FCHK(launch_flow_bwd(ctx->stream, blocks, F.fwd, F.bwd, l, n, F.xs + (size_t)3 * F.cap * l, g_dev, F.abar,
-                             F.gp + (size_t)l * F.gp_rows * FPL));
-    }
+    FCHK(launch_flow_bwd(ctx->stream, blocks, F.fwd, F.bwd, n, F.xs, 3 * F.cap, g_dev, g_dev ? nullptr : F.w, rec,
+                         F.abar, F.gp, F.gp_rows, g_dev ? nullptr : F.hist, step));
+    const AdamStep a = adam ? *adam : AdamStep{};
     hipLaunchKernelGGL(k_grad_sum, dim3((FL * FPL + 63) / 64), dim3(256), 0, ctx->stream, F.gp, rows, F.gp_rows,
-                       F.grad);
+                       F.grad, a);
     FCHK(hipGetLastError());
     return MI_OK;
 }
@@ -628,8 +621,7 @@ static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
     FlowState &F = *ctx->flow;
     // forward pass keeping every layer input
     FCHK(run_chain(ctx, false, false, n, F.xs, nullptr, F.ld, F.xs));
-    FCHK(hipMemsetAsync(F.abar, 0, sizeof(float) * 3 * n, ctx->stream));
-    return launch_backward(ctx, n, g_dev);
+    return launch_backward(ctx, n, g_dev, nullptr, 0, nullptr);
 }
 
 int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g_dev, float *grad_dev)
@@ -712,19 +704,15 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
     for (int s = 0; s < steps; ++s) {
         rc = flow_sample_weights(ctx, run, run->counter0 + (uint64_t)s * batch, batch, 2u, ctx->red, true);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_flow_coef, dim3((uint32_t)((batch + 255) / 256)), dim3(256), 0, ctx->stream, F.w, ctx->red,
-                           batch, F.g, F.abar, F.hist, s);
-        FCHK(hipGetLastError());
-        // backward from the stored layer inputs (x^(0) = y ... x^(FL) = z from the sampling pass)
-        rc = launch_backward(ctx, batch, F.g);
-        if (rc) return rc;
+        // backward from the stored layer inputs (x^(0) = y ... x^(FL) = z from the sampling pass),
+        // gradient and Adam's step
         F.step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f - std::pow(b1, (float)F.step), c2 = 1.0f - std::pow(b2, (float)F.step);
-        hipLaunchKernelGGL(k_adam, dim3((MI_FLOW_NPARAMS + 255) / 256), dim3(256), 0, ctx->stream, F.params, F.m, F.v,
-                           F.grad, MI_FLOW_NPARAMS, (float)lr, b1, b2, 1e-8f, c1, c2);
-        FCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_flow_pack, dim3(FL), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
+        const AdamStep adam{F.params, F.m, F.v, (float)lr, b1, b2, 1e-8f, c1, c2};
+        rc = launch_backward(ctx, batch, nullptr, ctx->red, s, &adam);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_flow_pack, dim3(FL, kPackSplit), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
         FCHK(hipGetLastError());
     }
     FCHK(hipEventRecord(ctx->ev1, ctx->stream));

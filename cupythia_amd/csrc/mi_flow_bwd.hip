@@ -1,7 +1,7 @@
 // mi_flow_bwd.hip -- backward kernel of the spline-coupling flow: the adjoint of one layer for
 // L = sum_i (abar_out_i . x_out_i + g_i log S'_i) summed with the later layers' contributions.
-// abar (n x 3) holds dL/d(output coordinates) on entry and dL/d(input coordinates) on exit; the
-// layer's parameter gradient of the block's points goes to row blockIdx.x of gp (rows x FPL).
+// Per layer, abar (n x 3) holds dL/d(output coordinates) on entry and dL/d(input coordinates) on
+// exit; the layer's parameter gradient of the block's points goes to one row of gp.
 //
 // A wave owns 64 points per tile and keeps every matrix of the tile in matrix-core registers
 // (layout in mi_flow_common.h): the MLP is recomputed as in the forward kernel (activations H1,
@@ -189,26 +189,39 @@ __device__ __forceinline__ float spline_bwd(const float (&o)[FO], float x, float
     return dx;
 }
 
-__global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const FwdLayer *__restrict__ fl,
-                                                             const BwdExtra *__restrict__ bl, int l, uint64_t n,
-                                                             const float *__restrict__ xin, const float *__restrict__ g,
-                                                             float *__restrict__ abar, float *__restrict__ gp)
+// Arguments of the backward kernel.
+struct BwdArgs {
+    const FwdLayer *fl;
+    const BwdExtra *bl;
+    uint64_t n;
+    const float *xs;            // layer inputs of the normalizing direction: x^(l) at xs + l * xstride
+    uint64_t xstride;
+    const float *g;             // dL/d(log q) per point, or null: the variance-loss coefficient
+    const float *w;             //   g_i = -(w_i / <w>)^2 / n with <w> = rec[kRS] / n
+    const double *rec;
+    float *abar;                // n x 3 work array (not read at the last layer, where dL/dz = 0)
+    float *gp;                  // gradient rows of layer l: gp + (l * gp_stride + blockIdx.x) * FPL
+    uint32_t gp_stride;
+    double *hist;               // optional: (sum w, sum w^2) of the batch at hist[2 step]
+    int step;
+};
+
+// All layers, last to first, in one launch: per layer the block stages the layer's weights, its
+// waves walk their tiles (the same tiles in every layer, so that abar passes from layer to layer
+// within a thread), and the block writes its gradient row.
+__global__ __launch_bounds__(64 * FWB) void k_flow_bwd(const BwdArgs A)
 {
     __shared__ BwdShared S;
-    {
-        const uint4 *s0 = reinterpret_cast<const uint4 *>(fl + l);
-        uint4 *d0 = reinterpret_cast<uint4 *>(&S.t.f);
-        for (int j = threadIdx.x; j < (int)(sizeof(FwdLayer) / 16); j += blockDim.x) d0[j] = s0[j];
-        const uint4 *s1 = reinterpret_cast<const uint4 *>(bl + l);
-        uint4 *d1 = reinterpret_cast<uint4 *>(&S.t.b);
-        for (int j = threadIdx.x; j < (int)(sizeof(BwdExtra) / 16); j += blockDim.x) d1[j] = s1[j];
-        __syncthreads();
-    }
     const FwdLayer &W = S.t.f;
     const BwdExtra &T = S.t.b;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, hh = lane >> 5, c = lane & 31;
     uint16_t(*comp)[64] = S.t.comp[wave];
-    const int d = l % 3, ca = cond0(d), cb = cond1(d);
+    if (A.hist && blockIdx.x == 0 && threadIdx.x == 0) {
+        A.hist[2 * A.step] = A.rec[kRS];
+        A.hist[2 * A.step + 1] = A.rec[kRQ];
+    }
+    // g_i = -(w_i / <w>)^2 / n as -(w_i * inv_mean)^2 * inv_n
+    const double inv_mean = A.g ? 1.0 : (double)A.n / A.rec[kRS], inv_n = 1.0 / (double)A.n;
     // one-hot fragments of the transposes: step kk, column 32 (kk / 2) + c, a one in the slot
     // that carries row 32 (kk / 2) + c
     bf16x8 I[4];
@@ -221,250 +234,275 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_layer_bwd(const FwdLayer *__r
         I[kk] = as_frag(make_uint4(w[0], w[1], w[2], w[3]));
     }
     const f32x16 zero = {};
-    f32x16 gW3[2][2], gW2[2][2], gW1[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        gW1[a] = zero;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) { gW3[a][b] = zero; gW2[a][b] = zero; }
-    }
-    float gb2[2] = {0.0f, 0.0f}, gb3[2] = {0.0f, 0.0f};
-    const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = (uint64_t)blockIdx.x * FWB + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWB) {
-        const uint64_t i = tile * 64 + lane;
-        const bool live = i < n;
-        float x[3] = {0.5f, 0.5f, 0.5f}, ab[3] = {0.0f, 0.0f, 0.0f}, gi = 0.0f;
-        if (live) {
-            x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2];
-            ab[0] = abar[3 * i]; ab[1] = abar[3 * i + 1]; ab[2] = abar[3 * i + 2];
-            gi = g[i];
-        }
-        const CondFrag cf = cond_frag(pick3(x, ca), pick3(x, cb));
-        wave_sync();                                   // the previous tile's reads of comp are done
-        comp[0][lane] = (uint16_t)cf.parts[0]; comp[3][lane] = (uint16_t)(cf.parts[0] >> 16);
-        comp[1][lane] = (uint16_t)cf.parts[1]; comp[4][lane] = (uint16_t)(cf.parts[1] >> 16);
-        comp[2][lane] = (uint16_t)cf.parts[2]; comp[5][lane] = (uint16_t)(cf.parts[2] >> 16);
-        // ---- forward recompute: H1, H2 as operands (rows = units, columns = points), outputs o
-        bf16x8 H1[4][2], H2[4][2];
-        f32x16 C[2][2];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            const bf16x8 a = as_frag(W.W1f[hh][32 * mb + c]);
-            C[mb][0] = mfma16(a, as_frag(cf.frag[0]), zero);
-            C[mb][1] = mfma16(a, as_frag(cf.frag[1]), zero);
-        }
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) H1[kk][nb] = relu_frag(C[kk >> 1][nb], 8 * (kk & 1));
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            C[mb][0] = bias_block(W.b2, mb, hh);
-            C[mb][1] = C[mb][0];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const bf16x8 a = as_frag(W.W2f[kk][hh][32 * mb + c]);
-                C[mb][0] = mfma16(a, H1[kk][0], C[mb][0]);
-                C[mb][1] = mfma16(a, H1[kk][1], C[mb][1]);
-            }
-        }
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) H2[kk][nb] = relu_frag(C[kk >> 1][nb], 8 * (kk & 1));
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            C[mb][0] = bias_block(W.b3, mb, hh);
-            C[mb][1] = C[mb][0];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const bf16x8 a = as_frag(W.W3f[kk][hh][32 * mb + c]);
-                C[mb][0] = mfma16(a, H2[kk][0], C[mb][0]);
-                C[mb][1] = mfma16(a, H2[kk][1], C[mb][1]);
-            }
-        }
-        // ---- spline adjoint of the lane's point, dO back to the block layout
-        float dO[FO];
-        float dx;
+    const uint64_t ntiles = (A.n + 63) / 64;
+#pragma unroll 1
+    for (int l = FL - 1; l >= 0; --l) {
         {
-            float o[FO];
-            rows_to_lane(C, o);
-            dx = spline_bwd(o, pick3(x, d), live ? pick3(ab, d) : 0.0f, live ? gi : 0.0f, dO);
+            const uint4 *s0 = reinterpret_cast<const uint4 *>(A.fl + l);
+            uint4 *d0 = reinterpret_cast<uint4 *>(&S.t.f);
+            for (int j = threadIdx.x; j < (int)(sizeof(FwdLayer) / 16); j += blockDim.x) d0[j] = s0[j];
+            const uint4 *s1 = reinterpret_cast<const uint4 *>(A.bl + l);
+            uint4 *d1 = reinterpret_cast<uint4 *>(&S.t.b);
+            for (int j = threadIdx.x; j < (int)(sizeof(BwdExtra) / 16); j += blockDim.x) d1[j] = s1[j];
+            __syncthreads();
         }
-        lane_to_rows(dO, C);
-        bf16x8 Df[4][2];
+        const int d = l % 3, ca = cond0(d), cb = cond1(d);
+        const float *xin = A.xs + (size_t)l * A.xstride;
+        float *abar = A.abar;
+        f32x16 gW3[2][2], gW2[2][2], gW1[2];
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
+        for (int a = 0; a < 2; ++a) {
+            gW1[a] = zero;
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) Df[kk][nb] = plain_frag(C[kk >> 1][nb], 8 * (kk & 1));
-        // ---- dW3 += dO^T H2, db3 += sum dO
-        {
-            f32x16 Tt[2][2];
-            bf16x8 Ot[4][2], Ht[4][2];
-            transpose64(Df, I, Tt, Ot);
-            gb3[0] += reg_sum(Tt[0][0], Tt[1][0]);
-            gb3[1] += reg_sum(Tt[0][1], Tt[1][1]);
-            transpose64(H2, I, Tt, Ht);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-                for (int ob = 0; ob < 2; ++ob)
-#pragma unroll
-                    for (int ub = 0; ub < 2; ++ub) mfma_acc(gW3[ob][ub], Ot[kk][ob], Ht[kk][ub]);
+            for (int b = 0; b < 2; ++b) { gW3[a][b] = zero; gW2[a][b] = zero; }
         }
-        // ---- dZ2 = (W3^T dO) where H2 > 0
-        bf16x8 Z2[4][2];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            C[mb][0] = zero;
-            C[mb][1] = zero;
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const bf16x8 a = as_frag(T.W3Tf[kk][hh][32 * mb + c]);
-                C[mb][0] = mfma16(a, Df[kk][0], C[mb][0]);
-                C[mb][1] = mfma16(a, Df[kk][1], C[mb][1]);
-            }
-        }
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) Z2[kk][nb] = masked_frag(C[kk >> 1][nb], 8 * (kk & 1), H2[kk][nb]);
-        // ---- dW2 += dZ2^T H1, db2 += sum dZ2
-        {
-            f32x16 Tt[2][2];
-            bf16x8 Zt[4][2], Ht[4][2];
-            transpose64(Z2, I, Tt, Zt);
-            gb2[0] += reg_sum(Tt[0][0], Tt[1][0]);
-            gb2[1] += reg_sum(Tt[0][1], Tt[1][1]);
-            transpose64(H1, I, Tt, Ht);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-                for (int ub = 0; ub < 2; ++ub)
-#pragma unroll
-                    for (int kb = 0; kb < 2; ++kb) mfma_acc(gW2[ub][kb], Zt[kk][ub], Ht[kk][kb]);
-        }
-        // ---- dZ1 = (W2^T dZ2) where H1 > 0
-        bf16x8 Z1[4][2];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            C[mb][0] = zero;
-            C[mb][1] = zero;
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const bf16x8 a = as_frag(T.W2Tf[kk][hh][32 * mb + c]);
-                C[mb][0] = mfma16(a, Z2[kk][0], C[mb][0]);
-                C[mb][1] = mfma16(a, Z2[kk][1], C[mb][1]);
-            }
-        }
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) Z1[kk][nb] = masked_frag(C[kk >> 1][nb], 8 * (kk & 1), H1[kk][nb]);
-        // ---- (dW1, db1) += dZ1^T (c0 parts, c1 parts, 1): column q < 6 of the operand is part q of
-        // the points' conditioners, column 6 is one
-        {
-            f32x16 Tt[2][2];
-            bf16x8 Zt[4][2];
-            transpose64(Z1, I, Tt, Zt);
-            wave_sync();                               // comp written by the whole wave
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                uint4 w = make_uint4(0u, 0u, 0u, 0u);
-                if (c < 6) {
-                    const uint2 a = *reinterpret_cast<const uint2 *>(&comp[c][16 * kk + 4 * hh]);
-                    const uint2 b = *reinterpret_cast<const uint2 *>(&comp[c][16 * kk + 8 + 4 * hh]);
-                    w = make_uint4(a.x, a.y, b.x, b.y);
-                } else if (c == 6) {
-                    w = make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u);
+        float gb2[2] = {0.0f, 0.0f}, gb3[2] = {0.0f, 0.0f};
+        for (uint64_t tile = (uint64_t)blockIdx.x * FWB + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWB) {
+            const uint64_t i = tile * 64 + lane;
+            const bool live = i < A.n;
+            float x[3] = {0.5f, 0.5f, 0.5f}, ab[3] = {0.0f, 0.0f, 0.0f}, gi = 0.0f;
+            if (live) {
+                x[0] = xin[3 * i]; x[1] = xin[3 * i + 1]; x[2] = xin[3 * i + 2];
+                if (l < FL - 1) { ab[0] = abar[3 * i]; ab[1] = abar[3 * i + 1]; ab[2] = abar[3 * i + 2]; }
+                if (A.g) {
+                    gi = A.g[i];
+                } else {
+                    const double r = (double)A.w[i] * inv_mean;
+                    gi = (float)(-(r * r) * inv_n);
                 }
+            }
+            const CondFrag cf = cond_frag(pick3(x, ca), pick3(x, cb));
+            wave_sync();                                   // the previous tile's reads of comp are done
+            comp[0][lane] = (uint16_t)cf.parts[0]; comp[3][lane] = (uint16_t)(cf.parts[0] >> 16);
+            comp[1][lane] = (uint16_t)cf.parts[1]; comp[4][lane] = (uint16_t)(cf.parts[1] >> 16);
+            comp[2][lane] = (uint16_t)cf.parts[2]; comp[5][lane] = (uint16_t)(cf.parts[2] >> 16);
+            // ---- forward recompute: H1, H2 as operands (rows = units, columns = points), outputs o
+            bf16x8 H1[4][2], H2[4][2];
+            f32x16 C[2][2];
 #pragma unroll
-                for (int ub = 0; ub < 2; ++ub) mfma_acc(gW1[ub], Zt[kk][ub], as_frag(w));
+            for (int mb = 0; mb < 2; ++mb) {
+                const bf16x8 a = as_frag(W.W1f[hh][32 * mb + c]);
+                C[mb][0] = mfma16(a, as_frag(cf.frag[0]), zero);
+                C[mb][1] = mfma16(a, as_frag(cf.frag[1]), zero);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) H1[kk][nb] = relu_frag(C[kk >> 1][nb], 8 * (kk & 1));
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                C[mb][0] = bias_block(W.b2, mb, hh);
+                C[mb][1] = C[mb][0];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const bf16x8 a = as_frag(W.W2f[kk][hh][32 * mb + c]);
+                    C[mb][0] = mfma16(a, H1[kk][0], C[mb][0]);
+                    C[mb][1] = mfma16(a, H1[kk][1], C[mb][1]);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) H2[kk][nb] = relu_frag(C[kk >> 1][nb], 8 * (kk & 1));
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                C[mb][0] = bias_block(W.b3, mb, hh);
+                C[mb][1] = C[mb][0];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const bf16x8 a = as_frag(W.W3f[kk][hh][32 * mb + c]);
+                    C[mb][0] = mfma16(a, H2[kk][0], C[mb][0]);
+                    C[mb][1] = mfma16(a, H2[kk][1], C[mb][1]);
+                }
+            }
+            // ---- spline adjoint of the lane's point, dO back to the block layout
+            float dO[FO];
+            float dx;
+            {
+                float o[FO];
+                rows_to_lane(C, o);
+                dx = spline_bwd(o, pick3(x, d), live ? pick3(ab, d) : 0.0f, live ? gi : 0.0f, dO);
+            }
+            lane_to_rows(dO, C);
+            bf16x8 Df[4][2];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) Df[kk][nb] = plain_frag(C[kk >> 1][nb], 8 * (kk & 1));
+            // ---- dW3 += dO^T H2, db3 += sum dO
+            {
+                f32x16 Tt[2][2];
+                bf16x8 Ot[4][2], Ht[4][2];
+                transpose64(Df, I, Tt, Ot);
+                gb3[0] += reg_sum(Tt[0][0], Tt[1][0]);
+                gb3[1] += reg_sum(Tt[0][1], Tt[1][1]);
+                transpose64(H2, I, Tt, Ht);
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                    for (int ob = 0; ob < 2; ++ob)
+#pragma unroll
+                        for (int ub = 0; ub < 2; ++ub) mfma_acc(gW3[ob][ub], Ot[kk][ob], Ht[kk][ub]);
+            }
+            // ---- dZ2 = (W3^T dO) where H2 > 0
+            bf16x8 Z2[4][2];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                C[mb][0] = zero;
+                C[mb][1] = zero;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const bf16x8 a = as_frag(T.W3Tf[kk][hh][32 * mb + c]);
+                    C[mb][0] = mfma16(a, Df[kk][0], C[mb][0]);
+                    C[mb][1] = mfma16(a, Df[kk][1], C[mb][1]);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) Z2[kk][nb] = masked_frag(C[kk >> 1][nb], 8 * (kk & 1), H2[kk][nb]);
+            // ---- dW2 += dZ2^T H1, db2 += sum dZ2
+            {
+                f32x16 Tt[2][2];
+                bf16x8 Zt[4][2], Ht[4][2];
+                transpose64(Z2, I, Tt, Zt);
+                gb2[0] += reg_sum(Tt[0][0], Tt[1][0]);
+                gb2[1] += reg_sum(Tt[0][1], Tt[1][1]);
+                transpose64(H1, I, Tt, Ht);
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+                        for (int kb = 0; kb < 2; ++kb) mfma_acc(gW2[ub][kb], Zt[kk][ub], Ht[kk][kb]);
+            }
+            // ---- dZ1 = (W2^T dZ2) where H1 > 0
+            bf16x8 Z1[4][2];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                C[mb][0] = zero;
+                C[mb][1] = zero;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const bf16x8 a = as_frag(T.W2Tf[kk][hh][32 * mb + c]);
+                    C[mb][0] = mfma16(a, Z2[kk][0], C[mb][0]);
+                    C[mb][1] = mfma16(a, Z2[kk][1], C[mb][1]);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) Z1[kk][nb] = masked_frag(C[kk >> 1][nb], 8 * (kk & 1), H1[kk][nb]);
+            // ---- (dW1, db1) += dZ1^T (c0 parts, c1 parts, 1): column q < 6 of the operand is part q of
+            // the points' conditioners, column 6 is one
+            {
+                f32x16 Tt[2][2];
+                bf16x8 Zt[4][2];
+                transpose64(Z1, I, Tt, Zt);
+                wave_sync();                               // comp written by the whole wave
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+                    if (c < 6) {
+                        const uint2 a = *reinterpret_cast<const uint2 *>(&comp[c][16 * kk + 4 * hh]);
+                        const uint2 b = *reinterpret_cast<const uint2 *>(&comp[c][16 * kk + 8 + 4 * hh]);
+                        w = make_uint4(a.x, a.y, b.x, b.y);
+                    } else if (c == 6) {
+                        w = make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u);
+                    }
+#pragma unroll
+                    for (int ub = 0; ub < 2; ++ub) mfma_acc(gW1[ub], Zt[kk][ub], as_frag(w));
+                }
+            }
+            // ---- dc = W1^T dZ1: rows 0, 1 of the product, in lanes 0..31 for the points of each block
+            float dc0, dc1;
+            {
+                f32x16 E[2] = {zero, zero};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const bf16x8 a = c < 2 ? as_frag(T.W1Tf[kk][hh][c & 1]) : as_frag(make_uint4(0u, 0u, 0u, 0u));
+                    E[0] = mfma16(a, Z1[kk][0], E[0]);
+                    E[1] = mfma16(a, Z1[kk][1], E[1]);
+                }
+                dc0 = __uint_as_float(swap32(__float_as_uint(E[0][0]), __float_as_uint(E[1][0])).a);
+                dc1 = __uint_as_float(swap32(__float_as_uint(E[0][1]), __float_as_uint(E[1][1])).a);
+            }
+            if (live) {
+                set3(ab, d, dx);
+                set3(ab, ca, pick3(ab, ca) + dc0);
+                set3(ab, cb, pick3(ab, cb) + dc1);
+                abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
             }
         }
-        // ---- dc = W1^T dZ1: rows 0, 1 of the product, in lanes 0..31 for the points of each block
-        float dc0, dc1;
-        {
-            f32x16 E[2] = {zero, zero};
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const bf16x8 a = c < 2 ? as_frag(T.W1Tf[kk][hh][c & 1]) : as_frag(make_uint4(0u, 0u, 0u, 0u));
-                E[0] = mfma16(a, Z1[kk][0], E[0]);
-                E[1] = mfma16(a, Z1[kk][1], E[1]);
-            }
-            dc0 = __uint_as_float(swap32(__float_as_uint(E[0][0]), __float_as_uint(E[1][0])).a);
-            dc1 = __uint_as_float(swap32(__float_as_uint(E[0][1]), __float_as_uint(E[1][1])).a);
+        for (int a = 0; a < 2; ++a) {
+            mfma_acc_drain(gW1[a]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) { mfma_acc_drain(gW3[a][b]); mfma_acc_drain(gW2[a][b]); }
         }
-        if (live) {
-            set3(ab, d, dx);
-            set3(ab, ca, pick3(ab, ca) + dc0);
-            set3(ab, cb, pick3(ab, cb) + dc1);
-            abar[3 * i] = ab[0]; abar[3 * i + 1] = ab[1]; abar[3 * i + 2] = ab[2];
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        mfma_acc_drain(gW1[a]);
-#pragma unroll
-        for (int b = 0; b < 2; ++b) { mfma_acc_drain(gW3[a][b]); mfma_acc_drain(gW2[a][b]); }
-    }
-    // the lanes l and l ^ 32 hold the two halves of the points
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        gb2[h] += __shfl_xor(gb2[h], 32, 64);
-        gb3[h] += __shfl_xor(gb3[h], 32, 64);
-    }
-    __syncthreads();                                   // every wave is past its tiles: LDS holds the rows now
-    float *out = S.rows[wave];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = 32 * a + 8 * (v >> 2) + 4 * hh + (v & 3), col = 32 * b + c;
-                if (row < FO) out[PW3 + row * FH + col] = gW3[a][b][v];
-                out[PW2 + row * FH + col] = gW2[a][b][v];
-            }
-    // dW1 and db1: columns 0..2 (c0 parts), 3..5 (c1 parts) and 6 (ones) of each row
-    const int base = lane & 32;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            float q[7];
-#pragma unroll
-            for (int k = 0; k < 7; ++k) q[k] = __shfl(gW1[a][v], base + k, 64);
-            if (c == 0) {
-                const int u = 32 * a + 8 * (v >> 2) + 4 * hh + (v & 3);
-                out[PW1 + 2 * u] = (q[0] + q[1]) + q[2];
-                out[PW1 + 2 * u + 1] = (q[3] + q[4]) + q[5];
-                out[PB1 + u] = q[6];
-            }
-        }
-    if (hh == 0) {
+        // the lanes l and l ^ 32 hold the two halves of the points
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            out[PB2 + 32 * h + c] = gb2[h];
-            if (32 * h + c < FO) out[PB3 + 32 * h + c] = gb3[h];
+            gb2[h] += __shfl_xor(gb2[h], 32, 64);
+            gb3[h] += __shfl_xor(gb3[h], 32, 64);
         }
-    }
-    __syncthreads();
-    float *dst = gp + (size_t)blockIdx.x * FPL;
-    for (int j = threadIdx.x; j < FPL; j += 64 * FWB) {
-        float t = S.rows[0][j];
+        __syncthreads();                                   // every wave is past its tiles: LDS holds the rows now
+        float *out = S.rows[wave];
+        // dW1 and db1: columns 0..2 (c0 parts), 3..5 (c1 parts) and 6 (ones) of each row, summed
+        // per row through the W2 part of the row before that is written
+        {
+            float *tmp = out + PW2;
 #pragma unroll
-        for (int w = 1; w < FWB; ++w) t += S.rows[w][j];
-        dst[j] = t;
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) tmp[(16 * a + v) * 64 + lane] = gW1[a][v];
+            wave_sync();
+            const int u = lane;                                  // row u: block u / 32, register v, half h
+            const int a = u >> 5, v = 4 * ((u >> 3) & 3) + (u & 3), h = (u >> 2) & 1;
+            const float *q = tmp + (16 * a + v) * 64 + 32 * h;
+            const float w0 = (q[0] + q[1]) + q[2], w1 = (q[3] + q[4]) + q[5], b1 = q[6];
+            wave_sync();
+            out[PW1 + 2 * u] = w0;
+            out[PW1 + 2 * u + 1] = w1;
+            out[PB1 + u] = b1;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int row = 32 * a + 8 * (v >> 2) + 4 * hh + (v & 3), col = 32 * b + c;
+                    if (row < FO) out[PW3 + row * FH + col] = gW3[a][b][v];
+                    out[PW2 + row * FH + col] = gW2[a][b][v];
+                }
+        if (hh == 0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                out[PB2 + 32 * h + c] = gb2[h];
+                if (32 * h + c < FO) out[PB3 + 32 * h + c] = gb3[h];
+            }
+        }
+        __syncthreads();
+        float *dst = A.gp + ((size_t)l * A.gp_stride + blockIdx.x) * FPL;
+        for (int j = threadIdx.x; j < FPL; j += 64 * FWB) {
+            float t = S.rows[0][j];
+#pragma unroll
+            for (int w = 1; w < FWB; ++w) t += S.rows[w][j];
+            dst[j] = t;
+        }
+        __syncthreads();                                   // rows read before the next layer's weights replace them
     }
 }
 
 }  // namespace
 
-hipError_t launch_flow_bwd(hipStream_t stream, uint32_t blocks, const FwdLayer *fl, const BwdExtra *bl, int l,
-                           uint64_t n, const float *xin, const float *g, float *abar, float *gp)
+hipError_t launch_flow_bwd(hipStream_t stream, uint32_t blocks, const FwdLayer *fl, const BwdExtra *bl, uint64_t n,
+                           const float *xs, uint64_t xstride, const float *g, const float *w, const double *rec,
+                           float *abar, float *gp, uint32_t gp_stride, double *hist, int step)
 {
-    hipLaunchKernelGGL(k_flow_layer_bwd, dim3(blocks), dim3(64 * FWB), 0, stream, fl, bl, l, n, xin, g, abar, gp);
+    const BwdArgs a{fl, bl, n, xs, xstride, g, w, rec, abar, gp, gp_stride, hist, step};
+    hipLaunchKernelGGL(k_flow_bwd, dim3(blocks), dim3(64 * FWB), 0, stream, a);
     return hipGetLastError();
 }
 
