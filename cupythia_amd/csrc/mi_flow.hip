@@ -161,7 +161,8 @@ __device__ __forceinline__ float chain_spline(const float (&o)[FO], float v, flo
 // ctr0 + i.  Output: xout (n x 3, optional, may alias xin), ld (n) = the summed log-Jacobian
 // (written, not accumulated).  keep (optional): every layer input of the normalizing direction,
 // x^(j) at keep + j * kstride (x^(0) is the caller's input in the normalizing direction and is
-// not rewritten).  One block of FWV waves per CU holds all layers' weights in LDS (111 KB).
+// not rewritten).  One block per CU holds all layers' weights in LDS (111 KB); blocks have FWV
+// waves, or FWV / 2 (one per SIMD) when there are too few tiles to give every SIMD two.
 template <bool INVERSE, bool SAMPLE>
 __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restrict__ layers, uint64_t n,
                                                          const float *xin, float *xout,
@@ -177,7 +178,8 @@ __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restr
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t ntiles = (n + 63) / 64;
-    for (uint64_t tile = (uint64_t)blockIdx.x * FWV + wave; tile < ntiles; tile += (uint64_t)gridDim.x * FWV) {
+    const int nwave = blockDim.x >> 6;
+    for (uint64_t tile = (uint64_t)blockIdx.x * nwave + wave; tile < ntiles; tile += (uint64_t)gridDim.x * nwave) {
         const uint64_t i = tile * 64 + lane;
         const bool live = i < n;
         float x[3] = {0.5f, 0.5f, 0.5f};
@@ -270,9 +272,51 @@ __global__ void k_flow_pack(const float *__restrict__ params, FwdLayer *__restri
     }
 }
 
+// Writes parameter j (0 .. FPL-1) of layer l, value v, into every slot of the weight images that
+// carries it: the permutation k_flow_pack applies to a whole layer, one parameter at a time (the
+// padding slots keep the zeros of the last full pack).
+__device__ __forceinline__ uint16_t &img16(uint4 &q, int jj) { return reinterpret_cast<uint16_t *>(&q)[jj]; }
+__device__ void pack_param(int l, int j, float v, FwdLayer *__restrict__ fwd, BwdExtra *__restrict__ bwd)
+{
+    FwdLayer &f = fwd[l];
+    BwdExtra &e = bwd[l];
+    if (j < PB1) {                                       // W1[m][c]
+        const int m = j >> 1, c = j & 1, t = swz(m);
+        const uint16_t b = f2bf(v);
+        img16(f.W1f[0][m], c) = b;
+        img16(f.W1f[0][m], 2 + c) = b;
+        img16(f.W1f[0][m], 4 + c) = b;
+        img16(e.W1Tf[t >> 4][(t >> 3) & 1][c], t & 7) = b;
+    } else if (j < PW2) {                                // b1[m] in three parts
+        const int m = j - PB1;
+        const uint16_t hi = f2bf(v);
+        const float r = v - bf2f(hi);
+        const uint16_t mid = f2bf(r);
+        img16(f.W1f[1][m], 0) = hi;
+        img16(f.W1f[1][m], 1) = mid;
+        img16(f.W1f[1][m], 2) = f2bf(r - bf2f(mid));
+    } else if (j < PB2) {                                // W2[m][k]
+        const int m = (j - PW2) >> 6, k = (j - PW2) & 63, sk = swz(k), sm = swz(m);
+        const uint16_t b = f2bf(v);
+        img16(f.W2f[sk >> 4][(sk >> 3) & 1][m], sk & 7) = b;
+        img16(e.W2Tf[sm >> 4][(sm >> 3) & 1][k], sm & 7) = b;
+    } else if (j < PW3) {
+        f.b2[j - PB2] = v;
+    } else if (j < PB3) {                                // W3[o][u]
+        const int o = (j - PW3) >> 6, u = (j - PW3) & 63, su = swz(u), so = swz(o);
+        const uint16_t b = f2bf(v);
+        img16(f.W3f[su >> 4][(su >> 3) & 1][o], su & 7) = b;
+        img16(e.W3Tf[so >> 4][(so >> 3) & 1][u], so & 7) = b;
+    } else {
+        f.b3[j - PB3] = v;
+    }
+}
+
 // Weight of a flow point: y -> VEGAS map -> FP32 integrand, times J_V and |dy/dz| = exp(ld).
-// Writes w (optional) and one kRec record per block.  HEAVY: mask may select codes 121..124.
-template <bool HEAVY>
+// Writes w (optional) and one kRec record per block.  HEAVY: mask may select codes 121..124;
+// HW: the integrand with the hardware transcendentals (MI_PREC_FP32_HW); STATS: only the record's
+// sum w and sum w^2 (the training step's statistics; the other slots are not written).
+template <bool HEAVY, bool HW, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int mask, const float *__restrict__ gtab,
                                                         int nbins, uint64_t n, const float *__restrict__ y,
                                                         const float *__restrict__ ld, float *__restrict__ wout,
@@ -300,13 +344,14 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
             jac = jac * (nbf * q[1]);
         }
         float ws[kMaxProc];
-        const float f = had_point_f32<HEAVY>(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr,
-                                             nullptr, nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
+        const float f = had_point_f32<HEAVY, HW>(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr,
+                                                 nullptr, nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
         const float wt = f * jac * expf(ld[i]);
         if (wout) wout[i] = wt;
         const double dw = (double)wt;
         v[kRS] += dw;
         v[kRQ] = fma(dw, dw, v[kRQ]);
+        if (STATS) continue;
         v[kRM] = fmax(v[kRM], dw);
         v[kRNz] += (wt > 0.0f) ? 1.0 : 0.0;
         const float scale = jac * expf(ld[i]);
@@ -318,13 +363,28 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
             v[kRM + 1 + p] = fmax(v[kRM + 1 + p], dp);
         }
     }
-    block_record(v, partials + (size_t)blockIdx.x * kRec);
+    if (STATS) {
+        __shared__ double red[kBlock / 64][2];
+        const double s = wave_sum(v[kRS]), q = wave_sum(v[kRQ]);
+        if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6][0] = s; red[threadIdx.x >> 6][1] = q; }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            double t = red[0][threadIdx.x];
+            for (int w = 1; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+            partials[(size_t)blockIdx.x * kRec + (threadIdx.x == 0 ? kRS : kRQ)] = t;
+        }
+    } else {
+        block_record(v, partials + (size_t)blockIdx.x * kRec);
+    }
 }
 
-// Adam's step (Kingma, Ba 2015) with bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t.
+// Adam's step (Kingma, Ba 2015) with bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t; the updated
+// parameters also go into the weight images fwd, bwd.
 struct AdamStep {
     float *p, *m, *v;
     float lr, b1, b2, eps, c1, c2;
+    FwdLayer *fwd;
+    BwdExtra *bwd;
 };
 
 // Parameter gradient of all layers: grad[l*FPL + j] = sum over r < rows of
@@ -352,7 +412,9 @@ __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ part
             const float vi = adam.b2 * adam.v[c] + (1.0f - adam.b2) * gi * gi;
             adam.m[c] = mi;
             adam.v[c] = vi;
-            adam.p[c] -= adam.lr * (mi / adam.c1) / (sqrtf(vi / adam.c2) + adam.eps);
+            const float pn = adam.p[c] - adam.lr * (mi / adam.c1) / (sqrtf(vi / adam.c2) + adam.eps);
+            adam.p[c] = pn;
+            pack_param((int)(c / FPL), (int)(c % FPL), pn, adam.fwd, adam.bwd);
         }
     }
 }
@@ -425,10 +487,16 @@ int num_cu(mi_ctx *ctx) { return ctx->num_cu > 0 ? ctx->num_cu : 256; }
 
 // Grids: the forward / inverse kernel (111 KB of LDS) and the backward kernel (159 KB) run one
 // block per CU and walk the tiles with a grid stride.
-uint32_t fwd_blocks(mi_ctx *ctx, uint64_t n)
+// Waves per block of the forward / inverse kernel for n points, and its grid.
+int fwd_waves(mi_ctx *ctx, uint64_t n)
 {
     const uint64_t ntiles = (n + 63) / 64;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + FWV - 1) / FWV, (uint64_t)num_cu(ctx)));
+    return ntiles < (uint64_t)num_cu(ctx) * FWV ? FWV / 2 : FWV;
+}
+uint32_t fwd_blocks(mi_ctx *ctx, uint64_t n)
+{
+    const uint64_t ntiles = (n + 63) / 64, nw = (uint64_t)fwd_waves(ctx, n);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + nw - 1) / nw, (uint64_t)num_cu(ctx)));
 }
 uint32_t bwd_blocks(mi_ctx *ctx, uint64_t n)
 {
@@ -474,15 +542,16 @@ hipError_t run_chain(mi_ctx *ctx, bool inverse, bool sample, uint64_t n, const f
 {
     FlowState &F = *ctx->flow;
     const uint32_t blocks = fwd_blocks(ctx, n);
+    const dim3 threads(64 * fwd_waves(ctx, n));
     const uint64_t ks = 3 * F.cap;
     if (inverse && sample)
-        hipLaunchKernelGGL((k_flow_chain<true, true>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin, xout,
+        hipLaunchKernelGGL((k_flow_chain<true, true>), dim3(blocks), threads, 0, ctx->stream, F.fwd, n, xin, xout,
                            ld, keep, ks, key, ctr0, sid);
     else if (inverse)
-        hipLaunchKernelGGL((k_flow_chain<true, false>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin,
+        hipLaunchKernelGGL((k_flow_chain<true, false>), dim3(blocks), threads, 0, ctx->stream, F.fwd, n, xin,
                            xout, ld, keep, ks, key, ctr0, sid);
     else if (!sample)
-        hipLaunchKernelGGL((k_flow_chain<false, false>), dim3(blocks), dim3(64 * FWV), 0, ctx->stream, F.fwd, n, xin,
+        hipLaunchKernelGGL((k_flow_chain<false, false>), dim3(blocks), threads, 0, ctx->stream, F.fwd, n, xin,
                            xout, ld, keep, ks, key, ctr0, sid);
     else
         return hipErrorInvalidValue;
@@ -597,17 +666,18 @@ int mi_flow_inverse(mi_ctx *ctx, uint64_t n, const float *z_dev, float *y_dev, f
 
 // Backward through all layers from the layer inputs in F.xs (dL/dz = 0), then the parameter
 // gradient of every layer into F.grad.  g_dev: dL/d(log q) per point, or null for the training
-// step's variance-loss coefficient from F.w and the batch record rec (its sums go to F.hist[step]);
+// step's variance-loss coefficient from F.w and the nparts per-block sums of w, w^2 in
+// ctx->partials (their totals go to F.hist[step]);
 // adam: the optimizer step on top.
-static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev, const double *rec, int step,
+static int launch_backward(mi_ctx *ctx, uint64_t n, const float *g_dev, uint32_t nparts, int step,
                            const AdamStep *adam)
 {
     FlowState &F = *ctx->flow;
     const uint32_t blocks = bwd_blocks(ctx, n);
     const uint32_t rows = blocks;
     if (rows > F.gp_rows) return ferr(ctx, MI_ERR_STATE, "gradient partials buffer too small");
-    FCHK(launch_flow_bwd(ctx->stream, blocks, F.fwd, F.bwd, n, F.xs, 3 * F.cap, g_dev, g_dev ? nullptr : F.w, rec,
-                         F.abar, F.gp, F.gp_rows, g_dev ? nullptr : F.hist, step));
+    FCHK(launch_flow_bwd(ctx->stream, blocks, F.fwd, F.bwd, n, F.xs, 3 * F.cap, g_dev, g_dev ? nullptr : F.w,
+                         ctx->partials, nparts, F.abar, F.gp, F.gp_rows, g_dev ? nullptr : F.hist, step));
     const AdamStep a = adam ? *adam : AdamStep{};
     hipLaunchKernelGGL(k_grad_sum, dim3((FL * FPL + 63) / 64), dim3(256), 0, ctx->stream, F.gp, rows, F.gp_rows,
                        F.grad, a);
@@ -621,7 +691,7 @@ static int flow_backward_internal(mi_ctx *ctx, uint64_t n, const float *g_dev)
     FlowState &F = *ctx->flow;
     // forward pass keeping every layer input
     FCHK(run_chain(ctx, false, false, n, F.xs, nullptr, F.ld, F.xs));
-    return launch_backward(ctx, n, g_dev, nullptr, 0, nullptr);
+    return launch_backward(ctx, n, g_dev, 0, 0, nullptr);
 }
 
 int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g_dev, float *grad_dev)
@@ -643,9 +713,10 @@ int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g
 }
 
 // Sample n points of the current flow (+ VEGAS) for events ctr0 ..: y in F.xs[0], ld, w, and
-// the estimate record of the batch in out_rec (device).
+// the estimate record of the batch in out_rec (device), or with out_rec null only the per-block
+// sums of w and w^2 in ctx->partials (*nblocks of them, record layout).
 static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, uint64_t n, uint32_t sid,
-                               double *out_rec, bool keep_all)
+                               double *out_rec, bool keep_all, uint32_t *nblocks = nullptr)
 {
     FlowState &F = *ctx->flow;
     // z from Philox, y = T^-1(z) into F.xs (x^(0)); with keep_all every layer input as well
@@ -661,14 +732,17 @@ static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, ui
                                                       (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ);
     if ((size_t)nblk * kRec > ctx->partials_cap) return ferr(ctx, MI_ERR_STATE, "partials buffer too small");
     const size_t sm = sizeof(float) * 9 * ctx->grid.nbins;
-    if (has_heavy(run->phys.proc_mask))
-        hipLaunchKernelGGL(k_flow_weight<true>, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask,
-                           ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
-    else
-        hipLaunchKernelGGL(k_flow_weight<false>, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask,
-                           ctx->grid.tab32, ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
+    const bool heavy = has_heavy(run->phys.proc_mask), hw = run->precision == MI_PREC_FP32_HW, stats = !out_rec;
+    auto kern = k_flow_weight<false, false, false>;
+    if (heavy) kern = hw ? (stats ? k_flow_weight<true, true, true> : k_flow_weight<true, true, false>)
+                         : (stats ? k_flow_weight<true, false, true> : k_flow_weight<true, false, false>);
+    else kern = hw ? (stats ? k_flow_weight<false, true, true> : k_flow_weight<false, true, false>)
+                   : (stats ? k_flow_weight<false, false, true> : k_flow_weight<false, false, false>);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask, ctx->grid.tab32,
+                       ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
     FCHK(hipGetLastError());
-    FCHK(launch_reduce_records(ctx->stream, ctx->partials, nblk, out_rec));
+    if (nblocks) *nblocks = nblk;
+    if (out_rec) FCHK(launch_reduce_records(ctx->stream, ctx->partials, nblk, out_rec));
     return MI_OK;
 }
 
@@ -702,18 +776,17 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
     }
     FCHK(hipEventRecord(ctx->ev0, ctx->stream));
     for (int s = 0; s < steps; ++s) {
-        rc = flow_sample_weights(ctx, run, run->counter0 + (uint64_t)s * batch, batch, 2u, ctx->red, true);
+        uint32_t nparts = 0;
+        rc = flow_sample_weights(ctx, run, run->counter0 + (uint64_t)s * batch, batch, 2u, nullptr, true, &nparts);
         if (rc) return rc;
         // backward from the stored layer inputs (x^(0) = y ... x^(FL) = z from the sampling pass),
         // gradient and Adam's step
         F.step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f - std::pow(b1, (float)F.step), c2 = 1.0f - std::pow(b2, (float)F.step);
-        const AdamStep adam{F.params, F.m, F.v, (float)lr, b1, b2, 1e-8f, c1, c2};
-        rc = launch_backward(ctx, batch, nullptr, ctx->red, s, &adam);
+        const AdamStep adam{F.params, F.m, F.v, (float)lr, b1, b2, 1e-8f, c1, c2, F.fwd, F.bwd};
+        rc = launch_backward(ctx, batch, nullptr, nparts, s, &adam);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_flow_pack, dim3(FL, kPackSplit), dim3(256), 0, ctx->stream, F.params, F.fwd, F.bwd);
-        FCHK(hipGetLastError());
     }
     FCHK(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<double> h(2 * (size_t)steps);

@@ -197,8 +197,9 @@ struct BwdArgs {
     const float *xs;            // layer inputs of the normalizing direction: x^(l) at xs + l * xstride
     uint64_t xstride;
     const float *g;             // dL/d(log q) per point, or null: the variance-loss coefficient
-    const float *w;             //   g_i = -(w_i / <w>)^2 / n with <w> = rec[kRS] / n
-    const double *rec;
+    const float *w;             //   g_i = -(w_i / <w>)^2 / n, <w> = sum w / n from the nparts
+    const double *parts;        //   per-block sums of w and w^2 (record slots kRS, kRQ)
+    uint32_t nparts;
     float *abar;                // n x 3 work array (not read at the last layer, where dL/dz = 0)
     float *gp;                  // gradient rows of layer l: gp + (l * gp_stride + blockIdx.x) * FPL
     uint32_t gp_stride;
@@ -216,12 +217,32 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_bwd(const BwdArgs A)
     const BwdExtra &T = S.t.b;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, hh = lane >> 5, c = lane & 31;
     uint16_t(*comp)[64] = S.t.comp[wave];
-    if (A.hist && blockIdx.x == 0 && threadIdx.x == 0) {
-        A.hist[2 * A.step] = A.rec[kRS];
-        A.hist[2 * A.step + 1] = A.rec[kRQ];
+    // the batch's sum w and sum w^2, the same in every block (fixed order: strided per thread, the
+    // wave's tree, the waves in order)
+    double inv_mean = 1.0;
+    if (!A.g) {
+        __shared__ double sums[2][FWB];
+        double sw = 0.0, sq = 0.0;
+        for (uint32_t b = threadIdx.x; b < A.nparts; b += blockDim.x) {
+            sw += A.parts[(size_t)b * kRec + kRS];
+            sq += A.parts[(size_t)b * kRec + kRQ];
+        }
+        sw = wave_sum(sw);
+        sq = wave_sum(sq);
+        if (lane == 0) { sums[0][wave] = sw; sums[1][wave] = sq; }
+        __syncthreads();
+        sw = sums[0][0];
+        sq = sums[1][0];
+#pragma unroll
+        for (int w = 1; w < FWB; ++w) { sw += sums[0][w]; sq += sums[1][w]; }
+        if (A.hist && blockIdx.x == 0 && threadIdx.x == 0) {
+            A.hist[2 * A.step] = sw;
+            A.hist[2 * A.step + 1] = sq;
+        }
+        inv_mean = (double)A.n / sw;
     }
     // g_i = -(w_i / <w>)^2 / n as -(w_i * inv_mean)^2 * inv_n
-    const double inv_mean = A.g ? 1.0 : (double)A.n / A.rec[kRS], inv_n = 1.0 / (double)A.n;
+    const double inv_n = 1.0 / (double)A.n;
     // one-hot fragments of the transposes: step kk, column 32 (kk / 2) + c, a one in the slot
     // that carries row 32 (kk / 2) + c
     bf16x8 I[4];
@@ -498,10 +519,10 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_bwd(const BwdArgs A)
 }  // namespace
 
 hipError_t launch_flow_bwd(hipStream_t stream, uint32_t blocks, const FwdLayer *fl, const BwdExtra *bl, uint64_t n,
-                           const float *xs, uint64_t xstride, const float *g, const float *w, const double *rec,
-                           float *abar, float *gp, uint32_t gp_stride, double *hist, int step)
+                           const float *xs, uint64_t xstride, const float *g, const float *w, const double *parts,
+                           uint32_t nparts, float *abar, float *gp, uint32_t gp_stride, double *hist, int step)
 {
-    const BwdArgs a{fl, bl, n, xs, xstride, g, w, rec, abar, gp, gp_stride, hist, step};
+    const BwdArgs a{fl, bl, n, xs, xstride, g, w, parts, nparts, abar, gp, gp_stride, hist, step};
     hipLaunchKernelGGL(k_flow_bwd, dim3(blocks), dim3(64 * FWB), 0, stream, a);
     return hipGetLastError();
 }

@@ -214,10 +214,11 @@ __device__ __forceinline__ int cond0(int d) { return d == 0 ? 1 : 0; }
 __device__ __forceinline__ int cond1(int d) { return d == 2 ? 1 : 2; }
 
 // The backward kernel over all layers (mi_flow_bwd.hip): g (per point) or, with g null, the
-// variance-loss coefficient from the weights w and the batch record rec; gradient rows of layer l
-// at gp + (l * gp_stride + block) * FPL; hist (optional) receives the batch's (sum w, sum w^2).
+// variance-loss coefficient from the weights w and the per-block sums of w and w^2 in parts
+// (record layout, nparts records); gradient rows of layer l at gp + (l * gp_stride + block) * FPL;
+// hist (optional) receives the batch's (sum w, sum w^2).
 hipError_t launch_flow_bwd(hipStream_t stream, uint32_t blocks, const FwdLayer *fl, const BwdExtra *bl, uint64_t n,
-                           const float *xs, uint64_t xstride, const float *g, const float *w, const double *rec,
-                           float *abar, float *gp, uint32_t gp_stride, double *hist, int step);
+                           const float *xs, uint64_t xstride, const float *g, const float *w, const double *parts,
+                           uint32_t nparts, float *abar, float *gp, uint32_t gp_stride, double *hist, int step);
 
 }  // namespace mi

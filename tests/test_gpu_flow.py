@@ -80,6 +80,31 @@ def test_backward_vs_autograd(ctx):
         assert rel < 3e-2 and cos > 0.999, (l, rel.item(), cos.item())
 
 
+def test_training_updates_the_weight_images_like_a_full_pack(ctx):
+    """The training step writes each updated parameter straight into the kernels' BF16 weight images;
+    a full repack of the same parameters (flow_set_params) must leave the forward, inverse and
+    backward results unchanged bit for bit."""
+    phys = mi.physics_default(1)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 18, key=42), 4, 1.5)
+    ctx.flow_init(3)
+    ctx.flow_train(mi.make_run(phys, 1 << 18, key=43), 5, 1 << 14, 5e-3)
+    gen = torch.Generator().manual_seed(5)
+    y = torch.rand(4096, 3, generator=gen).cuda()
+    g = (torch.randn(4096, generator=gen) / 4096).cuda()
+    z1, l1 = ctx.flow_forward(y)
+    y1, m1 = ctx.flow_inverse(z1)
+    g1 = ctx.flow_backward(y, g)
+    ctx.flow_set_params(ctx.flow_get_params())
+    z2, l2 = ctx.flow_forward(y)
+    y2, m2 = ctx.flow_inverse(z1)
+    g2 = ctx.flow_backward(y, g)
+    assert torch.equal(z1, z2) and torch.equal(l1, l2)
+    assert torch.equal(y1, y2) and torch.equal(m1, m2)
+    assert torch.equal(g1, g2)
+    assert not torch.equal(z1, y)                   # the trained flow is not the identity
+
+
 def test_training_and_unbiased_integration(ctx, O):
     phys = mi.physics_default(1)
     ctx.vegas_init(3, 64)
