@@ -5,9 +5,10 @@
 // Protocol of a solve: a variance pilot of P points fixes the points a final pass needs,
 // N = max(N_min, ceil(1.1 P (rel / target)^2)) (the pilot's relative error rel; 10% margin); the
 // final pass alone is the estimate.  Plain VEGAS: pilot + final.  Flow + VEGAS: the VEGAS pilot
-// (the fallback's own projection), an identity-flow probe for the flow's cost per point, training
-// in chunks, a flow pilot, the final pass -- or, whenever the flow cannot beat it, the VEGAS final
-// pass (plain VEGAS stays the fallback).
+// (the fallback's own projection), a VEGAS pass of one training batch (with the pilot, VEGAS's
+// affine pass cost), an identity-flow probe for the flow's cost per point, training in chunks, a
+// flow pilot, the final pass -- or, whenever the flow cannot beat it, the VEGAS final pass (plain
+// VEGAS stays the fallback).
 #pragma once
 
 #include <cmath>
@@ -31,16 +32,39 @@ inline uint64_t solve_points_for_cv(double cv, uint64_t n_pilot, double target, 
     return solve_final_points(cv / std::sqrt((double)n_pilot), n_pilot, target, n_min);
 }
 
-// What the flow still costs at best once it has been trained enough: its pilot and a final pass
-// of the points its coefficient of variation cv needs (ms_pt: the flow's cost per point).
-inline double solve_flow_rest_ms(double cv, double ms_pt, uint64_t n_pilot, double target, uint64_t n_min)
+
+// Affine cost of a pass of n points, t0 + per_pt n: the fixed part is the launches, the reduction
+// and the ramp of the grid, which a pass of 2^20 points still spends as much time on as on its
+// points (measured on MI355X: 12.7 us + 11.3 ps per VEGAS point).
+struct PassCost {
+    double t0, per_pt;
+};
+inline double pass_ms(const PassCost &c, double n) { return c.t0 + c.per_pt * n; }
+
+// The affine cost from two passes of n_small < n_large points; with degenerate or noisy timings
+// (no larger time for the larger pass) the large pass's average per point.
+inline PassCost solve_affine(double t_small, uint64_t n_small, double t_large, uint64_t n_large)
 {
-    return ms_pt * ((double)n_pilot + (double)solve_points_for_cv(cv, n_pilot, target, n_min));
+    if (n_large > n_small && t_large > t_small) {
+        const double c = (t_large - t_small) / (double)(n_large - n_small);
+        const double t0 = t_small - c * (double)n_small;
+        return {t0 > 0.0 ? t0 : 0.0, c};
+    }
+    return {0.0, n_large > 0 ? t_large / (double)n_large : 0.0};
+}
+
+// What the flow still costs once trained to the coefficient of variation cv: its pilot and a final
+// pass of the points cv needs, at the flow's pass cost.
+inline double solve_flow_rest_ms(double cv, const PassCost &flow, uint64_t n_pilot, double target, uint64_t n_min)
+{
+    return pass_ms(flow, (double)n_pilot) + pass_ms(flow, (double)solve_points_for_cv(cv, n_pilot, target, n_min));
 }
 
 // Before any training: the cheapest possible continuation of the flow -- one chunk of training,
 // which samples at least chunk * batch points through the flow, then its pilot and the smallest
-// final pass (a perfect flow) -- against the VEGAS final pass.  True: fall back now.
+// final pass (a perfect flow) -- against the VEGAS final pass.  True: fall back now.  ms_pt: a
+// cost per point no flow point can undercut (VEGAS's: a flow point evaluates the same integrand
+// through the same map, plus the flow).
 inline bool solve_flow_hopeless(double vegas_final_ms, double flow_ms_pt, int chunk, uint64_t batch,
                                 uint64_t n_pilot, uint64_t n_min)
 {
@@ -48,19 +72,41 @@ inline bool solve_flow_hopeless(double vegas_final_ms, double flow_ms_pt, int ch
     return !(lb < vegas_final_ms);
 }
 
-// After a training chunk that cost chunk_ms and moved the flow's projected rest from prev_rest to
-// rest: train another chunk?  Only while steps remain, the last chunk saved more than it cost, and
-// another chunk saving as much again would end below the cheaper of the two ways to finish now
-// (the flow's rest, VEGAS's final pass): training is judged against plain VEGAS, not only against
-// the flow's own previous state.
-inline bool solve_keep_training(double prev_rest, double rest, double chunk_ms, int steps, int max_steps,
-                                double vegas_final_ms)
+// The same with an affine lower cost: each training step is at least one pass of batch points,
+// then the flow's pilot and its smallest final pass.
+inline bool solve_flow_hopeless_passes(double vegas_final_ms, const PassCost &lower, int chunk, uint64_t batch,
+                                       uint64_t n_pilot, uint64_t n_min)
 {
-    if (steps >= max_steps) return false;
-    const double saving = prev_rest - rest;
-    if (!(saving > chunk_ms)) return false;
-    const double best_now = rest < vegas_final_ms ? rest : vegas_final_ms;
-    return rest - saving + chunk_ms < best_now;
+    const double lb = (double)chunk * pass_ms(lower, (double)batch) + pass_ms(lower, (double)n_pilot) +
+                      pass_ms(lower, (double)n_min);
+    return !(lb < vegas_final_ms);
+}
+
+// After k >= 1 training chunks: rest[0..k] is the flow's projected rest (pilot + final pass)
+// before training and after each chunk.  Train another chunk?  The rest is taken to keep falling
+// at the geometric rate the last (up to three) chunks show, r = (rest[k] / rest[k-w])^(1/w); train
+// on while steps remain and some number j >= 1 of further chunks, chunk_ms each, is projected to
+// end below the cheaper of the two ways to finish now (the flow's rest, VEGAS's final pass).
+// Training is judged against plain VEGAS, not only against the flow's own state, and the window
+// keeps one noisy batch (the coefficients of variation are estimated on one batch each) from
+// ending it.
+inline bool solve_keep_training(const double *rest, int k, double chunk_ms, int steps, int max_steps,
+                                int chunk_steps, double vegas_final_ms)
+{
+    if (k < 1 || steps >= max_steps || chunk_steps < 1) return false;
+    const int w = k < 3 ? k : 3;
+    const double now = rest[k], then = rest[k - w];
+    if (!(now > 0.0) || !(then > 0.0)) return false;
+    const double r = std::pow(now / then, 1.0 / (double)w);
+    if (!(r < 1.0)) return false;
+    const int left = (max_steps - steps) / chunk_steps;
+    const double best_now = now < vegas_final_ms ? now : vegas_final_ms;
+    double rj = 1.0;
+    for (int j = 1; j <= left; ++j) {
+        rj *= r;
+        if ((double)j * chunk_ms + now * rj < best_now) return true;
+    }
+    return false;
 }
 
 // The final pass: the flow (its pilot and final) only if that projection is below VEGAS's final.

@@ -353,15 +353,20 @@ int mi_reweight_events(mi_ctx *ctx, const mi_run *run, uint64_t n, const uint64_
  * top of VEGAS and plain VEGAS as the fallback.  The context's VEGAS grid is used as it is (adapt it
  * first with mi_vegas_adapt).  Passes, on consecutive counter ranges from run->counter0 (training on
  * counters from run->counter0 + 2^41):
- *   VEGAS pilot (pilot_points) -> the points N_V a VEGAS final pass needs and its cost;
- *   sampler = MI_SAMPLER_FLOW: fall back at once, with no flow pass at all, if one training chunk
- *     (chunk_steps x batch points through the flow) plus the flow's pilot and smallest final pass
- *     would cost more than the VEGAS final pass even at VEGAS's own cost per point (a flow point
- *     evaluates the same integrand through the same map, plus the flow); else mi_flow_init(flow_seed)
- *     and an identity-flow probe (probe_points) for the flow's cost per point, the same test with
- *     it; else train in chunks while the last chunk saved more than it cost on the projected flow
- *     pilot + final pass and another chunk saving as much would end below both that projection and
- *     VEGAS's final pass; then the flow (pilot, final pass) if its projection is below VEGAS's
+ *   VEGAS pilot (pilot_points) -> the points N_V a VEGAS final pass needs;
+ *   sampler = MI_SAMPLER_FLOW, each test falling back to VEGAS with no further flow work:
+ *     - one training chunk (chunk_steps x batch points through the flow) plus the flow's pilot and
+ *       smallest final pass would cost more than the VEGAS final pass even at the pilot's cost per
+ *       point (a flow point evaluates the same integrand through the same map, plus the flow);
+ *     - a VEGAS pass of batch points: with the pilot, VEGAS's affine pass cost t0 + c n (a pass
+ *       has a fixed cost of launches and reductions); the same test with chunk_steps such passes;
+ *     - mi_flow_init(flow_seed) and an identity-flow probe (probe_points) for the flow's pass cost
+ *       (t0 + its own cost per point); the same test at that cost;
+ *     - one training step: the rest of the first chunk at its cost, plus the flow's pilot and
+ *       smallest final pass, against VEGAS's final pass;
+ *     then training in chunks while the flow's projected pilot + final pass, falling at the rate
+ *     the last (up to three) chunks show, reaches below both that projection and VEGAS's final pass
+ *     within the steps left; then the flow (pilot, final pass) if its projection is below VEGAS's
  *     final pass, otherwise VEGAS;
  *   the final pass of max(min_points, 1.1 P (rel_pilot / rel_target)^2) points: its estimate is out.
  * The reference has no such driver: it integrates fixed sizes (cuPythia/11_vegas_importance/
@@ -371,7 +376,7 @@ typedef struct mi_solve_opts {
     double   rel_target;     /* sigmaErr / sigmaGen of the final pass, in (0, 1) */
     int      sampler;        /* MI_SAMPLER_VEGAS or MI_SAMPLER_FLOW (flow + VEGAS, VEGAS fallback) */
     int      chunk_steps;    /* training steps between decisions [10] */
-    int      max_steps;      /* [200] */
+    int      max_steps;      /* [400] */
     int      force_flow;     /* 1: train max_steps and finish with the flow regardless (measurement) */
     uint64_t pilot_points;   /* [2^20] */
     uint64_t probe_points;   /* [2^20] */
@@ -384,7 +389,8 @@ typedef struct mi_solve_opts {
 #define MI_SOLVE_VEGAS            0   /* plain VEGAS was asked for */
 #define MI_SOLVE_FALLBACK_PROBE   1   /* flow asked for; VEGAS's final pass is cheaper than any continuation of the flow
                                          (from VEGAS's cost per point, or the probe's: probe_ms tells which) */
-#define MI_SOLVE_FALLBACK_TRAINED 2   /* flow trained, then its projected pilot + final pass cost more than VEGAS's */
+#define MI_SOLVE_FALLBACK_TRAINED 2   /* flow trained (one step or more), then no continuation was projected to
+                                         finish below VEGAS's final pass */
 #define MI_SOLVE_FLOW             3   /* the flow finished the solve */
 #define MI_SOLVE_FLOW_FORCED      4   /* force_flow */
 
@@ -395,7 +401,7 @@ typedef struct mi_solve_info {
     int      reserved;
     double   gpu_ms;               /* device time of every pass of the solve (adaptation not included) */
     double   pilot_ms, probe_ms, train_ms, flow_pilot_ms, final_ms;
-    double   vegas_ms_per_point, flow_ms_per_point;
+    double   vegas_ms_per_point, flow_ms_per_point;   /* per-point part of the pass costs (flow arm: affine) */
     double   cv_vegas, cv_flow;    /* per-point coefficients of variation (flow: its pilot, else the last training batch) */
     double   vegas_final_ms_proj;  /* projected VEGAS final pass */
     double   flow_rest_ms_proj;    /* projected flow pilot + final pass after training (0 if not trained) */
