@@ -80,6 +80,37 @@ def test_backward_vs_autograd(ctx):
         assert rel < 3e-2 and cos > 0.999, (l, rel.item(), cos.item())
 
 
+@pytest.mark.parametrize("n", [1, 63, 65, 1000])
+def test_ragged_sizes(ctx, n):
+    """Sizes that leave a tile part-filled: the dead lanes of the last tile must neither write nor
+    contribute.  Forward and inverse against the torch reference point by point, the gradient
+    against FP32 autograd, and the gradient of the first n of a larger batch with g = 0 beyond n
+    equal to the gradient at size n."""
+    p = _noisy(6, 0.1)
+    ctx.flow_set_params(p.numpy())
+    gen = torch.Generator().manual_seed(100 + n)
+    y = torch.rand(n, 3, generator=gen)
+    z, ld = ctx.flow_forward(y.cuda())
+    zr, ldr = F.forward(y, p)
+    assert z.shape == (n, 3) and ld.shape == (n,)
+    assert (z.cpu() - zr).abs().max() < 2e-3 and (ld.cpu() - ldr).abs().max() < 2e-2
+    yb, _ = ctx.flow_inverse(z)
+    assert (yb.cpu() - y).abs().max() < 2e-3
+    g = torch.randn(n, generator=gen) / n
+    grad = ctx.flow_backward(y.cuda(), g.cuda()).cpu()
+    pr = p.clone().requires_grad_(True)
+    _, ldt = F.forward(y, pr)
+    (gr,) = torch.autograd.grad((g * ldt).sum(), pr)
+    rel = (grad - gr).norm() / gr.norm()
+    assert rel < 3e-2, rel.item()
+    # the same points inside a 2n + 64 batch whose other points have g = 0
+    m = 2 * n + 64
+    y2 = torch.cat([y, torch.rand(m - n, 3, generator=gen)])
+    g2 = torch.cat([g, torch.zeros(m - n)])
+    grad2 = ctx.flow_backward(y2.cuda(), g2.cuda()).cpu()
+    assert torch.allclose(grad2, grad, rtol=1e-4, atol=1e-6 * grad.abs().max().item())
+
+
 def test_training_updates_the_weight_images_like_a_full_pack(ctx):
     """The training step writes each updated parameter straight into the kernels' BF16 weight images;
     a full repack of the same parameters (flow_set_params) must leave the forward, inverse and
