@@ -96,6 +96,26 @@ def test_grid_integral_vs_quadrature(ctx, O, grid_files):
     assert abs(et.sigmaGen() - e.sigmaGen()) > 5 * e.sigmaErr()
 
 
+def test_grid_flow_sampler(ctx, O, grid_files):
+    """The flow sampler with grid PDFs: a short training, then 2^22 points through the flow,
+    sigma within 3 sigma_stat of the grid quadrature; the same events with the hardware
+    transcendentals (MI_PREC_FP32_HW) within 1e-5 of the FP32 estimate."""
+    main, _ = grid_files
+    ctx.pdf_load(main)
+    g = O.LhaGrid(main)
+    q, _ = O.had_quadrature(O.hadcfg_default(grid=g), 12, 12, 24)
+    phys = mi.physics_default(1, pdf_source=mi.MI_PDF_GRID)
+    ctx.vegas_init(3, 64)
+    ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 6, 1.5)
+    ctx.flow_init(9)
+    info = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), 20, 1 << 16, 2e-3)
+    assert math.isfinite(info.loss_last) and info.loss_last < info.loss_first
+    e = ctx.integrate(mi.make_run(phys, 1 << 22, key=79, sampler=mi.MI_SAMPLER_FLOW))
+    assert abs(e.sigmaGen() - q) < 3.0 * e.sigmaErr(), (e.sigmaGen(), e.sigmaErr(), q)
+    eh = ctx.integrate(mi.make_run(phys, 1 << 22, key=79, sampler=mi.MI_SAMPLER_FLOW, precision=mi.MI_PREC_FP32_HW))
+    assert eh.sigmaGen() == pytest.approx(e.sigmaGen(), rel=1e-5)
+
+
 @pytest.mark.parametrize("which,n,sampler,mask", [
     (0, 1 << 19, mi.MI_SAMPLER_FLAT, 0x1F),
     (1, (1 << 19) + 4096, mi.MI_SAMPLER_FLAT, 0x3F),        # an odd count of 256-wide eval blocks
