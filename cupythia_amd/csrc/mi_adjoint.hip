@@ -31,14 +31,14 @@ template <typename T, int F> struct FamAux;
 template <int F> struct FamAux<float, F> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0, float u1,
                                                  float o1, float u2, float o2, float *ws, PointAux<float> *ax,
-                                                 const GridPdf<float> *gp)
-    { return had_family_f32<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp); }
+                                                 const GridPdf<float> *gp, bool grid)
+    { return had_family_f32<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp, grid); }
 };
 template <int F> struct FamAux<double, F> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
-                                                  PointAux<double> *ax, const GridPdf<double> *gp)
-    { return had_family_f64<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp); }
+                                                  PointAux<double> *ax, const GridPdf<double> *gp, bool grid)
+    { return had_family_f64<F>(k, mask, u0, o0, u1, o1, u2, o2, ws, nullptr, nullptr, nullptr, nullptr, ax, gp, grid); }
 };
 
 // Derivatives of one family's weights at a point: the coupling terms hold for every code
@@ -51,7 +51,7 @@ __device__ __forceinline__ void fam_derivs(const HadConst<T> &k, const AdjConst<
 {
     T ws[kMaxProc];
     PointAux<T> ax;
-    const T tf = FamAux<T, F>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, &ax, gp);
+    const T tf = FamAux<T, F>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, &ax, gp, GRID);
     tot += tf;
     if (k.as_order == (T)0) {
         d_as += ac.as_fac * tf / ax.alpS;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(HadConst<T> k, AdjConst<T> a
     double v[kRec];
 #pragma unroll
     for (int i = 0; i < kRec; ++i) v[i] = 0.0;
-    const GridPdf<T> *gp = GRID ? &gpdf : nullptr;
+    const GridPdf<T> *gp = &gpdf;
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
     for (uint32_t j = 0; j < ppt; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;

@@ -263,7 +263,7 @@ bool had_window(const mi_ctx *ctx, const mi_run *run, bool vegas, bool hist, Gri
         run->phys.pdf_source != MI_PDF_GRID || ctx->lds_max <= 0)
         return false;
     const size_t tab = vegas ? (sizeof(float) * 9 * ctx->grid.nbins + 7) & ~(size_t)7 : 0;
-    const size_t hb = (vegas && hist) ? sizeof(double) * (kWinBlock / 64) * 3 * ctx->grid.nbins : 0;
+    const size_t hb = (vegas && hist) ? (kWinBlock / 64) * wave_hist_bytes(3, ctx->grid.nbins) : 0;
     const size_t off = (tab + hb + 15) & ~(size_t)15;
     const size_t stat = (size_t)(kWinBlock / 64) * kRec * sizeof(double) + 1024;   // block_record + slack
     if ((size_t)ctx->lds_max <= stat) return false;

@@ -579,7 +579,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
                                                 float u0, float om0, float u1, float om1,
                                                 float u2, float om2, float* ws, float* sHo, float* tHo,
                                                 float* x1o, float* x2o, PointAux<float>* aux,
-                                                const GridPdf<float>* gp)
+                                                const GridPdf<float>* gp, bool grid)
 {
     const float LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
     const float targ = -(u0 * LT);
@@ -622,7 +622,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     const float ia = b * inv_ab, ib = a * inv_ab;
 
     float Lgg, Lqg, LQQ, Lsame, Lopp, l1a = 0.0f, l1b = 0.0f, g1, g2, Q1, Q2s;
-    if (gp) {
+    if (grid) {
         // PDF grid at Q2_F = factFac^2 pT2 (q = sqrt, ln q = ln Q2_F / 2)
         const float Q2F = gp->factFac2 * pT2;
         const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lx1, x2, lx2, fsqrt<HW>(Q2F),
@@ -711,25 +711,25 @@ template <bool HEAVY = false, bool HW = false>
 __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
                                                float u0, float om0, float u1, float om1,
                                                float u2, float om2, float* ws, float* sHo, float* tHo,
-                                               float* x1o, float* x2o, PointAux<float>* aux = nullptr,
-                                               const GridPdf<float>* gp = nullptr)
+                                               float* x1o, float* x2o, PointAux<float>* aux,
+                                               const GridPdf<float>* gp, bool grid)
 {
     if (!HEAVY) {
 #pragma unroll
         for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0f;
-        return had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        return had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     }
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0f;
     float tot = 0.0f;
     if (mask & kFamMask[0])
-        tot += had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        tot += had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     if (mask & kFamMask[1])
         tot += had_family_f32<1, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
-                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp);
+                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp, grid);
     if (mask & kFamMask[2])
         tot += had_family_f32<2, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
-                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp);
+                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp, grid);
     return tot;
 }
 
@@ -816,7 +816,7 @@ __device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int 
                                                  double u0, double om0, double u1, double om1,
                                                  double u2, double om2, double* ws, double* sHo,
                                                  double* tHo, double* x1o, double* x2o,
-                                                 PointAux<double>* aux, const GridPdf<double>* gp)
+                                                 PointAux<double>* aux, const GridPdf<double>* gp, bool grid)
 {
     const double LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
     const double targ = -(u0 * LT);
@@ -839,7 +839,7 @@ __device__ __forceinline__ double had_family_f64(const HadConst<double>& k, int 
 
     Lumi<double> L;
     double l1a = 0.0, l1b = 0.0;
-    if (gp) {
+    if (grid) {
         const double Q2F = gp->factFac2 * pT2;
         const double q = sqrt(Q2F);
         L = grid_lumi<double>(*gp, (int)(0.5 * k.nSea), x1, lx1, x2, lx2, q, log(q));
@@ -874,25 +874,24 @@ __device__ __forceinline__ double had_point_f64(const HadConst<double>& k, int m
                                                 double u0, double om0, double u1, double om1,
                                                 double u2, double om2, double* ws, double* sHo,
                                                 double* tHo, double* x1o, double* x2o,
-                                                PointAux<double>* aux = nullptr,
-                                                const GridPdf<double>* gp = nullptr)
+                                                PointAux<double>* aux, const GridPdf<double>* gp, bool grid)
 {
     if (!HEAVY) {
 #pragma unroll
         for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0;
-        return had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        return had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     }
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0;
     double tot = 0.0;
     if (mask & kFamMask[0])
-        tot += had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp);
+        tot += had_family_f64<0>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     if (mask & kFamMask[1])
         tot += had_family_f64<1>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
-                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp);
+                                 x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp, grid);
     if (mask & kFamMask[2])
         tot += had_family_f64<2>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
-                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp);
+                                 x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp, grid);
     return tot;
 }
 

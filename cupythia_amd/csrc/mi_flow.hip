@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
         }
         float ws[kMaxProc];
         const float f = had_point_f32<HEAVY, HW>(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr,
-                                                 nullptr, nullptr, nullptr, gpdf.active ? &gpdf : nullptr);
+                                                 nullptr, nullptr, nullptr, &gpdf, gpdf.active);
         const float wt = f * jac * expf(ld[i]);
         if (wout) wout[i] = wt;
         const double dw = (double)wt;

@@ -33,6 +33,12 @@ __host__ __device__ constexpr int proc_code(int i) { return i < kLightProc ? 111
 constexpr int kBlock = 256;
 constexpr uint64_t kUnwChunk = 1ULL << 27;   // unweighting candidates per chunk (stored weights)
 constexpr int kMaxBins = 256;
+// Bytes of one wave's LDS area for the ordered VEGAS histogram over dims dimensions: the
+// dims x nbins FP64 histogram, then one u64 lane mask per bin.
+__host__ __device__ constexpr size_t wave_hist_bytes(int dims, int nbins)
+{
+    return (size_t)(dims + 1) * nbins * 8;
+}
 
 struct Grid {
     int ndim = 0, nbins = 0;

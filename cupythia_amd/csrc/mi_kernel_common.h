@@ -22,6 +22,75 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
+// Per-wave histogram of w^2 over D dimensions in which every addition happens in an order the
+// code fixes.  A batch is one point per lane (w2 = 0 for a lane without a point: it adds nothing).
+// For each dimension, every lane with w2 != 0 ORs its lane bit into the group mask of its bin (an
+// OR gives the same mask in any order), reads the mask back and counts the lanes below it: its
+// rank in the bin.  Round t then adds the w2 of the rank-t lanes, which lie in distinct bins, so
+// each bin takes its batch's values in increasing lane order, batches in sequence, and the sums
+// do not depend on the order in which the hardware applies atomics to one address.  A batch takes
+// as many rounds per dimension as its largest number of points in one bin.  The masks are cleared
+// after each dimension, so one mask per bin serves every dimension.
+//
+// The wave's LDS area holds the histogram h[d][bin] (FP64), then the masks m[bin] (u64).
+template <int D>
+struct WaveHist {
+    __host__ __device__ static constexpr size_t area_bytes(int nbins) { return wave_hist_bytes(D, nbins); }
+
+    // every lane of the wave, before the first batch
+    __device__ static void init(unsigned char *area, int nbins)
+    {
+        double *h = reinterpret_cast<double *>(area);
+        uint64_t *m = reinterpret_cast<uint64_t *>(area + (size_t)D * nbins * 8);
+        for (int i = threadIdx.x & 63; i < D * nbins; i += 64) h[i] = 0.0;
+        for (int i = threadIdx.x & 63; i < nbins; i += 64) m[i] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // every lane of the wave, once per batch
+    __device__ static void add(double w2, const int *bins, unsigned char *area, int nbins)
+    {
+        double *h = reinterpret_cast<double *>(area);
+        uint64_t *m = reinterpret_cast<uint64_t *>(area + (size_t)D * nbins * 8);
+        const uint64_t bit = 1ull << (threadIdx.x & 63);
+        const bool has = w2 != 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            uint64_t *mb = m + bins[d];
+            if (has) __hip_atomic_fetch_or(mb, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const uint64_t grp = has ? __hip_atomic_load(mb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+            // lanes below this one in its bin (< 64, so the rounds end); 64 for a lane that adds nothing
+            const uint32_t rank =
+                has ? __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u))
+                    : 64u;
+            double *hb = h + d * nbins + bins[d];
+            // The first kUnrolled rounds without a test (a round with no lane of its rank skips its
+            // add).  The fence after each round keeps the rounds' adds in separate instructions: a
+            // thread takes at most one of them, so without it they may be merged into one add that
+            // all lanes of rank < kUnrolled issue together.
+            constexpr uint32_t kUnrolled = 4;
+#pragma unroll
+            for (uint32_t t = 0; t < kUnrolled; ++t) {
+                if (rank == t) __hip_atomic_fetch_add(hb, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            }
+            for (uint32_t t = kUnrolled; __ballot(rank >= t && rank < 64u) != 0; ++t) {
+                if (rank == t) __hip_atomic_fetch_add(hb, w2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (has) __hip_atomic_store(mb, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+    }
+
+    // the wave's histogram as [d][bin], once the block has synchronised
+    __device__ static const double *hist(const unsigned char *area) { return reinterpret_cast<const double *>(area); }
+};
+
 // Combine kRec per-thread values into one block record (sum except the max slots).
 // If cnt is given, it receives slot kRAcc (the unweighting accept count) as an integer.
 template <int NT = kBlock>
@@ -126,22 +195,23 @@ __device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbin
     return m;
 }
 
-// gp: the PDF grid, or nullptr for the toy PDFs.  HEAVY: see had_point_f32.  HW: the FP32 point
+// grid: the PDFs of the grid *gp, else the toy PDFs (gp is then not read).  The choice is a flag
+// rather than a null test of gp, which keeps a kernel argument's view out of scratch memory.  HEAVY: see had_point_f32.  HW: the FP32 point
 // with the hardware transcendentals (MI_PREC_FP32_HW); the FP64 point ignores it.
 template <typename T, bool HEAVY, bool HW = false> struct HadPoint;
 template <bool HEAVY, bool HW> struct HadPoint<float, HEAVY, HW> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
                                                  float u1, float o1, float u2, float o2, float *ws,
                                                  float *sH, float *tH, float *x1, float *x2,
-                                                 const GridPdf<float> *gp)
-    { return had_point_f32<HEAVY, HW>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
+                                                 const GridPdf<float> *gp, bool grid)
+    { return had_point_f32<HEAVY, HW>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp, grid); }
 };
 template <bool HEAVY, bool HW> struct HadPoint<double, HEAVY, HW> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
                                                   double *sH, double *tH, double *x1, double *x2,
-                                                  const GridPdf<double> *gp)
-    { return had_point_f64<HEAVY>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp); }
+                                                  const GridPdf<double> *gp, bool grid)
+    { return had_point_f64<HEAVY>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp, grid); }
 };
 
 // One candidate of the hadronic integrand: Philox stream 0 of event e -> three 24-bit
@@ -151,7 +221,7 @@ template <bool HEAVY, bool HW> struct HadPoint<double, HEAVY, HW> {
 template <typename T, bool VEGAS, bool HEAVY = false, bool HW = false>
 __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint64_t key, uint64_t e,
                                            const T *stab, int nbins, T nbf, T *ws, int *bins,
-                                           T *sH, T *tH, T *x1, T *x2, const GridPdf<T> *gp = nullptr)
+                                           T *sH, T *tH, T *x1, T *x2, const GridPdf<T> *gp, bool grid)
 {
     const U4 r = philox_event(key, e, 0u);
     MapOut<T> m0, m1, m2;
@@ -162,7 +232,7 @@ __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint6
     } else {
         m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
     }
-    T tot = HadPoint<T, HEAVY, HW>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp);
+    T tot = HadPoint<T, HEAVY, HW>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp, grid);
     if (VEGAS) {
         T jac = (T)1;
         jac = jac * m0.jac;

@@ -138,12 +138,11 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
     unsigned char *smem = mi_lds;
     T *stab = reinterpret_cast<T *>(smem);                                   // 3 dims x nbins x {lo, wd, 1-lo}
     const size_t tabBytes = (sizeof(T) * 9 * (VEGAS ? nbins : 0) + 7) & ~(size_t)7;
-    double *shist = reinterpret_cast<double *>(smem + tabBytes);             // NT/64 waves x 3 dims x nbins
+    const size_t areaBytes = WaveHist<3>::area_bytes(nbins);                 // per wave: its histogram
+    unsigned char *harea = smem + tabBytes;
     const int wave = threadIdx.x >> 6;
     if (VEGAS) {
         for (int i = threadIdx.x; i < 9 * nbins; i += NT) stab[i] = gtab[i];
-        if (HIST)
-            for (int i = threadIdx.x; i < (NT / 64) * 3 * nbins; i += NT) shist[i] = 0.0;
     }
     if (WIN) {
         float4 *dst = reinterpret_cast<float4 *>(smem + gpdf.woff);
@@ -169,15 +168,21 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
     for (int i = 0; i <= NP; ++i) { mx[i] = 0; sw[i] = 0.0; sw2[i] = 0.0; }
     uint32_t nnz = 0;
 
+    if (HIST) WaveHist<3>::init(harea + wave * areaBytes, nbins);
     const uint64_t base = (uint64_t)blockIdx.x * NT * ppt;
     for (uint32_t j = 0; j < ppt; ++j) {
         const uint64_t i = base + (uint64_t)j * NT + threadIdx.x;
-        if (i >= n) break;
+        const bool live = i < n;
+        // the histogram's batches need the whole wave: it leaves the loop together
+        if (HIST ? __ballot(live) == 0 : !live) break;
+        // a lane past the end (HIST only) evaluates the last point again and adds nothing
         T ws[kMaxProc];
         int bins[3];
-        const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + i, stab, nbins, nbf, ws, bins,
-                                                     (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
-                                                     GRID ? &gpdf : nullptr);
+        const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + (live ? i : n - 1), stab, nbins, nbf, ws,
+                                                         bins, (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
+                                                         &gpdf, GRID);
+        if (HIST) WaveHist<3>::add(live ? (double)tot * (double)tot : 0.0, bins, harea + wave * areaBytes, nbins);
+        if (!live) continue;
         const double dt = (double)tot;
         sw[0] += dt;
         sw2[0] = fma(dt, dt, sw2[0]);
@@ -191,13 +196,6 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
             sw2[1 + p] = fma(d, d, sw2[1 + p]);
             mx[1 + p] = run_max(mx[1 + p], ws[p]);
         }
-        if (HIST) {
-            const double w2 = dt * dt;
-            double *h = shist + wave * 3 * nbins;
-            atomicAdd(&h[bins[0]], w2);
-            atomicAdd(&h[nbins + bins[1]], w2);
-            atomicAdd(&h[2 * nbins + bins[2]], w2);
-        }
     }
     double mxd[NP + 1];
 #pragma unroll
@@ -206,8 +204,8 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
     if (HIST) {
         __syncthreads();
         for (int c = threadIdx.x; c < 3 * nbins; c += NT) {
-            double s = shist[c];
-            for (int w = 1; w < NT / 64; ++w) s += shist[w * 3 * nbins + c];
+            double s = WaveHist<3>::hist(harea)[c];
+            for (int w = 1; w < NT / 64; ++w) s += WaveHist<3>::hist(harea + w * areaBytes)[c];
             hist[(size_t)blockIdx.x * 3 * nbins + c] = s;
         }
     }
@@ -217,7 +215,7 @@ static size_t had_smem(const HadLaunch &L)
 {
     if (!L.vegas) return 0;
     const size_t tb = ((L.prec == MI_PREC_FP32 ? sizeof(float) : sizeof(double)) * 9 * L.grid->nbins + 7) & ~(size_t)7;
-    return tb + (L.hist ? sizeof(double) * 4 * 3 * L.grid->nbins : 0);
+    return tb + (L.hist ? (kBlock / 64) * WaveHist<3>::area_bytes(L.grid->nbins) : 0);
 }
 
 template <typename T, bool GRID, bool HEAVY, bool HW = false>
@@ -290,28 +288,31 @@ __global__ __launch_bounds__(kBlock) void k_integrate_c1(double pref, double cMa
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *stab = reinterpret_cast<double *>(smem);
-    double *shist = stab + 3 * (VEGAS ? nbins : 0);
+    unsigned char *harea = smem + sizeof(double) * 3 * (VEGAS ? nbins : 0);
+    const size_t areaBytes = WaveHist<1>::area_bytes(nbins);
     const int wave = threadIdx.x >> 6;
     if (VEGAS) {
         for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) stab[i] = gtab[i];
-        if (HIST)
-            for (int i = threadIdx.x; i < 4 * nbins; i += blockDim.x) shist[i] = 0.0;
         __syncthreads();
     }
+    if (HIST) WaveHist<1>::init(harea + wave * areaBytes, nbins);
     double v[kRec];
 #pragma unroll
     for (int i = 0; i < kRec; ++i) v[i] = 0.0;
     const uint64_t base = (uint64_t)blockIdx.x * kBlock * ppt;
     for (uint32_t j = 0; j < ppt; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        if (i >= n) break;
+        const bool live = i < n;
+        if (HIST ? __ballot(live) == 0 : !live) break;
+        // a lane past the end (HIST only) evaluates the last point again and adds nothing
         int bin;
-        const double w = c1_candidate(pref, cMax, key, ctr0 + i, VEGAS, stab, nbins, &bin);
+        const double w = c1_candidate(pref, cMax, key, ctr0 + (live ? i : n - 1), VEGAS, stab, nbins, &bin);
+        if (HIST) WaveHist<1>::add(live ? w * w : 0.0, &bin, harea + wave * areaBytes, nbins);
+        if (!live) continue;
         v[kRS] += w;
         v[kRQ] = fma(w, w, v[kRQ]);
         v[kRM] = fmax(v[kRM], w);
         v[kRNz] += (w > 0.0) ? 1.0 : 0.0;
-        if (HIST) atomicAdd(&shist[wave * nbins + bin], w * w);
     }
     // gg -> gg is the only process: copy total into slot of code 111
     v[kRS + 1] = v[kRS]; v[kRQ + 1] = v[kRQ]; v[kRM + 1] = v[kRM];
@@ -319,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_c1(double pref, double cMa
     if (HIST) {
         __syncthreads();
         for (int c = threadIdx.x; c < nbins; c += blockDim.x) {
-            double s = shist[c];
-            for (int w = 1; w < kBlock / 64; ++w) s += shist[w * nbins + c];
+            double s = WaveHist<1>::hist(harea)[c];
+            for (int w = 1; w < kBlock / 64; ++w) s += WaveHist<1>::hist(harea + w * areaBytes)[c];
             hist[(size_t)blockIdx.x * nbins + c] = s;
         }
     }
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_c1(double pref, double cMa
 hipError_t launch_integrate_c1(hipStream_t s, const C1Launch &L)
 {
     const int nb = L.vegas ? L.grid->nbins : 0;
-    const size_t sm = L.vegas ? sizeof(double) * (3 * nb + (L.hist ? 4 * nb : 0)) : 0;
+    const size_t sm = L.vegas ? sizeof(double) * 3 * nb + (L.hist ? (kBlock / 64) * WaveHist<1>::area_bytes(nb) : 0) : 0;
     const double *tab = L.vegas ? L.grid->tab64 : nullptr;
     dim3 grid(L.g.blocks), blk(kBlock);
     if (!L.vegas)
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_points_had(HadConst<T> k, int mask, 
     constexpr int NF = HEAVY ? 3 : 1;
     T sH[NF], tH[NF], x1[NF], x2[NF];
     const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + i, gtab, nbins, (T)nbins, ws, bins, sH,
-                                                     tH, x1, x2, gpdf.active ? &gpdf : nullptr);
+                                                     tH, x1, x2, &gpdf, gpdf.active);
     w[i * kRecN] = (double)tot;
 #pragma unroll
     for (int p = 0; p < kMaxProc; ++p) w[i * kRecN + 1 + p] = (double)ws[p];
@@ -659,7 +660,7 @@ int occ_integrate_had(int prec, bool vegas, bool hist, int nbins, bool grid, boo
 int occ_integrate_c1(bool vegas, bool hist, int nbins)
 {
     const int nb = vegas ? nbins : 0;
-    const size_t sm = vegas ? sizeof(double) * (3 * nb + (hist ? 4 * nb : 0)) : 0;
+    const size_t sm = vegas ? sizeof(double) * 3 * nb + (hist ? (kBlock / 64) * WaveHist<1>::area_bytes(nb) : 0) : 0;
     int b = 0;
     hipError_t e;
     if (!vegas) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_integrate_c1<false, false>, kBlock, sm);
@@ -817,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_reweight_regen(const RwRegen *__rest
         int bins[3];
         had_candidate<double, VEGAS, HEAVY>(q.k, mask, key, e, gtab, nbins, (double)nbins, ws, bins, (double *)nullptr,
                                             (double *)nullptr, (double *)nullptr, (double *)nullptr,
-                                            q.g.active ? &q.g : nullptr);
+                                            &q.g, q.g.active);
         double Wv = 0.0;
 #pragma unroll
         for (int p = 0; p < kMaxProc; ++p) Wv = (p == slot) ? ws[p] : Wv;   // selects: no dynamic register index

@@ -101,10 +101,9 @@ __device__ __forceinline__ T eval_cand(const Params<KIND, T> &P, const T *stab, 
         return (T)w;
     }
     int bins[3];
-    const GridPdf<T> *gp = kind_grid(KIND) ? &P.gpdf : nullptr;
     return had_candidate<T, kind_vegas(KIND), kind_heavy(KIND)>(P.k, P.mask, P.key, e, stab, P.nbins, (T)P.nbins, ws,
                                                                 bins, (T *)nullptr, (T *)nullptr, (T *)nullptr,
-                                                                (T *)nullptr, gp);
+                                                                (T *)nullptr, &P.gpdf, kind_grid(KIND));
 }
 
 // The full candidate for an accepted event: weights, kinematics of the chosen process's family
@@ -128,9 +127,8 @@ __device__ __forceinline__ void full_cand(const Params<KIND, T> &P, const T *sta
     int bins[3];
     constexpr int NF = HEAVY ? 3 : 1;
     T sH[NF], tH[NF], x1[NF], x2[NF];
-    const GridPdf<T> *gp = kind_grid(KIND) ? &P.gpdf : nullptr;
     const T tot = had_candidate<T, kind_vegas(KIND), HEAVY>(P.k, P.mask, P.key, e, stab, P.nbins, (T)P.nbins, ws,
-                                                            bins, sH, tH, x1, x2, gp);
+                                                            bins, sH, tH, x1, x2, &P.gpdf, kind_grid(KIND));
     c.w = (double)tot;
     decide<T>(P.key, e, tot, ws, P.mask, P.wmax, c);
     int f = 0;
