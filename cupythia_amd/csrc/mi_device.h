@@ -470,7 +470,9 @@ __device__ inline void grid_qw_f32(const GridPdf<float> &g, float q, float lnq, 
 template <typename P>
 __device__ __forceinline__ void fold_stencil(P p, size_t rs, int n3q, const float *wq, const float *wx, float *out)
 {
-    for (int i3q = 0; i3q < n3q; ++i3q) {
+#pragma unroll
+    for (int i3q = 0; i3q < 4; ++i3q) {             // n3q <= 4 rows; constant indices keep wq in registers
+        if (i3q >= n3q) continue;
 #pragma unroll
         for (int ix = 0; ix < 4; ++ix) {
             const float w = wq[i3q] * wx[ix];
@@ -492,7 +494,9 @@ __device__ inline void grid_xf_f32(const GridPdf<float> &g, float x, float lnx, 
 #pragma unroll
     for (int f = 0; f < 11; ++f) out[f] = 0.0f;
     if (x <= g.xMin) {
-        for (int i3q = 0; i3q < n3q; ++i3q) {
+#pragma unroll
+        for (int i3q = 0; i3q < 4; ++i3q) {
+            if (i3q >= n3q) continue;
             float v[11];
             const bool lds = g.woff >= 0 && g.wx0 == 0 && m3q + i3q >= g.wq0 && m3q + i3q <= g.wq1;
             if (lds)
@@ -549,7 +553,11 @@ __host__ __device__ inline Lumi<T> grid_lumi(const GridPdf<T> &g, int nQ, T x1, 
     }
     Lumi<T> L;
     T Q1 = 0, Q2 = 0, same = 0, opp = 0;
-    for (int k = 1; k <= nQ; ++k) {
+    // a fixed trip count with the flavours beyond nQ skipped: constant indices keep f1, f2 in
+    // registers
+#pragma unroll
+    for (int k = 1; k <= 5; ++k) {
+        if (k > nQ) continue;
         const T q1 = f1[k], a1 = f1[5 + k], q2 = f2[k], a2 = f2[5 + k];
         Q1 = Q1 + (q1 + a1);
         Q2 = Q2 + (q2 + a2);
