@@ -243,7 +243,7 @@ def workloads(ctx, mi):
     # flow on top of the adapted grid
     ctx.flow_init(7)
     steps, batch = 40, 1 << 20
-    fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), steps, batch, 2e-3)
+    fi = ctx.flow_train(mi.make_run(phys, 1 << 20, key=43), steps, batch, 1e-2)
     fe = ctx.integrate(mi.make_run(phys, 1 << 26, key=47, sampler=mi.MI_SAMPLER_FLOW))
     ve = ctx.integrate(mi.make_run(phys, 1 << 26, key=47, sampler=mi.MI_SAMPLER_VEGAS))
     n = 1 << 26

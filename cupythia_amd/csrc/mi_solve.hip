@@ -51,7 +51,7 @@ int mi_solve(mi_ctx *ctx, const mi_run *run, const mi_solve_opts *opts_in, mi_si
     if (o.probe_points == 0) o.probe_points = 1ULL << 20;
     if (o.batch == 0) o.batch = 1ULL << 16;
     if (o.min_points == 0) o.min_points = 1ULL << 16;
-    if (!(o.lr > 0.0)) o.lr = 2e-3;
+    if (!(o.lr > 0.0)) o.lr = 1e-2;
     if (o.chunk_steps < 1 || o.max_steps < o.chunk_steps || o.max_steps > 100000)
         return serr(ctx, MI_ERR_ARG, "need 1 <= chunk_steps <= max_steps <= 100000");
     const uint64_t cap = 1ULL << 40;

@@ -364,9 +364,10 @@ int mi_reweight_events(mi_ctx *ctx, const mi_run *run, uint64_t n, const uint64_
  *       (t0 + its own cost per point); the same test at that cost;
  *     - one training step: the rest of the first chunk at its cost, plus the flow's pilot and
  *       smallest final pass, against VEGAS's final pass;
- *     then training in chunks while the flow's projected pilot + final pass, falling at the rate
- *     the last (up to three) chunks show, reaches below both that projection and VEGAS's final pass
- *     within the steps left; then the flow (pilot, final pass) if its projection is below VEGAS's
+ *     then training in chunks while the flow's projected pilot + final pass reaches below both that
+ *     projection and VEGAS's final pass within the steps left.  The projection starts from the
+ *     rate the last (up to three) chunks show and lets each further chunk gain 0.8 of the
+ *     previous chunk's gain.  Then the flow (pilot, final pass) if its projection is below VEGAS's
  *     final pass, otherwise VEGAS;
  *   the final pass of max(min_points, 1.1 P (rel_pilot / rel_target)^2) points: its estimate is out.
  * The reference has no such driver: it integrates fixed sizes (cuPythia/11_vegas_importance/
@@ -382,7 +383,7 @@ typedef struct mi_solve_opts {
     uint64_t probe_points;   /* [2^20] */
     uint64_t batch;          /* training batch [2^16] */
     uint64_t min_points;     /* smallest final pass [2^16] */
-    double   lr;             /* Adam learning rate [2e-3] */
+    double   lr;             /* Adam learning rate [1e-2] */
     uint64_t flow_seed;
 } mi_solve_opts;
 

@@ -230,10 +230,11 @@ def test_solve_tight_target_probes_then_decides(ctx, O):
 
 
 def test_solve_tight_target_the_trained_flow_wins(ctx, O):
-    """mi_solve at 1e-5 relative error, where plain VEGAS needs ~1e10 points (~90 ms): the flow arm
+    """mi_solve at 1e-5 relative error, where plain VEGAS needs ~1e10 points (~100 ms): the flow arm
     trains (the projected pilot + final pass keeps falling faster than a chunk costs), finishes with
-    the flow at CV ~0.13 instead of VEGAS's 0.96, and spends less device time than plain VEGAS (the
-    training included); both estimates within 3 sigma_stat of the quadrature."""
+    the flow at CV ~0.11 instead of VEGAS's 0.96, and spends well under plain VEGAS's device time
+    (0.41 of it measured on MI355X, the training included); both estimates within 3 sigma_stat of
+    the quadrature."""
     phys = mi.physics_default(1)
     ctx.vegas_init(3, 64)
     ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
@@ -245,7 +246,7 @@ def test_solve_tight_target_the_trained_flow_wins(ctx, O):
     assert jf.decision == mi._abi.MI_SOLVE_FLOW and jf.sampler_used == mi.MI_SAMPLER_FLOW
     assert jf.train_steps >= 100 and jf.cv_flow < 0.2
     assert jf.flow_rest_ms_proj < jf.vegas_final_ms_proj
-    assert jf.gpu_ms < 0.8 * iv.gpu_ms, (jf.as_dict(), iv.as_dict())
+    assert jf.gpu_ms < 0.6 * iv.gpu_ms, (jf.as_dict(), iv.as_dict())
     q, _ = O.had_quadrature(O.hadcfg_default(), 12, 12, 24)
     for e in (ev, ef):
         assert e.sigmaErr() / e.sigmaGen() < 1.05e-5
