@@ -133,7 +133,19 @@ void fill_consts(const mi_physics &p, HadConst<T> &k)
         k.fam[f].pref = (T)(kConvert2mb * kPi * LTf * p.kfactor / fourMT2);
         k.fam[f].m2 = (T)m2;
         k.fam[f].mu = (T)(m2 / fourMT2);
+        k.b2.famLT[f] = (T)(LTf / kLn2);
+        k.b2.famPref[f] = (T)(kConvert2mb * kPi * LTf * p.kfactor / fourMT2 * kLn2);
     }
+    k.b2.LT = (T)(LT / kLn2);
+    k.b2.pref = (T)(kConvert2mb * kPi * LT * p.kfactor / fourPT2 * kLn2);
+    k.b2.lnAg = (T)std::log2(n.Ag); k.b2.lnAs = (T)std::log2(n.As); k.b2.lnAuv = (T)std::log2(n.Auv);
+    if (order == 1) {
+        const AsLambda a = as_lambda(p.alphaS_value);
+        k.b2.lam6 = (T)std::log2(a.L6s); k.b2.lam5 = (T)std::log2(a.L5s);
+        k.b2.lam4 = (T)std::log2(a.L4s); k.b2.lam3 = (T)std::log2(a.L3s);
+    }
+    k.b2.c6 = (T)(12.0 * kPi / 21.0 / kLn2); k.b2.c5 = (T)(12.0 * kPi / 23.0 / kLn2);
+    k.b2.c4 = (T)(12.0 * kPi / 25.0 / kLn2); k.b2.c3 = (T)(12.0 * kPi / 27.0 / kLn2);
 }
 
 int check_physics(mi_ctx *ctx, const mi_physics &p)

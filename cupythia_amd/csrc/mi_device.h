@@ -40,6 +40,7 @@ namespace mi {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kConvert2mb = 0.389380;   // Pythia's GeV^-2 -> mb
+constexpr double kLn2 = 0.69314718055994530942;
 constexpr int kMaxProc = 10;      // slots 0..5 = codes 111..116, 6..9 = codes 121..124
 constexpr int kLightProc = 6;
 constexpr int kFamMask[3] = {0x3F, 0xC0, 0x300};   // proc_mask bits of each family
@@ -56,8 +57,9 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
         const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
         const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        // three-input XOR in one v_bitop3_b32 (truth table 0x96 = odd parity of the inputs)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
@@ -221,31 +223,10 @@ __device__ __forceinline__ float own_logf(float x)
 }
 
 // ------------------------------------------------------------------ FP32 math of the integrand
-// HW = false: the own forms above plus IEEE division and square root (bit-reproducible on the
-// host).  HW = true (MI_PREC_FP32_HW): the hardware's base-2 exponential and logarithm
-// (v_exp_f32, v_log_f32) and its reciprocal and square root (v_rcp_f32, v_sqrt_f32), about 1 ulp
-// each; e^x keeps the rounding error of x log2(e) (|x| up to ~90 here) as a first-order
-// correction, 2^(t + e) = 2^t (1 + e ln 2), so its relative error stays ~2e-7.
-template <bool HW>
-__device__ __forceinline__ float fexp(float x)
-{
-    if constexpr (!HW) {
-        return own_expf(x);
-    } else {
-        const float t = x * 1.4426950408889634f;
-        float e = __builtin_fmaf(x, 1.4426950408889634f, -t);      // exact rounding error of t
-        e = __builtin_fmaf(x, 1.925963033500011e-8f, e);             // log2(e) - RN(log2(e))
-        const float r = __builtin_amdgcn_exp2f(t);
-        const float v = __builtin_fmaf(r, e * 0.6931471805599453f, r);
-        return x < -86.0f ? 0.0f : v;
-    }
-}
-template <bool HW>
-__device__ __forceinline__ float flog(float x)
-{
-    if constexpr (!HW) return own_logf(x);
-    else return (x > 0.0f) ? __builtin_amdgcn_logf(x) * 0.6931471805599453f : -1.0e30f;
-}
+// HW = false: IEEE division and square root (with the own exp / log above: bit-reproducible on
+// the host).  HW = true (MI_PREC_FP32_HW): the hardware's reciprocal and square root (v_rcp_f32,
+// v_sqrt_f32, about 1 ulp each); that mode also takes the hardware's base-2 exponential and
+// logarithm (v_exp_f32, v_log_f32) directly, in the base-2 form of the integrand.
 template <bool HW>
 __device__ __forceinline__ float frcp(float x)
 {
@@ -257,6 +238,21 @@ __device__ __forceinline__ float fsqrt(float x)
 {
     if constexpr (!HW) return sqrtf(x);
     else return __builtin_amdgcn_sqrtf(x);
+}
+
+// 1 - 2^t (t <= 0) without cancellation near 0, for the base-2 form of MI_PREC_FP32_HW; e = 2^t
+// already computed.  For t > -0.50494 (x = t ln 2 > -0.35): one_minus_expf's series
+// -x q(x) written in t, i.e. its coefficients c_k times ln 2^(k+1); below, 1 - e >= 0.295.
+__device__ __forceinline__ float one_minus_exp2f(float t, float e)
+{
+    constexpr double L = kLn2, L2 = L * L, L3 = L2 * L, L4 = L3 * L, L5 = L4 * L, L6 = L5 * L;
+    float p = __builtin_fmaf(t, (float)(0.0011900244 * L6), (float)(0.008254852 * L5));
+    p = __builtin_fmaf(t, p, (float)(0.041652385 * L4));
+    p = __builtin_fmaf(t, p, (float)(0.16666552 * L3));
+    p = __builtin_fmaf(t, p, (float)(0.49999997 * L2));
+    p = __builtin_fmaf(t, p, (float)L);
+    const float series = -(t * p);
+    return (t > -0.50494f) ? series : 1.0f - e;
 }
 
 // ------------------------------------------------------------------ constants of the integrand
@@ -284,8 +280,19 @@ struct HadConst {
         T m2;      // m^2
         T mu;      // m^2 / (4 mT^2), so m^2 / sH = mu r
     } fam[2];
+    // MI_PREC_FP32_HW evaluates the map, alphaS and the toy PDFs in base 2 (v_exp_f32 and
+    // v_log_f32 take and return base-2 values): the constants above that sit in an exponent or
+    // come out of a logarithm, divided by ln 2, and the Jacobian's prefactors times ln 2.
+    struct Base2 {
+        T LT;                      // LT / ln 2
+        T pref;                    // pref ln 2: the Jacobian's LT om0 = ln 2 (LT / ln 2) om0
+        T lnAg, lnAs, lnAuv;       // log2 of the PDF normalisations
+        T lam6, lam5, lam4, lam3;  // ln(Lambda_nf^2) / ln 2
+        T c6, c5, c4, c3;          // (12 pi / b0) / ln 2: alphaS = c / (log2 Q2 - lam)
+        T famLT[2], famPref[2];    // the heavy families' LT / ln 2 and pref ln 2
+    } b2;
 };
-static_assert(sizeof(HadConst<float>) == 42 * sizeof(float), "HadConst layout");
+static_assert(sizeof(HadConst<float>) == 59 * sizeof(float), "HadConst layout");
 
 // ------------------------------------------------------------------ PDF grid (LHAGrid1)
 // A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
@@ -580,8 +587,11 @@ struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 // ------------------------------------------------------------------ FP32 hadronic point
 // One family (FAM 0: the massless codes 111..116 into ws[0..5]; 1, 2: c cbar, b bbar into
 // ws[6..7], ws[8..9]).  Inputs: (u, 1-u) for each dimension (exact pairs); writes the family's
-// slots (0 if masked out) and returns their sum.  Operation-for-operation identical to the
-// host restatement used to check compaction indices.
+// slots (0 if masked out) and returns their sum.  HW = false: operation-for-operation identical
+// to the host restatement used to check compaction indices.  HW = true (MI_PREC_FP32_HW): the
+// map, alphaS and the toy PDFs in base 2, so that v_exp_f32 / v_log_f32 take and return the
+// values directly (log2 tau, log2 x, log2 (1 - x), log2 Q2 and the PDF exponents; HadConst::b2);
+// the grid PDFs still receive ln x, computed as the bit-exact path does.  aux: HW = false only.
 template <int FAM, bool HW = false>
 __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int mask,
                                                 float u0, float om0, float u1, float om1,
@@ -589,19 +599,32 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
                                                 float* x1o, float* x2o, PointAux<float>* aux,
                                                 const GridPdf<float>* gp, bool grid)
 {
-    const float LT = FAM == 0 ? k.LT : k.fam[FAM - 1].LT;
-    const float targ = -(u0 * LT);
-    const float r = fexp<HW>(targ);                        // tauMin / tau
-    const float yMax = 0.5f * (LT * om0);
-    const float twoYMax = yMax + yMax;
-    const float lx1 = -(twoYMax * om1), lx2 = -(twoYMax * u1);
-    const float x1 = fexp<HW>(lx1), x2 = fexp<HW>(lx2);
-    const float zMax = fsqrt<HW>(one_minus_expf(targ, r));
+    constexpr int F = FAM > 0 ? FAM - 1 : 0;                 // heavy family index
+    const float LT = FAM == 0 ? k.LT : k.fam[F].LT;
+    float r, jacY, lx1, lx2, x1, x2, zMax;                 // jacY: 2 yMax in the prefactor's units
+    if constexpr (HW) {
+        const float LT2 = FAM == 0 ? k.b2.LT : k.b2.famLT[F];
+        const float targ = -(u0 * LT2);                    // log2(tauMin / tau)
+        r = __builtin_amdgcn_exp2f(targ);                  // tauMin / tau
+        jacY = LT2 * om0;                                  // 2 yMax / ln 2
+        lx1 = -(jacY * om1); lx2 = -(jacY * u1);           // log2 x1, log2 x2
+        x1 = __builtin_amdgcn_exp2f(lx1); x2 = __builtin_amdgcn_exp2f(lx2);
+        zMax = __builtin_amdgcn_sqrtf(one_minus_exp2f(targ, r));
+    } else {
+        const float targ = -(u0 * LT);
+        r = own_expf(targ);                             // tauMin / tau
+        const float yMax = 0.5f * (LT * om0);
+        const float twoYMax = yMax + yMax;
+        lx1 = -(twoYMax * om1); lx2 = -(twoYMax * u1);
+        x1 = own_expf(lx1); x2 = own_expf(lx2);
+        zMax = fsqrt<false>(one_minus_expf(targ, r));
+        jacY = LT * om0;
+    }
     // one division gives both 1 / r and 1 / (1 + zMax)
     const float opz = 1.0f + zMax;
     const float R1 = frcp<HW>(r * opz);
     const float omz = (r * r) * R1;                        // r / (1 + zMax) = 1 - zMax
-    const float sH = ((FAM == 0 ? k.fourPT2 : k.fam[FAM - 1].fourMT2) * opz) * R1;   // 4 mT^2 / r
+    const float sH = ((FAM == 0 ? k.fourPT2 : k.fam[F].fourMT2) * opz) * R1;   // 4 mT^2 / r
     const float twoz = zMax + zMax;
     const float A = __builtin_fmaf(twoz, om2, omz);        // 1 - z
     const float B = __builtin_fmaf(twoz, u2, omz);         // 1 + z
@@ -616,10 +639,17 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     } else {
         float Q2 = k.xiR2 * pT2;
         Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
-        const float lq = flog<HW>(Q2);
         const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0f, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
-        const float lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
-        const float cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
+        float lq, lam, cf;
+        if constexpr (HW) {                                // base 2: Q2 >= scale2Min > 0
+            lq = __builtin_amdgcn_logf(Q2);
+            lam = t6 ? k.b2.lam6 : (t5 ? k.b2.lam5 : (t4 ? k.b2.lam4 : k.b2.lam3));
+            cf = t6 ? k.b2.c6 : (t5 ? k.b2.c5 : (t4 ? k.b2.c4 : k.b2.c3));
+        } else {
+            lq = own_logf(Q2);
+            lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
+            cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
+        }
         // one division gives both 1 / (a b) and alphaS = cf / (lq - lam)
         const float den = lq - lam;
         const float R2 = frcp<HW>(ab * den);
@@ -633,18 +663,37 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     if (grid) {
         // PDF grid at Q2_F = factFac^2 pT2 (q = sqrt, ln q = ln Q2_F / 2)
         const float Q2F = gp->factFac2 * pT2;
-        const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lx1, x2, lx2, fsqrt<HW>(Q2F),
-                                               0.5f * flog<HW>(Q2F));
+        float lnx1 = lx1, lnx2 = lx2, lnq;
+        if constexpr (HW) {
+            const float twoYMax = LT * om0;                // ln x as the bit-exact path forms it
+            lnx1 = -(twoYMax * om1); lnx2 = -(twoYMax * u1);
+            lnq = (float)(0.5 * kLn2) * __builtin_amdgcn_logf(Q2F);
+        } else {
+            lnq = 0.5f * own_logf(Q2F);
+        }
+        const Lumi<float> L = grid_lumi<float>(*gp, (int)(0.5f * k.nSea), x1, lnx1, x2, lnx2, fsqrt<HW>(Q2F), lnq);
         Lgg = L.gg; Lqg = L.qg; LQQ = L.QQ; Lsame = L.same; Lopp = L.opp;
         g1 = L.g1; g2 = L.g2; Q1 = L.Q1; Q2s = L.Q2;
     } else {
-        l1a = flog<HW>(one_minus_expf(lx1, x1)); l1b = flog<HW>(one_minus_expf(lx2, x2));
-        g1 = fexp<HW>(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
-        g2 = fexp<HW>(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
-        const float S1 = fexp<HW>(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
-        const float S2 = fexp<HW>(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
-        const float V1 = fexp<HW>(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
-        const float V2 = fexp<HW>(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
+        float S1, S2, V1, V2;
+        if constexpr (HW) {                                // log2 xf = log2 A + a log2 x + b log2 (1 - x)
+            l1a = __builtin_amdgcn_logf(one_minus_exp2f(lx1, x1));
+            l1b = __builtin_amdgcn_logf(one_minus_exp2f(lx2, x2));
+            g1 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.b2.lnAg)));
+            g2 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.b2.lnAg)));
+            S1 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.b2.lnAs)));
+            S2 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.b2.lnAs)));
+            V1 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.b2.lnAuv)));
+            V2 = __builtin_amdgcn_exp2f(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.b2.lnAuv)));
+        } else {
+            l1a = own_logf(one_minus_expf(lx1, x1)); l1b = own_logf(one_minus_expf(lx2, x2));
+            g1 = own_expf(__builtin_fmaf(k.bg, l1a, __builtin_fmaf(k.ag, lx1, k.lnAg)));
+            g2 = own_expf(__builtin_fmaf(k.bg, l1b, __builtin_fmaf(k.ag, lx2, k.lnAg)));
+            S1 = own_expf(__builtin_fmaf(k.bs, l1a, __builtin_fmaf(k.as, lx1, k.lnAs)));
+            S2 = own_expf(__builtin_fmaf(k.bs, l1b, __builtin_fmaf(k.as, lx2, k.lnAs)));
+            V1 = own_expf(__builtin_fmaf(k.bv, l1a, __builtin_fmaf(k.av, lx1, k.lnAuv)));
+            V2 = own_expf(__builtin_fmaf(k.bv, l1b, __builtin_fmaf(k.av, lx2, k.lnAuv)));
+        }
         const float Vd1 = k.rdu * V1, Vd2 = k.rdu * V2;
         const float U1 = V1 + S1, U2 = V2 + S2, D1 = Vd1 + S1, D2 = Vd2 + S2;
         Q1 = __builtin_fmaf(k.nSea, S1, V1 + Vd1); Q2s = __builtin_fmaf(k.nSea, S2, V2 + Vd2);
@@ -657,7 +706,8 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     }
 
     const float q2 = __builtin_fmaf(a, a, b * b);
-    const float common = (FAM == 0 ? k.pref : k.fam[FAM - 1].pref) * (alpS * alpS) * (LT * om0) * zMax * r;
+    const float pref = HW ? (FAM == 0 ? k.b2.pref : k.b2.famPref[F]) : (FAM == 0 ? k.pref : k.fam[F].pref);
+    const float common = pref * (alpS * alpS) * jacY * zMax * r;
     float tot = 0.0f;
     if (FAM == 0) {
         float w[6];
@@ -686,7 +736,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         // Sigma2gg2QQbar / Sigma2qqbar2QQbar::sigmaKin with tHQ = -sH a, uHQ = -sH b, mu = m^2 / sH:
         //   gg:  [b/a - 9/4 b^2 + mu/a^2 (a (9/2 b - 1/2 + mu) - 4 mu)  +  (a <-> b)] / 6
         //   qqb: 4/9 (a^2 + b^2 + 2 mu)
-        const float mu = k.fam[FAM - 1].mu * r;
+        const float mu = k.fam[F].mu * r;
         const float cT = __builtin_fmaf(a, __builtin_fmaf(4.5f, b, -0.5f) + mu, -4.0f * mu);
         const float cU = __builtin_fmaf(b, __builtin_fmaf(4.5f, a, -0.5f) + mu, -4.0f * mu);
         const float sTS = __builtin_fmaf(b, ia, __builtin_fmaf(-2.25f * b, b, (mu * (ia * ia)) * cT));
@@ -702,9 +752,9 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
             ws[lo + i] = wi;
             tot += wi;
         }
-        if (sHo) { *sHo = sH; *tHo = __builtin_fmaf(-sH, a, k.fam[FAM - 1].m2); *x1o = x1; *x2o = x2; }
+        if (sHo) { *sHo = sH; *tHo = __builtin_fmaf(-sH, a, k.fam[F].m2); *x1o = x1; *x2o = x2; }
     }
-    if (aux) {
+    if (!HW && aux) {
         aux->alpS = alpS; aux->cf = cfo; aux->lx1 = lx1; aux->lx2 = lx2; aux->l1a = l1a; aux->l1b = l1b;
         aux->g1 = g1; aux->g2 = g2; aux->Q1 = Q1; aux->Q2 = Q2s;
     }
