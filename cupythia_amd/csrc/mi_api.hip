@@ -123,6 +123,8 @@ void fill_consts(const mi_physics &p, HadConst<T> &k)
     k.lnAuv = (T)std::log(n.Auv); k.av = (T)p.pdf_val[0]; k.bv = (T)p.pdf_val[1];
     k.rdu = (T)(n.Adv / n.Auv);
     k.nSea = (T)(2 * p.nQuarkIn);
+    k.nSeaM2 = (T)(2 * p.nQuarkIn - 2);
+    k.nSeaM4 = (T)(2 * p.nQuarkIn - 4);
     k.nQuarkNew = (T)p.nQuarkNew;
     for (int f = 0; f < 2; ++f) {
         const double m2 = p.mass_Q[f] * p.mass_Q[f];

@@ -291,8 +291,9 @@ struct HadConst {
         T c6, c5, c4, c3;          // (12 pi / b0) / ln 2: alphaS = c / (log2 Q2 - lam)
         T famLT[2], famPref[2];    // the heavy families' LT / ln 2 and pref ln 2
     } b2;
+    T nSeaM2, nSeaM4;              // nSea - 2, nSea - 4 (exact), so no kernel recomputes them per point
 };
-static_assert(sizeof(HadConst<float>) == 59 * sizeof(float), "HadConst layout");
+static_assert(sizeof(HadConst<float>) == 61 * sizeof(float), "HadConst layout");
 
 // ------------------------------------------------------------------ PDF grid (LHAGrid1)
 // A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
@@ -592,7 +593,9 @@ struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 // map, alphaS and the toy PDFs in base 2, so that v_exp_f32 / v_log_f32 take and return the
 // values directly (log2 tau, log2 x, log2 (1 - x), log2 Q2 and the PDF exponents; HadConst::b2);
 // the grid PDFs still receive ln x, computed as the bit-exact path does.  aux: HW = false only.
-template <int FAM, bool HW = false>
+// U24 (HW only): u and 1 - u arrive as the integers k and 2^24 - k of the 24-bit uniforms, and
+// the 2^-24 goes into the products that take them (exact scalings).
+template <int FAM, bool HW = false, bool U24 = false>
 __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int mask,
                                                 float u0, float om0, float u1, float om1,
                                                 float u2, float om2, float* ws, float* sHo, float* tHo,
@@ -602,12 +605,15 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     constexpr int F = FAM > 0 ? FAM - 1 : 0;                 // heavy family index
     const float LT = FAM == 0 ? k.LT : k.fam[F].LT;
     float r, jacY, lx1, lx2, x1, x2, zMax;                 // jacY: 2 yMax in the prefactor's units
+    static_assert(HW || !U24, "integer uniforms are for the base-2 form");
+    constexpr float su = U24 ? 0x1p-24f : 1.0f;            // the uniforms' unit
     if constexpr (HW) {
-        const float LT2 = FAM == 0 ? k.b2.LT : k.b2.famLT[F];
+        const float LT2 = (FAM == 0 ? k.b2.LT : k.b2.famLT[F]) * su;
         const float targ = -(u0 * LT2);                    // log2(tauMin / tau)
         r = __builtin_amdgcn_exp2f(targ);                  // tauMin / tau
         jacY = LT2 * om0;                                  // 2 yMax / ln 2
-        lx1 = -(jacY * om1); lx2 = -(jacY * u1);           // log2 x1, log2 x2
+        const float jy = jacY * su;
+        lx1 = -(jy * om1); lx2 = -(jy * u1);               // log2 x1, log2 x2
         x1 = __builtin_amdgcn_exp2f(lx1); x2 = __builtin_amdgcn_exp2f(lx2);
         zMax = __builtin_amdgcn_sqrtf(one_minus_exp2f(targ, r));
     } else {
@@ -625,7 +631,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
     const float R1 = frcp<HW>(r * opz);
     const float omz = (r * r) * R1;                        // r / (1 + zMax) = 1 - zMax
     const float sH = ((FAM == 0 ? k.fourPT2 : k.fam[F].fourMT2) * opz) * R1;   // 4 mT^2 / r
-    const float twoz = zMax + zMax;
+    const float twoz = U24 ? zMax * (2.0f * su) : zMax + zMax;
     const float A = __builtin_fmaf(twoz, om2, omz);        // 1 - z
     const float B = __builtin_fmaf(twoz, u2, omz);         // 1 + z
     const float a = 0.5f * A, b = 0.5f * B;                // (m^2 - tH) / sH, (m^2 - uH) / sH
@@ -665,8 +671,8 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         const float Q2F = gp->factFac2 * pT2;
         float lnx1 = lx1, lnx2 = lx2, lnq;
         if constexpr (HW) {
-            const float twoYMax = LT * om0;                // ln x as the bit-exact path forms it
-            lnx1 = -(twoYMax * om1); lnx2 = -(twoYMax * u1);
+            const float twoYMax = (LT * su) * om0;         // ln x as the bit-exact path forms it
+            lnx1 = -((twoYMax * su) * om1); lnx2 = -((twoYMax * su) * u1);
             lnq = (float)(0.5 * kLn2) * __builtin_amdgcn_logf(Q2F);
         } else {
             lnq = 0.5f * own_logf(Q2F);
@@ -701,8 +707,8 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         Lgg = g1 * g2;
         Lqg = __builtin_fmaf(Q1, g2, g1 * Q2s);
         LQQ = Q1 * Q2s;
-        Lsame = __builtin_fmaf(U1, U2, __builtin_fmaf(D1, D2, (k.nSea - 2.0f) * SS));
-        Lopp = __builtin_fmaf(U1 + D1, S2, __builtin_fmaf(S1, U2 + D2, (k.nSea - 4.0f) * SS));
+        Lsame = __builtin_fmaf(U1, U2, __builtin_fmaf(D1, D2, k.nSeaM2 * SS));
+        Lopp = __builtin_fmaf(U1 + D1, S2, __builtin_fmaf(S1, U2 + D2, k.nSeaM4 * SS));
     }
 
     const float q2 = __builtin_fmaf(a, a, b * b);
@@ -765,7 +771,7 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
 // massless family only (codes 111..116) and writes ws[6..9] = 0; kinematics outputs are then
 // single values.  HEAVY = true: sHo .. x2o (if given) are arrays of 3, one entry per family,
 // written for the families that mask selects.
-template <bool HEAVY = false, bool HW = false>
+template <bool HEAVY = false, bool HW = false, bool U24 = false>
 __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mask,
                                                float u0, float om0, float u1, float om1,
                                                float u2, float om2, float* ws, float* sHo, float* tHo,
@@ -775,18 +781,18 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     if (!HEAVY) {
 #pragma unroll
         for (int i = kLightProc; i < kMaxProc; ++i) ws[i] = 0.0f;
-        return had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
+        return had_family_f32<0, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     }
 #pragma unroll
     for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0f;
     float tot = 0.0f;
     if (mask & kFamMask[0])
-        tot += had_family_f32<0, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
+        tot += had_family_f32<0, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     if (mask & kFamMask[1])
-        tot += had_family_f32<1, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
+        tot += had_family_f32<1, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
                                  x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp, grid);
     if (mask & kFamMask[2])
-        tot += had_family_f32<2, HW>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
+        tot += had_family_f32<2, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 2 : nullptr, tHo ? tHo + 2 : nullptr,
                                  x1o ? x1o + 2 : nullptr, x2o ? x2o + 2 : nullptr, nullptr, gp, grid);
     return tot;
 }
@@ -829,8 +835,8 @@ __host__ __device__ inline Lumi<double> toy_lumi_f64(const HadConst<double>& k, 
     L.gg = g1 * g2;
     L.qg = fma(Q1, g2, g1 * Q2s);
     L.QQ = Q1 * Q2s;
-    L.same = fma(U1, U2, fma(D1, D2, (k.nSea - 2.0) * SS));
-    L.opp = fma(U1 + D1, S2, fma(S1, U2 + D2, (k.nSea - 4.0) * SS));
+    L.same = fma(U1, U2, fma(D1, D2, k.nSeaM2 * SS));
+    L.opp = fma(U1 + D1, S2, fma(S1, U2 + D2, k.nSeaM4 * SS));
     return L;
 }
 

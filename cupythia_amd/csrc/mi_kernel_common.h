@@ -178,6 +178,19 @@ __device__ __forceinline__ MapOut<T> map_flat(uint32_t k)
     return m;
 }
 
+// The flat map's integers k and 2^24 - k (exact in FP32): the base-2 FP32_HW point takes these
+// and scales inside the products that use them.
+template <typename T>
+__device__ __forceinline__ MapOut<T> map_flat_u24(uint32_t k)
+{
+    MapOut<T> m;
+    m.x = (T)k;
+    m.omx = (T)16777216.0f - m.x;
+    m.jac = (T)1;
+    m.bin = 0;
+    return m;
+}
+
 template <typename T>
 __device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbins, T nbf)
 {
@@ -198,15 +211,16 @@ __device__ __forceinline__ MapOut<T> map_grid(uint32_t k, const T *tab, int nbin
 // grid: the PDFs of the grid *gp, else the toy PDFs (gp is then not read).  The choice is a flag
 // rather than a null test of gp, which keeps a kernel argument's view out of scratch memory.  HEAVY: see had_point_f32.  HW: the FP32 point
 // with the hardware transcendentals (MI_PREC_FP32_HW); the FP64 point ignores it.
-template <typename T, bool HEAVY, bool HW = false> struct HadPoint;
-template <bool HEAVY, bool HW> struct HadPoint<float, HEAVY, HW> {
+// U24: the uniforms are map_flat_u24's integers (FP32 HW only).
+template <typename T, bool HEAVY, bool HW = false, bool U24 = false> struct HadPoint;
+template <bool HEAVY, bool HW, bool U24> struct HadPoint<float, HEAVY, HW, U24> {
     static __device__ __forceinline__ float eval(const HadConst<float> &k, int mask, float u0, float o0,
                                                  float u1, float o1, float u2, float o2, float *ws,
                                                  float *sH, float *tH, float *x1, float *x2,
                                                  const GridPdf<float> *gp, bool grid)
-    { return had_point_f32<HEAVY, HW>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp, grid); }
+    { return had_point_f32<HEAVY, HW, U24>(k, mask, u0, o0, u1, o1, u2, o2, ws, sH, tH, x1, x2, nullptr, gp, grid); }
 };
-template <bool HEAVY, bool HW> struct HadPoint<double, HEAVY, HW> {
+template <bool HEAVY, bool HW, bool U24> struct HadPoint<double, HEAVY, HW, U24> {
     static __device__ __forceinline__ double eval(const HadConst<double> &k, int mask, double u0, double o0,
                                                   double u1, double o1, double u2, double o2, double *ws,
                                                   double *sH, double *tH, double *x1, double *x2,
@@ -225,14 +239,17 @@ __device__ __forceinline__ T had_candidate(const HadConst<T> &k, int mask, uint6
 {
     const U4 r = philox_event(key, e, 0u);
     MapOut<T> m0, m1, m2;
+    constexpr bool U24 = HW && !VEGAS && sizeof(T) == 4;
     if (VEGAS) {
         m0 = map_grid<T>(k24(r.x), stab, nbins, nbf);
         m1 = map_grid<T>(k24(r.y), stab + 3 * nbins, nbins, nbf);
         m2 = map_grid<T>(k24(r.z), stab + 6 * nbins, nbins, nbf);
+    } else if (U24) {
+        m0 = map_flat_u24<T>(k24(r.x)); m1 = map_flat_u24<T>(k24(r.y)); m2 = map_flat_u24<T>(k24(r.z));
     } else {
         m0 = map_flat<T>(k24(r.x)); m1 = map_flat<T>(k24(r.y)); m2 = map_flat<T>(k24(r.z));
     }
-    T tot = HadPoint<T, HEAVY, HW>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp, grid);
+    T tot = HadPoint<T, HEAVY, HW, U24>::eval(k, mask, m0.x, m0.omx, m1.x, m1.omx, m2.x, m2.omx, ws, sH, tH, x1, x2, gp, grid);
     if (VEGAS) {
         T jac = (T)1;
         jac = jac * m0.jac;

@@ -170,19 +170,16 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
 
     if (HIST) WaveHist<3>::init(harea + wave * areaBytes, nbins);
     const uint64_t base = (uint64_t)blockIdx.x * NT * ppt;
-    for (uint32_t j = 0; j < ppt; ++j) {
-        const uint64_t i = base + (uint64_t)j * NT + threadIdx.x;
-        const bool live = i < n;
-        // the histogram's batches need the whole wave: it leaves the loop together
-        if (HIST ? __ballot(live) == 0 : !live) break;
-        // a lane past the end (HIST only) evaluates the last point again and adds nothing
+    // point i of the thread, live or (HIST only) a dead lane that evaluates the last point again
+    // and adds nothing
+    auto step = [&](uint64_t i, bool live) {
         T ws[kMaxProc];
         int bins[3];
         const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + (live ? i : n - 1), stab, nbins, nbf, ws,
                                                          bins, (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
                                                          &gpdf, GRID);
         if (HIST) WaveHist<3>::add(live ? (double)tot * (double)tot : 0.0, bins, harea + wave * areaBytes, nbins);
-        if (!live) continue;
+        if (!live) return;
         const double dt = (double)tot;
         sw[0] += dt;
         sw2[0] = fma(dt, dt, sw2[0]);
@@ -195,6 +192,18 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
             sw[1 + p] += d;
             sw2[1 + p] = fma(d, d, sw2[1 + p]);
             mx[1 + p] = run_max(mx[1 + p], ws[p]);
+        }
+    };
+    if (!HIST && base + (uint64_t)NT * ppt <= n) {
+        // every point of the block is live (all blocks but the last): a uniform loop, no tests
+        for (uint32_t j = 0; j < ppt; ++j) step(base + (uint64_t)j * NT + threadIdx.x, true);
+    } else {
+        for (uint32_t j = 0; j < ppt; ++j) {
+            const uint64_t i = base + (uint64_t)j * NT + threadIdx.x;
+            const bool live = i < n;
+            // the histogram's batches need the whole wave: it leaves the loop together
+            if (HIST ? __ballot(live) == 0 : !live) break;
+            step(i, live);
         }
     }
     double mxd[NP + 1];
