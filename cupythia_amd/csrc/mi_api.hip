@@ -114,6 +114,9 @@ void fill_consts(const mi_physics &p, HadConst<T> &k)
         k.scale2Min = (T)a.scale2Min;
         k.lnL6s = (T)std::log(a.L6s); k.lnL5s = (T)std::log(a.L5s);
         k.lnL4s = (T)std::log(a.L4s); k.lnL3s = (T)std::log(a.L3s);
+        // Q2 = max(renorm_fac^2 pT2, scale2Min) with pT2 >= pTmin^2 by the cut (heavy codes: pT2 +
+        // m^2); the 1% margin covers the rounding of pT2 in FP32
+        k.above_mb = (p.renorm_fac * p.renorm_fac * p.pTHatMin * p.pTHatMin > 1.01 * 4.8 * 4.8) ? (T)1 : (T)0;
     }
     k.c6 = (T)(12.0 * kPi / 21.0); k.c5 = (T)(12.0 * kPi / 23.0);
     k.c4 = (T)(12.0 * kPi / 25.0); k.c3 = (T)(12.0 * kPi / 27.0);

@@ -292,8 +292,9 @@ struct HadConst {
         T famLT[2], famPref[2];    // the heavy families' LT / ln 2 and pref ln 2
     } b2;
     T nSeaM2, nSeaM4;              // nSea - 2, nSea - 4 (exact), so no kernel recomputes them per point
+    T above_mb;                    // 1: renorm_fac^2 pTmin^2 > 1.01 mb^2, so every point's Q2 > mb^2
 };
-static_assert(sizeof(HadConst<float>) == 61 * sizeof(float), "HadConst layout");
+static_assert(sizeof(HadConst<float>) == 62 * sizeof(float), "HadConst layout");
 
 // ------------------------------------------------------------------ PDF grid (LHAGrid1)
 // A proton PDF read from an LHAPDF6 "lhagrid1" file (mi_pdf_load), evaluated as Pythia's
@@ -647,15 +648,22 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         Q2 = (Q2 < k.scale2Min) ? k.scale2Min : Q2;
         const bool t6 = Q2 > k.mt2 && k.nf6 != 0.0f, t5 = Q2 > k.mb2, t4 = Q2 > k.mc2;
         float lq, lam, cf;
-        if constexpr (HW) {                                // base 2: Q2 >= scale2Min > 0
-            lq = __builtin_amdgcn_logf(Q2);
-            lam = t6 ? k.b2.lam6 : (t5 ? k.b2.lam5 : (t4 ? k.b2.lam4 : k.b2.lam3));
-            cf = t6 ? k.b2.c6 : (t5 ? k.b2.c5 : (t4 ? k.b2.c4 : k.b2.c3));
+        // HW: base 2 (Q2 >= scale2Min > 0).  above_mb (uniform): Q2 > mb^2 at every point of the
+        // run, so the flavour number is 5 or 6 and the chain shortens to one test
+        const float L6 = HW ? k.b2.lam6 : k.lnL6s, L5 = HW ? k.b2.lam5 : k.lnL5s;
+        const float C6 = HW ? k.b2.c6 : k.c6, C5 = HW ? k.b2.c5 : k.c5;
+        if (k.above_mb != 0.0f) {
+            lam = t6 ? L6 : L5;
+            cf = t6 ? C6 : C5;
+        } else if constexpr (HW) {
+            lam = t6 ? L6 : (t5 ? L5 : (t4 ? k.b2.lam4 : k.b2.lam3));
+            cf = t6 ? C6 : (t5 ? C5 : (t4 ? k.b2.c4 : k.b2.c3));
         } else {
-            lq = own_logf(Q2);
-            lam = t6 ? k.lnL6s : (t5 ? k.lnL5s : (t4 ? k.lnL4s : k.lnL3s));
-            cf = t6 ? k.c6 : (t5 ? k.c5 : (t4 ? k.c4 : k.c3));
+            lam = t6 ? L6 : (t5 ? L5 : (t4 ? k.lnL4s : k.lnL3s));
+            cf = t6 ? C6 : (t5 ? C5 : (t4 ? k.c4 : k.c3));
         }
+        if constexpr (HW) lq = __builtin_amdgcn_logf(Q2);
+        else lq = own_logf(Q2);
         // one division gives both 1 / (a b) and alphaS = cf / (lq - lam)
         const float den = lq - lam;
         const float R2 = frcp<HW>(ab * den);
@@ -730,12 +738,16 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         }
         w[4] = Lopp * (q2 * __builtin_fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
         w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
+        // a slot mask leaves out is 0 and adds nothing (a uniform branch, no select)
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-            float wi = (mask & (1 << i)) ? common * w[i] : 0.0f;
-            wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;          // negative -> 0, NaN kept
-            ws[i] = wi;
-            tot += wi;
+            ws[i] = 0.0f;
+            if (mask & (1 << i)) {
+                float wi = common * w[i];
+                wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;      // negative -> 0, NaN kept
+                ws[i] = wi;
+                tot += wi;
+            }
         }
         if (sHo) { *sHo = sH; *tHo = -(sH * a); *x1o = x1; *x2o = x2; }
     } else {
@@ -753,10 +765,13 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         constexpr int lo = 4 + 2 * FAM;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            float wi = (mask & (1 << (lo + i))) ? common * w[i] : 0.0f;
-            wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;          // negative -> 0, NaN kept
-            ws[lo + i] = wi;
-            tot += wi;
+            ws[lo + i] = 0.0f;
+            if (mask & (1 << (lo + i))) {
+                float wi = common * w[i];
+                wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;      // negative -> 0, NaN kept
+                ws[lo + i] = wi;
+                tot += wi;
+            }
         }
         if (sHo) { *sHo = sH; *tHo = __builtin_fmaf(-sH, a, k.fam[F].m2); *x1o = x1; *x2o = x2; }
     }
