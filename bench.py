@@ -268,7 +268,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--points", type=int, default=1 << 28)
     ap.add_argument("--precision", choices=["fp32hw", "fp32"], default="fp32hw",
                     help="headline precision: fp32hw = FP32 with the hardware exp2/log2/rcp/sqrt (per point within "
