@@ -595,8 +595,9 @@ struct PointAux { T alpS, cf, lx1, lx2, l1a, l1b, g1, g2, Q1, Q2; };
 // values directly (log2 tau, log2 x, log2 (1 - x), log2 Q2 and the PDF exponents; HadConst::b2);
 // the grid PDFs still receive ln x, computed as the bit-exact path does.  aux: HW = false only.
 // U24 (HW only): u and 1 - u arrive as the integers k and 2^24 - k of the 24-bit uniforms, and
-// the 2^-24 goes into the products that take them (exact scalings).
-template <int FAM, bool HW = false, bool U24 = false>
+// the 2^-24 goes into the products that take them (exact scalings).  BR: keep the massless
+// slots' mask tests as branches (off where the three families share a point: registers).
+template <int FAM, bool HW = false, bool U24 = false, bool BR = true>
 __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int mask,
                                                 float u0, float om0, float u1, float om1,
                                                 float u2, float om2, float* ws, float* sHo, float* tHo,
@@ -738,11 +739,13 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         }
         w[4] = Lopp * (q2 * __builtin_fmaf(inv_ab, 0.59259259259259259f, -1.3333333333333333f));
         w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
-        // a slot mask leaves out is 0 and adds nothing (a uniform branch, no select)
+        // a slot mask leaves out is 0 and adds nothing (a uniform branch, no select: the empty
+        // volatile asm keeps the compiler from if-converting it)
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             ws[i] = 0.0f;
             if (mask & (1 << i)) {
+                if (BR) __asm__ volatile("");
                 float wi = common * w[i];
                 wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;      // negative -> 0, NaN kept
                 ws[i] = wi;
@@ -802,7 +805,7 @@ __device__ __forceinline__ float had_point_f32(const HadConst<float>& k, int mas
     for (int i = 0; i < kMaxProc; ++i) ws[i] = 0.0f;
     float tot = 0.0f;
     if (mask & kFamMask[0])
-        tot += had_family_f32<0, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
+        tot += had_family_f32<0, HW, U24, false>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo, tHo, x1o, x2o, aux, gp, grid);
     if (mask & kFamMask[1])
         tot += had_family_f32<1, HW, U24>(k, mask, u0, om0, u1, om1, u2, om2, ws, sHo ? sHo + 1 : nullptr, tHo ? tHo + 1 : nullptr,
                                  x1o ? x1o + 1 : nullptr, x2o ? x2o + 1 : nullptr, nullptr, gp, grid);

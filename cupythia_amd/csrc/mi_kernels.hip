@@ -127,6 +127,39 @@ hipError_t launch_sigma_hat(hipStream_t s, int code, uint64_t n, const double *s
 // columns wx0..nx-1) into its dynamic LDS at gpdf.woff (after the VEGAS table and histograms),
 // so the interpolation stencils read LDS instead of gathering from L2.  HW (FP32): the hardware
 // transcendentals (MI_PREC_FP32_HW).  Only the slots of the processes mask selects are summed.
+// One point of k_integrate_had: the candidate at counter e, its histogram entry (HIST) and, if
+// live, its sums.  A function of its own rather than a lambda: a closure that refers to the
+// kernel's GridPdf argument made the compiler copy that argument to scratch.
+template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, bool HW, int NP, typename MaxT>
+__device__ __forceinline__ void integrate_point(const HadConst<T> &k, int mask, uint64_t key, uint64_t e, bool live,
+                                                const T *stab, int nbins, T nbf, const GridPdf<T> *gp,
+                                                unsigned char *hwave, double *sw, double *sw2, MaxT *mx, uint32_t &nnz)
+{
+    T ws[kMaxProc];
+    int bins[3];
+    const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, e, stab, nbins, nbf, ws, bins, (T *)nullptr,
+                                                     (T *)nullptr, (T *)nullptr, (T *)nullptr, gp, GRID);
+    if (HIST) WaveHist<3>::add(live ? (double)tot * (double)tot : 0.0, bins, hwave, nbins);
+    if (!live) return;
+    const double dt = (double)tot;
+    sw[0] += dt;
+    sw2[0] = fma(dt, dt, sw2[0]);
+    mx[0] = run_max(mx[0], tot);
+    nnz += (tot > (T)0) ? 1u : 0u;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        if (!((mask >> p) & 1)) continue;                       // uniform: the slot stays 0
+        // an empty volatile asm keeps this a branch: if-converted, each of the slot's updates
+        // became a select per register (5 per slot, for every slot).  Not for the ten slots of
+        // HEAVY, where the branches cost registers (115 -> 146 VGPRs)
+        if (!HEAVY) __asm__ volatile("");
+        const double d = (double)ws[p];
+        sw[1 + p] += d;
+        sw2[1 + p] = fma(d, d, sw2[1 + p]);
+        mx[1 + p] = run_max(mx[1 + p], ws[p]);
+    }
+}
+
 template <typename T, bool VEGAS, bool HIST, bool GRID, bool HEAVY, int NT = kBlock, bool WIN = false, bool HW = false>
 __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, uint64_t key,
                                                       uint64_t ctr0, uint64_t n, uint32_t ppt,
@@ -170,40 +203,21 @@ __global__ __launch_bounds__(NT) void k_integrate_had(HadConst<T> k, int mask, u
 
     if (HIST) WaveHist<3>::init(harea + wave * areaBytes, nbins);
     const uint64_t base = (uint64_t)blockIdx.x * NT * ppt;
-    // point i of the thread, live or (HIST only) a dead lane that evaluates the last point again
-    // and adds nothing
-    auto step = [&](uint64_t i, bool live) {
-        T ws[kMaxProc];
-        int bins[3];
-        const T tot = had_candidate<T, VEGAS, HEAVY, HW>(k, mask, key, ctr0 + (live ? i : n - 1), stab, nbins, nbf, ws,
-                                                         bins, (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr,
-                                                         &gpdf, GRID);
-        if (HIST) WaveHist<3>::add(live ? (double)tot * (double)tot : 0.0, bins, harea + wave * areaBytes, nbins);
-        if (!live) return;
-        const double dt = (double)tot;
-        sw[0] += dt;
-        sw2[0] = fma(dt, dt, sw2[0]);
-        mx[0] = run_max(mx[0], tot);
-        nnz += (tot > (T)0) ? 1u : 0u;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            if (!((mask >> p) & 1)) continue;                   // uniform: the slot stays 0
-            const double d = (double)ws[p];
-            sw[1 + p] += d;
-            sw2[1 + p] = fma(d, d, sw2[1 + p]);
-            mx[1 + p] = run_max(mx[1 + p], ws[p]);
-        }
-    };
     if (!HIST && base + (uint64_t)NT * ppt <= n) {
         // every point of the block is live (all blocks but the last): a uniform loop, no tests
-        for (uint32_t j = 0; j < ppt; ++j) step(base + (uint64_t)j * NT + threadIdx.x, true);
+        for (uint32_t j = 0; j < ppt; ++j)
+            integrate_point<T, VEGAS, HIST, GRID, HEAVY, HW, NP>(k, mask, key, ctr0 + base + (uint64_t)j * NT + threadIdx.x,
+                                                                 true, stab, nbins, nbf, &gpdf, harea + wave * areaBytes,
+                                                                 sw, sw2, mx, nnz);
     } else {
         for (uint32_t j = 0; j < ppt; ++j) {
             const uint64_t i = base + (uint64_t)j * NT + threadIdx.x;
             const bool live = i < n;
             // the histogram's batches need the whole wave: it leaves the loop together
             if (HIST ? __ballot(live) == 0 : !live) break;
-            step(i, live);
+            // a lane past the end (HIST only) evaluates the last point again and adds nothing
+            integrate_point<T, VEGAS, HIST, GRID, HEAVY, HW, NP>(k, mask, key, ctr0 + (live ? i : n - 1), live, stab, nbins,
+                                                                 nbf, &gpdf, harea + wave * areaBytes, sw, sw2, mx, nnz);
         }
     }
     double mxd[NP + 1];
