@@ -22,7 +22,7 @@ constexpr int FPL = MI_FLOW_NPARAMS / MI_FLOW_LAYERS;   // 7537 parameters per l
 constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
 static_assert(PB3 + FO == FPL, "flow parameter layout");
 constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
-constexpr int FWV = 8;      // waves per block of the forward / inverse kernel (one block per CU)
+constexpr int FWV = 12;     // waves per block of the forward / inverse kernel (one block per CU)
 constexpr int FWB = 4;      // waves per block of the backward kernel (one block per CU, one wave per SIMD)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
