@@ -198,7 +198,11 @@ __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restr
         float lsum = 0.0f;
 #pragma unroll 1
         for (int s = 0; s < FL; ++s) {
-            const int l = INVERSE ? FL - 1 - s : s;
+            int l = INVERSE ? FL - 1 - s : s;
+            // l opaque (an SGPR the compiler cannot rewrite in s): otherwise the inverse direction's
+            // W[FL - 1 - s] becomes a 95 KB constant offset past the ds_read offset field, and every
+            // weight fragment read costs two address additions
+            __asm__ volatile("" : "+s"(l));
             const int d = l % 3;
             float o[FO];
             chain_mlp(W[l], pick3(x, cond0(d)), pick3(x, cond1(d)), o);
