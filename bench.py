@@ -91,16 +91,17 @@ SOLVE_DECISIONS = {0: "plain VEGAS", 1: "fallback to VEGAS before training",
                    2: "fallback to VEGAS after training", 3: "flow", 4: "flow (forced)"}
 
 
-def time_to_solution(ctx, mi, sampler, target=1e-3, **opts):
+def time_to_solution(ctx, mi, sampler, target=1e-3, precision=None, **opts):
     """GPU time (device time of every pass) until sigma_err/sigma <= target: the VEGAS adaptation
     (3-D, 64 bins, 8 x 2^20 points, alpha 1.5), then mi_solve -- a 2^20-point VEGAS pilot, for the
     flow arms the identity-flow probe, training and the flow's pilot, and the final pass sized for
-    the target."""
+    the target.  precision: of every pass (default the bit-reproducible FP32)."""
     phys = mi.physics_default(1)
+    prec = mi.MI_PREC_FP32 if precision is None else precision
     t0 = time.perf_counter()
     ctx.vegas_init(3, 64)
-    info = ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42), 8, 1.5)
-    est, si = ctx.solve(mi.make_run(phys, 1, key=44), target, sampler, **opts)
+    info = ctx.vegas_adapt(mi.make_run(phys, 1 << 20, key=42, precision=prec), 8, 1.5)
+    est, si = ctx.solve(mi.make_run(phys, 1, key=44, precision=prec), target, sampler, **opts)
     wall = time.perf_counter() - t0
     d = si.as_dict()
     d["solve_gpu_ms"] = d.pop("gpu_ms")
@@ -119,9 +120,23 @@ def tts_all(ctx, mi):
     tf = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, flow_seed=7, force_flow=1, max_steps=200)
     sv = time_to_solution(ctx, mi, mi.MI_SAMPLER_VEGAS, target=1e-5)
     sa = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, target=1e-5, flow_seed=7)
-    return {"target_rel_err": 1e-3, "vegas": tv, "flow_vegas_fallback": ta, "flow_vegas_forced": tf,
+    # the same two targets with every pass in FP32 with the hardware transcendentals (the
+    # headline integration mode): both arms cheaper per point, VEGAS's by ~1.5x, the flow's by a
+    # few % (its cost is the MLP's)
+    hw = mi.MI_PREC_FP32_HW
+    hv = time_to_solution(ctx, mi, mi.MI_SAMPLER_VEGAS, precision=hw)
+    ha = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, precision=hw, flow_seed=7)
+    hsv = time_to_solution(ctx, mi, mi.MI_SAMPLER_VEGAS, target=1e-5, precision=hw)
+    hsa = time_to_solution(ctx, mi, mi.MI_SAMPLER_FLOW, target=1e-5, precision=hw, flow_seed=7)
+    return {"target_rel_err": 1e-3, "precision": "fp32", "vegas": tv, "flow_vegas_fallback": ta,
+            "flow_vegas_forced": tf,
             "speedup_fallback": tv["gpu_ms"] / ta["gpu_ms"], "speedup_forced": tv["gpu_ms"] / tf["gpu_ms"],
-            "tight": {"target_rel_err": 1e-5, "vegas": sv, "flow_vegas": sa, "speedup": sv["gpu_ms"] / sa["gpu_ms"]}}
+            "tight": {"target_rel_err": 1e-5, "vegas": sv, "flow_vegas": sa, "speedup": sv["gpu_ms"] / sa["gpu_ms"]},
+            "fp32hw": {"precision": "fp32hw",
+                       "target_rel_err": 1e-3, "vegas_gpu_ms": hv["gpu_ms"], "flow_vegas_gpu_ms": ha["gpu_ms"],
+                       "flow_vegas_decision": ha["decision_name"], "speedup": hv["gpu_ms"] / ha["gpu_ms"],
+                       "tight": {"target_rel_err": 1e-5, "vegas": hsv, "flow_vegas": hsa,
+                                 "speedup": hsv["gpu_ms"] / hsa["gpu_ms"]}}}
 
 
 # Flow: per layer the MLP 2 -> 64 -> 64 -> 49 is 2*64 + 64*64 + 64*49 = 7360 multiply-adds per
