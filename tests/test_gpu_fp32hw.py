@@ -123,3 +123,22 @@ def test_unweighting_of_an_fp32_hw_run_is_bit_exact(ctx, O):
     cnt, flags, pid, _ = O.unweight32(O.hadcfg_default(), 9, 0, n, 2.0)
     assert ev.n == cnt
     assert np.array_equal(ev.index.cpu().numpy(), np.nonzero(flags)[0])
+
+
+@pytest.mark.parametrize("pt,xi", [(1.0, 1.0), (3.0, 0.5), (1.2, 2.0)])
+def test_points_across_the_flavour_thresholds(ctx, O, pt, xi):
+    """Runs whose scales cross m_c^2 and m_b^2 (pTmin 1-3 GeV, renorm_fac 0.5-2): alphaS takes
+    its full flavour chain (nf 3, 4, 5), not the one-test form of runs above m_b^2.  The
+    bit-exact FP32 path equals the host mirror, FP32_HW stays within 1e-5 of the long-double
+    oracle, for the ten-process library."""
+    phys = mi.physics_default(1, pTHatMin=pt, renorm_fac=xi, proc_mask=0x3FF)
+    cfg = O.hadcfg_default(pTHatMin=pt, renormFac=xi, procMask=0x3FF)
+    n = 1 << 17
+    w32, _ = ctx.points(mi.make_run(phys, n, precision=mi.MI_PREC_FP32, key=11, counter0=3), with_kin=False)
+    o32, _ = O.had_points32(cfg, 11, 3, n)
+    assert np.array_equal(w32.cpu().numpy().astype(np.float32), o32)
+    wh, _ = ctx.points(mi.make_run(phys, n, precision=HW, key=11, counter0=3), with_kin=False)
+    wld, _ = O.had_points_ld(cfg, 11, 3, n)
+    mx, med = _rel(wh.cpu().numpy(), wld)
+    print(f"FP32_HW pT {pt} xi {xi}: max rel {mx:.2e}, median {med:.2e}")
+    assert mx < 1e-5 and med < 1e-6
