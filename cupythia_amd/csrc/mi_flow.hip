@@ -319,8 +319,12 @@ __device__ void pack_param(int l, int j, float v, FwdLayer *__restrict__ fwd, Bw
 // Weight of a flow point: y -> VEGAS map -> FP32 integrand, times J_V and |dy/dz| = exp(ld).
 // Writes w (optional) and one kRec record per block.  HEAVY: mask may select codes 121..124;
 // HW: the integrand with the hardware transcendentals (MI_PREC_FP32_HW); STATS: only the record's
-// sum w and sum w^2 (the training step's statistics; the other slots are not written).
-template <bool HEAVY, bool HW, bool STATS>
+// sum w and sum w^2 (the training step's statistics; the other slots are not written).  The sums
+// per process slot run as in the integration kernel: FP64 sums, the FP32 maximum as the integer
+// order of its bits, slots the mask leaves out skipped by a uniform branch; the next point's y
+// and ld are loaded before this point's integrand, so their latency hides behind it.  GRID: the
+// grid PDFs (a template argument, so that the toy-PDF kernels do not carry the grid's registers).
+template <bool HEAVY, bool HW, bool STATS, bool GRID>
 __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int mask, const float *__restrict__ gtab,
                                                         int nbins, uint64_t n, const float *__restrict__ y,
                                                         const float *__restrict__ ld, float *__restrict__ wout,
@@ -330,15 +334,25 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
     float *stab = reinterpret_cast<float *>(smem);
     for (int i = threadIdx.x; i < 9 * nbins; i += blockDim.x) stab[i] = gtab[i];
     __syncthreads();
-    double v[kRec];
+    constexpr int NP = STATS ? 0 : (HEAVY ? kMaxProc : kLightProc);
+    uint32_t mx[NP + 1];
+    double sw[NP + 1], sw2[NP + 1];
 #pragma unroll
-    for (int q = 0; q < kRec; ++q) v[q] = 0.0;
+    for (int p = 0; p <= NP; ++p) { mx[p] = 0u; sw[p] = 0.0; sw2[p] = 0.0; }
+    uint32_t nnz = 0;
     const float nbf = (float)nbins;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float ny[3] = {0.0f, 0.0f, 0.0f}, nld = 0.0f;
+    if (i < n) { ny[0] = y[3 * i]; ny[1] = y[3 * i + 1]; ny[2] = y[3 * i + 2]; nld = ld[i]; }
+    for (; i < n; i += stride) {
+        const float yv[3] = {ny[0], ny[1], ny[2]}, lj = nld;
+        const uint64_t inext = i + stride;
+        if (inext < n) { ny[0] = y[3 * inext]; ny[1] = y[3 * inext + 1]; ny[2] = y[3 * inext + 2]; nld = ld[inext]; }
         float u[3], om[3], jac = 1.0f;
 #pragma unroll
         for (int dd = 0; dd < 3; ++dd) {
-            const float yy = fminf(fmaxf(y[3 * i + dd], 0.0f), 1.0f);
+            const float yy = fminf(fmaxf(yv[dd], 0.0f), 1.0f);
             const float t = yy * nbf;
             const int b = min((int)t, nbins - 1);
             const float frac = t - (float)b;
@@ -349,27 +363,31 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
         }
         float ws[kMaxProc];
         const float f = had_point_f32<HEAVY, HW>(k, mask, u[0], om[0], u[1], om[1], u[2], om[2], ws, nullptr, nullptr,
-                                                 nullptr, nullptr, nullptr, &gpdf, gpdf.active);
-        const float wt = f * jac * expf(ld[i]);
+                                                 nullptr, nullptr, nullptr, &gpdf, GRID);
+        const float ej = expf(lj);
+        const float wt = f * jac * ej;
         if (wout) wout[i] = wt;
         const double dw = (double)wt;
-        v[kRS] += dw;
-        v[kRQ] = fma(dw, dw, v[kRQ]);
+        sw[0] += dw;
+        sw2[0] = fma(dw, dw, sw2[0]);
         if (STATS) continue;
-        v[kRM] = fmax(v[kRM], dw);
-        v[kRNz] += (wt > 0.0f) ? 1.0 : 0.0;
-        const float scale = jac * expf(ld[i]);
+        mx[0] = run_max(mx[0], wt);
+        nnz += (wt > 0.0f) ? 1u : 0u;
+        const float scale = jac * ej;
 #pragma unroll
-        for (int p = 0; p < (HEAVY ? kMaxProc : kLightProc); ++p) {
-            const double dp = (double)(ws[p] * scale);
-            v[kRS + 1 + p] += dp;
-            v[kRQ + 1 + p] = fma(dp, dp, v[kRQ + 1 + p]);
-            v[kRM + 1 + p] = fmax(v[kRM + 1 + p], dp);
+        for (int p = 0; p < NP; ++p) {
+            if (!((mask >> p) & 1)) continue;                   // uniform: the slot stays 0
+            if (!HEAVY) __asm__ volatile("");                   // kept a branch, not selects
+            const float wp = ws[p] * scale;
+            const double dp = (double)wp;
+            sw[1 + p] += dp;
+            sw2[1 + p] = fma(dp, dp, sw2[1 + p]);
+            mx[1 + p] = run_max(mx[1 + p], wp);
         }
     }
     if (STATS) {
         __shared__ double red[kBlock / 64][2];
-        const double s = wave_sum(v[kRS]), q = wave_sum(v[kRQ]);
+        const double s = wave_sum(sw[0]), q = wave_sum(sw2[0]);
         if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6][0] = s; red[threadIdx.x >> 6][1] = q; }
         __syncthreads();
         if (threadIdx.x < 2) {
@@ -378,7 +396,10 @@ __global__ __launch_bounds__(kBlock) void k_flow_weight(HadConst<float> k, int m
             partials[(size_t)blockIdx.x * kRec + (threadIdx.x == 0 ? kRS : kRQ)] = t;
         }
     } else {
-        block_record(v, partials + (size_t)blockIdx.x * kRec);
+        double mxd[NP + 1];
+#pragma unroll
+        for (int p = 0; p <= NP; ++p) mxd[p] = max_value<float>(mx[p]);
+        block_record_sums<kBlock, NP>(sw, sw2, mxd, (double)nnz, partials + (size_t)blockIdx.x * kRec);
     }
 }
 
@@ -720,6 +741,26 @@ int mi_flow_backward(mi_ctx *ctx, uint64_t n, const float *y_dev, const float *g
     return MI_OK;
 }
 
+// Blocks per CU of the weight kernel's grid (the toy-PDF kernels hold 60-96 VGPRs: 5-8 waves per SIMD).
+constexpr uint32_t kWeightBlocksPerCU = 6;
+
+// The weight kernel of a run's options.
+using FlowWeightKernel = void (*)(HadConst<float>, int, const float *, int, uint64_t, const float *, const float *,
+                                  float *, double *, GridPdf<float>);
+static FlowWeightKernel pick_flow_weight(bool grid, bool heavy, bool hw, bool stats)
+{
+    if (grid) {
+        if (heavy) return hw ? (stats ? k_flow_weight<true, true, true, true> : k_flow_weight<true, true, false, true>)
+                             : (stats ? k_flow_weight<true, false, true, true> : k_flow_weight<true, false, false, true>);
+        return hw ? (stats ? k_flow_weight<false, true, true, true> : k_flow_weight<false, true, false, true>)
+                  : (stats ? k_flow_weight<false, false, true, true> : k_flow_weight<false, false, false, true>);
+    }
+    if (heavy) return hw ? (stats ? k_flow_weight<true, true, true, false> : k_flow_weight<true, true, false, false>)
+                         : (stats ? k_flow_weight<true, false, true, false> : k_flow_weight<true, false, false, false>);
+    return hw ? (stats ? k_flow_weight<false, true, true, false> : k_flow_weight<false, true, false, false>)
+              : (stats ? k_flow_weight<false, false, true, false> : k_flow_weight<false, false, false, false>);
+}
+
 // Sample n points of the current flow (+ VEGAS) for events ctr0 ..: y in F.xs[0], ld, w, and
 // the estimate record of the batch in out_rec (device), or with out_rec null only the per-block
 // sums of w and w^2 in ctx->partials (*nblocks of them, record layout).
@@ -735,17 +776,12 @@ static int flow_sample_weights(mi_ctx *ctx, const mi_run *run, uint64_t ctr0, ui
     GridPdf<float> gpdf;
     int rc = pdf_views(ctx, run->phys, &gpdf, nullptr);
     if (rc) return rc;
-    const int occ = 2;
     const uint32_t nblk = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock,
-                                                      (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * occ);
+                                                      (uint64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * kWeightBlocksPerCU);
     if ((size_t)nblk * kRec > ctx->partials_cap) return ferr(ctx, MI_ERR_STATE, "partials buffer too small");
     const size_t sm = sizeof(float) * 9 * ctx->grid.nbins;
     const bool heavy = has_heavy(run->phys.proc_mask), hw = run->precision == MI_PREC_FP32_HW, stats = !out_rec;
-    auto kern = k_flow_weight<false, false, false>;
-    if (heavy) kern = hw ? (stats ? k_flow_weight<true, true, true> : k_flow_weight<true, true, false>)
-                         : (stats ? k_flow_weight<true, false, true> : k_flow_weight<true, false, false>);
-    else kern = hw ? (stats ? k_flow_weight<false, true, true> : k_flow_weight<false, true, false>)
-                   : (stats ? k_flow_weight<false, false, true> : k_flow_weight<false, false, false>);
+    const FlowWeightKernel kern = pick_flow_weight(gpdf.active != 0, heavy, hw, stats);
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), sm, ctx->stream, k, run->phys.proc_mask, ctx->grid.tab32,
                        ctx->grid.nbins, n, F.xs, F.ld, F.w, ctx->partials, gpdf);
     FCHK(hipGetLastError());
@@ -774,7 +810,7 @@ int mi_flow_train(mi_ctx *ctx, const mi_run *run, int steps, uint64_t batch, dou
         FCHK(hipMalloc(&F.hist, sizeof(double) * 2 * steps));
         F.hist_cap = steps;
     }
-    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 2;
+    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * kWeightBlocksPerCU;
     if ((size_t)need * kRec > ctx->partials_cap) {
         (void)hipFree(ctx->partials);
         ctx->partials = nullptr;
@@ -837,7 +873,7 @@ int mi_flow_integrate_internal(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out
     rc = flow_alloc(ctx, chunk);
     if (rc) return rc;
     FlowState &F = *ctx->flow;
-    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 2;
+    const uint32_t need = (uint32_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * kWeightBlocksPerCU;
     if ((size_t)need * kRec > ctx->partials_cap) {
         (void)hipFree(ctx->partials);
         ctx->partials = nullptr;
