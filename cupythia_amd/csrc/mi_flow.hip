@@ -869,7 +869,9 @@ int mi_flow_integrate_internal(mi_ctx *ctx, const mi_run *run, mi_sigma_est *out
     if (run->precision != MI_PREC_FP32 && run->precision != MI_PREC_FP32_HW)
         return ferr(ctx, MI_ERR_ARG, "the flow sampler evaluates FP32 weights");
     if (ctx->grid.ndim != 3) return ferr(ctx, MI_ERR_STATE, "the flow composes with a 3-D VEGAS grid");
-    const uint64_t chunk = std::min<uint64_t>(run->n_points, 1ULL << 22);
+    // chunks of 32 tiles per wave of the forward kernel's full grid (6.3M points on 256 CUs): every
+    // wave walks the same number of tiles
+    const uint64_t chunk = std::min<uint64_t>(run->n_points, (uint64_t)num_cu(ctx) * FWV * 64 * 32);
     rc = flow_alloc(ctx, chunk);
     if (rc) return rc;
     FlowState &F = *ctx->flow;
