@@ -413,7 +413,7 @@ struct AdamStep {
 };
 
 // Parameter gradient of all layers: grad[l*FPL + j] = sum over r < rows of
-// parts[(l*stride + r)*FPL + j], in a fixed order (four interleaved row groups, then their sum);
+// parts[(l*stride + r)*FPLP + j], in a fixed order (four interleaved row groups, then their sum);
 // with adam.p set, also Adam's step of parameter l*FPL + j.
 __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ parts, uint32_t rows, uint32_t stride,
                                                   float *__restrict__ grad, AdamStep adam)
@@ -423,9 +423,9 @@ __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ part
     float s = 0.0f;
     if (c < (uint32_t)(FL * FPL)) {
         const uint32_t layer = c / FPL, j = c % FPL;
-        const float *q = parts + (size_t)layer * stride * FPL + j;
+        const float *q = parts + (size_t)layer * stride * FPLP + j;
 #pragma unroll 4
-        for (uint32_t r = rg; r < rows; r += 4) s += q[(size_t)r * FPL];
+        for (uint32_t r = rg; r < rows; r += 4) s += q[(size_t)r * FPLP];
     }
     red[rg][threadIdx.x & 63] = s;
     __syncthreads();
@@ -468,7 +468,7 @@ struct FlowState {
     uint64_t cap = 0;
     float *xs = nullptr;        // (FL + 1) x cap x 3: layer inputs in the normalizing direction
     float *ld = nullptr, *w = nullptr, *abar = nullptr;
-    float *gp = nullptr;        // per-block layer gradients, FL x gp_rows x FPL
+    float *gp = nullptr;        // per-block layer gradients, FL x gp_rows x FPLP
     uint32_t gp_rows = 0;
     double *hist = nullptr;     // training history (sum w, sum w^2 per step)
     int hist_cap = 0;
@@ -550,7 +550,7 @@ int flow_alloc(mi_ctx *ctx, uint64_t npts)
         (void)hipFree(F.gp);
         F.gp = nullptr;
         F.gp_rows = 0;
-        FCHK(hipMalloc(&F.gp, sizeof(float) * (size_t)FPL * FL * rows));
+        FCHK(hipMalloc(&F.gp, sizeof(float) * (size_t)FPLP * FL * rows));
         F.gp_rows = rows;
     }
     return MI_OK;

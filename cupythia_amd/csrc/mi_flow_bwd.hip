@@ -27,7 +27,7 @@ struct BwdTiles {
 };
 union BwdShared {
     BwdTiles t;
-    float rows[FWB][FPL];
+    float rows[FWB][FPLP];
 };
 static_assert(sizeof(BwdShared) <= 160 * 1024, "backward LDS");
 
@@ -201,7 +201,7 @@ struct BwdArgs {
     const double *parts;        //   per-block sums of w and w^2 (record slots kRS, kRQ)
     uint32_t nparts;
     float *abar;                // n x 3 work array (not read at the last layer, where dL/dz = 0)
-    float *gp;                  // gradient rows of layer l: gp + (l * gp_stride + blockIdx.x) * FPL
+    float *gp;                  // gradient rows of layer l: gp + (l * gp_stride + blockIdx.x) * FPLP
     uint32_t gp_stride;
     double *hist;               // optional: (sum w, sum w^2) of the batch at hist[2 step]
     int step;
@@ -505,11 +505,15 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_bwd(const BwdArgs A)
             }
         }
         __syncthreads();
-        float *dst = A.gp + ((size_t)l * A.gp_stride + blockIdx.x) * FPL;
-        for (int j = threadIdx.x; j < FPL; j += 64 * FWB) {
-            float t = S.rows[0][j];
+        // the block's row, four floats at a time (the three of the padding carry no parameter)
+        float4 *dst = reinterpret_cast<float4 *>(A.gp + ((size_t)l * A.gp_stride + blockIdx.x) * FPLP);
+        for (int j = threadIdx.x; j < FPLP / 4; j += 64 * FWB) {
+            float4 t = reinterpret_cast<const float4 *>(S.rows[0])[j];
 #pragma unroll
-            for (int w = 1; w < FWB; ++w) t += S.rows[w][j];
+            for (int w = 1; w < FWB; ++w) {
+                const float4 u = reinterpret_cast<const float4 *>(S.rows[w])[j];
+                t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+            }
             dst[j] = t;
         }
         __syncthreads();                                   // rows read before the next layer's weights replace them

@@ -19,6 +19,7 @@ namespace mi {
 
 constexpr int FL = MI_FLOW_LAYERS, FK = MI_FLOW_BINS, FH = MI_FLOW_HIDDEN, FO = MI_FLOW_NOUT;
 constexpr int FPL = MI_FLOW_NPARAMS / MI_FLOW_LAYERS;   // 7537 parameters per layer
+constexpr int FPLP = (FPL + 3) & ~3;                    // row stride of the gradient rows (16-byte aligned)
 constexpr int PW1 = 0, PB1 = 128, PW2 = 192, PB2 = 4288, PW3 = 4352, PB3 = 7488;
 static_assert(PB3 + FO == FPL, "flow parameter layout");
 constexpr float kMinW = 1e-3f, kMinH = 1e-3f, kMinD = 1e-3f;
