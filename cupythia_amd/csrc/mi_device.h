@@ -741,13 +741,20 @@ __device__ __forceinline__ float had_family_f32(const HadConst<float>& k, int ma
         w[5] = Lopp * (k.nQuarkNew * 0.44444444444444444f * q2);
         // a slot mask leaves out is 0 and adds nothing (a uniform branch, no select: the empty
         // volatile asm keeps the compiler from if-converting it)
+        // Pythia clamps a negative sigma to 0 and keeps a NaN.  The bit-exact form does so literally;
+        // in FP32_HW the clamp is left out, being a no-op here: every factor of these six weights is
+        // >= 0 (luminosities of clipped or exponential densities, the prefactor) and so is every
+        // bracket -- sums of positive terms, or 1/(6ab) - 3/8 >= 0.29 and 16/(27ab) - 4/3 >= 1.03
+        // since ab <= 1/4; for q q' -> q q' with csame < 0, Lsame <= LQQ gives w3 >= LQQ (sigT +
+        // csame) = LQQ (sigT + sigU + sigTU) / 2, the identical-quark bracket, which is at least
+        // 0.93 max(sigT, sigU) -- far beyond FP32 rounding -- so no weight is negative or -0.
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             ws[i] = 0.0f;
             if (mask & (1 << i)) {
                 if (BR) __asm__ volatile("");
                 float wi = common * w[i];
-                wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;      // negative -> 0, NaN kept
+                if (!HW) wi = (wi > 0.0f || wi != wi) ? wi : 0.0f;   // negative -> 0, NaN kept
                 ws[i] = wi;
                 tot += wi;
             }
