@@ -30,6 +30,9 @@ union BwdShared {
     float rows[FWB][FPLP];
 };
 static_assert(sizeof(BwdShared) <= 160 * 1024, "backward LDS");
+static_assert(offsetof(BwdTiles, b) == sizeof(FwdLayer), "the two weight images stage as one array");
+constexpr int kStageF = sizeof(FwdLayer) / 16, kStageB = sizeof(BwdExtra) / 16;
+constexpr int kStageIt = (kStageF + kStageB + 64 * FWB - 1) / (64 * FWB);   // 16-byte loads per work-item
 
 // 0xFFFF where the BF16 activation (ReLU output: 0 .. 0x7FFF as a 16-bit integer) is zero, per
 // half: h - 1 is negative exactly there.
@@ -259,12 +262,30 @@ __global__ __launch_bounds__(64 * FWB) void k_flow_bwd(const BwdArgs A)
 #pragma unroll 1
     for (int l = FL - 1; l >= 0; --l) {
         {
+            // The layer's two weight images (contiguous in LDS), every 16-byte load of the work-item
+            // in flight before its LDS writes: one wave per SIMD has no other wave to hide a
+            // load-then-write loop's latency.  The index is an opaque copy, so that the
+            // addresses are formed here and not held across the tiles.
+            int tid = threadIdx.x;
+            __asm__ volatile("" : "+v"(tid));
             const uint4 *s0 = reinterpret_cast<const uint4 *>(A.fl + l);
-            uint4 *d0 = reinterpret_cast<uint4 *>(&S.t.f);
-            for (int j = threadIdx.x; j < (int)(sizeof(FwdLayer) / 16); j += blockDim.x) d0[j] = s0[j];
             const uint4 *s1 = reinterpret_cast<const uint4 *>(A.bl + l);
-            uint4 *d1 = reinterpret_cast<uint4 *>(&S.t.b);
-            for (int j = threadIdx.x; j < (int)(sizeof(BwdExtra) / 16); j += blockDim.x) d1[j] = s1[j];
+            uint4 *d = reinterpret_cast<uint4 *>(&S.t.f);
+            const auto ld = [=](int k) {
+                const int j = tid + 64 * FWB * k;
+                uint4 r = make_uint4(0u, 0u, 0u, 0u);
+                if (j < kStageF + kStageB) r = j < kStageF ? s0[j] : s1[j - kStageF];
+                return r;
+            };
+            const auto st = [=](int k, uint4 r) {
+                const int j = tid + 64 * FWB * k;
+                if (j < kStageF + kStageB) d[j] = r;
+            };
+            // named values, not an array: a private array is placed in LDS before it is unrolled
+            static_assert(kStageIt == 9, "nine 16-byte loads per work-item");
+            const uint4 v0 = ld(0), v1 = ld(1), v2 = ld(2), v3 = ld(3), v4 = ld(4), v5 = ld(5), v6 = ld(6),
+                        v7 = ld(7), v8 = ld(8);
+            st(0, v0); st(1, v1); st(2, v2); st(3, v3); st(4, v4); st(5, v5); st(6, v6); st(7, v7); st(8, v8);
             __syncthreads();
         }
         const int d = l % 3, ca = cond0(d), cb = cond1(d);
