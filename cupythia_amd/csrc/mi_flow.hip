@@ -171,9 +171,18 @@ __global__ __launch_bounds__(64 * FWV) void k_flow_chain(const FwdLayer *__restr
 {
     __shared__ FwdLayer W[FL];
     {
+        // eight 16-byte loads of a work-item in flight before their LDS writes (a load-wait-write
+        // loop cost ~28 memory round trips per work-item at 4 waves per block)
+        constexpr int NV = (int)(sizeof(W) / 16);
         const uint4 *src = reinterpret_cast<const uint4 *>(layers);
         uint4 *dst = reinterpret_cast<uint4 *>(W);
-        for (int j = threadIdx.x; j < (int)(sizeof(W) / 16); j += blockDim.x) dst[j] = src[j];
+        const int nt = (int)blockDim.x;
+        for (int j0 = threadIdx.x; j0 < NV; j0 += 8 * nt) {
+            const auto ld = [=](int k) { return j0 + k * nt < NV ? src[j0 + k * nt] : make_uint4(0u, 0u, 0u, 0u); };
+            const auto st = [=](int k, uint4 v) { if (j0 + k * nt < NV) dst[j0 + k * nt] = v; };
+            const uint4 v0 = ld(0), v1 = ld(1), v2 = ld(2), v3 = ld(3), v4 = ld(4), v5 = ld(5), v6 = ld(6), v7 = ld(7);
+            st(0, v0); st(1, v1); st(2, v2); st(3, v3); st(4, v4); st(5, v5); st(6, v6); st(7, v7);
+        }
         __syncthreads();
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
